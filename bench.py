@@ -1,0 +1,231 @@
+"""Benchmark of the virtual-CBCT chain on one MI355X.
+
+Headline: forward projection rays x energy-bins per second on the full-size plain workload — a 25 cm water
+cylinder on a 512 x 512 x 128 grid (0.5 mm in-plane, 1 mm slices, six uint16 elemental maps), 360 angles,
+400 x 400 detector with a 115 mm half-fan offset, 91 energy bins (10-100 keV).  One step = one forward
+projection of all 360 angles x 160 000 rays x 91 bins with the volume resident in HBM.
+
+Also reported on the same line: FDK (filter + backprojection to 270 x 270 x 128) voxel updates per second,
+adjoint gradients per second on the differentiable workload (256^2 x 64 FP32 maps, 90 angles, 256^2
+detector), the roofline of the dominant kernel (algorithmic bytes from the per-ray segment counts), and the
+reference's own CPU projector timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "forward rays*bins/s, FDK GUPS, adjoint grads/s vs plain GPU at <=1e-4 rel err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_c2(td):
+    from virtualcbct_amd import io, phantoms
+    w = phantoms.WORKLOADS["c2"]
+    phantoms.write_xraydb(f"{td}/db")
+    phantoms.write_spectrum_csv(f"{td}/spec100.csv", kvp=100)
+    atten, pdf = io.attenuation_coefficient(f"{td}/spec100.csv", f"{td}/db")
+    vol = phantoms.water_cylinder(w["N"], w["Nz"], w["s"], w["radius"])
+    ang, xs, ys = phantoms.flexmap_angles(w["n_ang"], w["xshift_proj"])
+    cs, sn, xc, yc = io.trig_table(w["W"], w["H"], ang, xs, ys)
+    fm = f"{td}/fdk.flexmap"
+    phantoms.write_flexmap(fm, w["n_ang"], w["xshift_fdk"])
+    a_fdk, xs_fdk, ys_fdk = io.read_fdk_angles(fm, w["n_ang"])
+    return dict(w=w, atten=atten, pdf=pdf, vol=vol, ang=ang, cs=cs, sn=sn, xc=xc, yc=yc, a_fdk=a_fdk, xs_fdk=xs_fdk,
+                ys_fdk=ys_fdk)
+
+
+def read_traffic():
+    """HBM bytes per launch of the forward kernel from the committed rocprofv3 PMC summary, if any."""
+    p = os.path.join(ROOT, "profiles", "forward_c2_traffic.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def cpu_baseline(inp, seconds_target=15.0):
+    """The reference's CPU projector (P/reprojection.cpp, built from its sources into oracle/_ref) on a bounded
+    sample of (angle, bin) pairs of the same workload, OpenMP over pairs (the function itself is single-threaded
+    per call).  Falls back to the FP64 oracle when the reference build is absent."""
+    from oracle import oracle as O
+    w = inp["w"]
+    nthreads = min(16, os.cpu_count() or 1)
+    g = O.Geometry(N=w["N"], s=w["s"], Nz=w["Nz"], t=w["t"], H=w["H"], W=w["W"], cos_t=inp["cs"], sin_t=inp["sn"],
+                   xc=inp["xc"], yc=inp["yc"])
+    rng = np.random.default_rng(0)
+    if O.ref_available():
+        # calibrate with one pair per thread, then size the sample to ~seconds_target
+        pairs = [(int(a), int(k)) for a, k in zip(rng.integers(0, w["n_ang"], nthreads), rng.integers(0, 91, nthreads))]
+        sec, _ = O.ref_reprojection_timed(g, inp["vol"], inp["ang"], 1000.0, inp["atten"], inp["pdf"], pairs, nthreads)
+        n = max(nthreads, int(nthreads * seconds_target / max(sec, 1e-3)) // nthreads * nthreads)
+        n = min(n, 8 * nthreads)
+        pairs = [(int(a), int(k)) for a, k in zip(rng.integers(0, w["n_ang"], n), rng.integers(0, 91, n))]
+        sec, _ = O.ref_reprojection_timed(g, inp["vol"], inp["ang"], 1000.0, inp["atten"], inp["pdf"], pairs, nthreads)
+        rate = n * w["W"] * w["H"] / sec
+        return {"value": rate, "unit": "rays*bins/s", "cores": nthreads, "kind": "reference",
+                "sample": f"{n} (angle, bin) pairs of the 360x91 workload, each a full 400x400 detector "
+                          f"(P/reprojection.cpp CPU twin, one call per pair), {sec:.1f} s"}
+    # oracle: one traversal per ray then 91 bins; time a slab of rays of one angle
+    sub = O.Geometry(N=w["N"], s=w["s"], Nz=w["Nz"], t=w["t"], H=w["H"], W=w["W"], cos_t=inp["cs"][:1],
+                     sin_t=inp["sn"][:1], xc=inp["xc"][:1], yc=inp["yc"][:1])
+    t0 = time.perf_counter()
+    O.forward_collapsed(sub, inp["vol"], inp["atten"], inp["pdf"], 1000.0)
+    sec = time.perf_counter() - t0
+    return {"value": w["W"] * w["H"] * 91 / sec, "unit": "rays*bins/s", "cores": nthreads, "kind": "port",
+            "sample": f"1 angle x 160000 rays x 91 bins with the FP64 oracle, {sec:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="headline forward only (used under the profiler)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from virtualcbct_amd import fdk
+    from virtualcbct_amd.projector import Projector
+
+    dev = torch.device("cuda:0")
+    td = tempfile.mkdtemp(prefix="vcbct_bench_")
+    log("building synthetic c2 inputs ...")
+    inp = build_c2(td)
+    w = inp["w"]
+    P = Projector(w["N"], w["s"], w["Nz"], w["t"], w["H"], w["W"], w["n_ang"], inp["cs"], inp["sn"], inp["xc"],
+                  inp["yc"])
+    P.set_volume(inp["vol"])
+    P.set_spectrum(inp["atten"], inp["pdf"])
+    out = torch.empty(P.shape_proj, dtype=torch.float32, device=dev)
+    NE = len(inp["pdf"])
+    stream = torch.cuda.current_stream()
+
+    log(f"forward warmup x{args.warmup} ...")
+    for _ in range(args.warmup):
+        P.forward(1000.0, out=out)
+    torch.cuda.synchronize()
+    # timed region: K forward projections, HIP events around each launch on the launch stream
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        evs[i][0].record(stream)
+        P.forward(1000.0, out=out)
+        evs[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [a.elapsed_time(b) for a, b in evs]
+    ms_step = wall * 1e3 / args.steps
+    rays = w["n_ang"] * w["H"] * w["W"]
+    value = rays * NE / (wall / args.steps)
+    log(f"forward: {ms_step:.3f} ms/step (kernel avg {np.mean(kms):.3f} ms), {value:.4e} rays*bins/s")
+
+    # roofline of the forward kernel: 12 B of voxel data per charged segment + 4 B output per ray
+    segs = float(P.segment_counts().double().sum().item())
+    torch.cuda.synchronize()
+    alg_bytes = 12.0 * segs + 4.0 * rays
+    k_avg_s = float(np.mean(kms)) * 1e-3
+    achieved = alg_bytes / k_avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
+                "kernel": "k_forward<0, uint4>", "segments_per_launch": segs,
+                "algorithmic_bytes_per_launch": alg_bytes}
+    line = {"metric": METRIC, "value": value, "unit": "rays*bins/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "25 cm water cylinder, 512x512x128 voxels (0.5 mm x 0.5 mm x 1 mm), 6 uint16 "
+                                   "elemental maps, 360 angles, 400x400 detector, 115 mm half-fan offset, "
+                                   "91 energy bins 10-100 keV, FP32 projections"},
+            "roofline": roofline}
+
+    if not args.no_extras:
+        # FDK on the projections just computed
+        proj = out
+        filt = torch.empty_like(proj)
+        rec = torch.zeros((w["Z"], w["R"], w["R"]), dtype=torch.float32, device=dev)
+        fdk.filter_sinogram_device(proj, inp["xs_fdk"], inp["ys_fdk"], out=filt)
+        fdk.reconstruct_image_device(rec, filt, w["zstart"], w["zstep"], w["rscale"], 1000, inp["a_fdk"],
+                                     inp["xs_fdk"])
+        torch.cuda.synchronize()
+        nrep = 5
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        tf, tb = [], []
+        for _ in range(nrep):
+            rec.zero_()
+            e[0].record(stream)
+            fdk.filter_sinogram_device(proj, inp["xs_fdk"], inp["ys_fdk"], out=filt)
+            e[1].record(stream)
+            fdk.reconstruct_image_device(rec, filt, w["zstart"], w["zstep"], w["rscale"], 1000, inp["a_fdk"],
+                                         inp["xs_fdk"])
+            e[2].record(stream)
+            torch.cuda.synchronize()
+            tf.append(e[0].elapsed_time(e[1]))
+            tb.append(e[1].elapsed_time(e[2]))
+        updates = w["R"] * w["R"] * w["Z"] * w["n_ang"]
+        line["fdk"] = {"gups": updates / (np.median(tb) * 1e-3) / 1e9, "bp_ms": float(np.median(tb)),
+                       "filter_ms": float(np.median(tf)),
+                       "filter_rows_per_s": w["n_ang"] * w["H"] / (np.median(tf) * 1e-3),
+                       "updates": updates, "shape": [w["Z"], w["R"], w["R"]],
+                       "bp_roofline_frac": (16.0 * updates / (np.median(tb) * 1e-3) / 1e9) / HBM_PEAK_GBS}
+        log(f"fdk: filter {np.median(tf):.3f} ms, backprojection {np.median(tb):.3f} ms, "
+            f"{line['fdk']['gups']:.1f} GUPS")
+        del filt, rec
+        # adjoint on the differentiable workload (256^2 x 64 FP32 maps, 90 angles, 256^2 detector)
+        from virtualcbct_amd import phantoms, io
+        N4, Z4, n4, D4 = 256, 64, 90, 256
+        ang4 = np.deg2rad(-180.0 + 4.0 * np.arange(n4) + 1e-4)
+        c4, s4, x4, y4 = io.trig_table(D4, D4, ang4, np.zeros(n4), np.zeros(n4))
+        P4 = Projector(N4, 0.1, Z4, 0.1, D4, D4, n4, c4, s4, x4, y4)
+        maps = torch.from_numpy(phantoms.water_cylinder_f32(N4, Z4, 0.1, 10.0)).to(dev).contiguous()
+        P4.set_volume(maps)
+        P4.set_spectrum(inp["atten"], inp["pdf"])
+        tgt = P4.logproj(1.0) * 1.05
+        P4.adjoint(tgt, 1.0)
+        torch.cuda.synchronize()
+        ta = []
+        for _ in range(5):
+            e[0].record(stream)
+            grad, loss = P4.adjoint(tgt, 1.0)
+            e[1].record(stream)
+            torch.cuda.synchronize()
+            ta.append(e[0].elapsed_time(e[1]))
+        segs4 = float(P4.segment_counts().double().sum().item())
+        rays4 = n4 * D4 * D4
+        adj_bytes = 48.0 * segs4 + 8.0 * rays4
+        line["adjoint"] = {"grads_per_s": 1e3 / float(np.median(ta)), "ms": float(np.median(ta)),
+                           "workload": "256^2 x 64 FP32 maps (1 mm), 90 angles, 256^2 detector, 91 bins",
+                           "roofline_frac": adj_bytes / (np.median(ta) * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log(f"adjoint: {np.median(ta):.3f} ms/gradient")
+        P4.close()
+
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(inp)
+        log(f"cpu baseline: {line['cpu_baseline']['value']:.4e} rays*bins/s on {line['cpu_baseline']['cores']} threads")
+    P.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,222 @@
+/*
+ * vcbct — MI355X-native virtual-CBCT chain: material-based polychromatic forward projector (MBFPA)
+ * and FDK reconstruction, behind a plain C ABI (extern "C", plain pointers and sizes).
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the reference
+ * repository; P/ = Release_CBCT_Projection_CM, F/ = Release_recon_from_forward_pro_float_CM).
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative VCBCT_E* code on failure; the message of the
+ *    last failure on the calling thread is vcbct_last_error().  Nothing calls exit().
+ *  - "dev_" pointers are device (HBM) pointers; "host_" pointers are host memory.  `stream` is a
+ *    hipStream_t passed as void* (NULL = the null stream).  Device entry points are asynchronous on
+ *    that stream; host entry points synchronise before returning.
+ *  - Array layouts are the reference's: volumes [z][mat][y][x] uint16 (g/cc x 1000, material order
+ *    H, C, N, O, P, Ca), projections [angle][row][col] float32 with the reference's row flip,
+ *    reconstructions [z][y][x].
+ */
+#ifndef VCBCT_H
+#define VCBCT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VCBCT_OK 0
+#define VCBCT_EINVAL -1   /* bad argument or unsupported size */
+#define VCBCT_EHIP -2     /* HIP runtime error */
+#define VCBCT_ENOMEM -3   /* device or host allocation failed */
+#define VCBCT_EIO -4      /* file missing, unreadable or malformed */
+#define VCBCT_ESTATE -5   /* call out of order (e.g. forward before a volume was set) */
+
+#define VCBCT_MAX_MAT 6
+#define VCBCT_MAX_NE 1024
+
+const char* vcbct_last_error(void);
+/* Library version string and the offload target it was built for. */
+const char* vcbct_version(void);
+
+/* ------------------------------------------------------------------------------------------------
+ * Forward projector.
+ *
+ * The argument block of initializeGPU (P/reprojectionOnGPU.h:76-82 and P/reprojectionOnGPU.cu:720-757),
+ * field for field.
+ */
+typedef struct vcbct_geometry {
+    int recon_size;          /* reconSize N (volume is N x N in x, y) */
+    double recon_scale;      /* reconScale, x/y voxel pitch [cm] */
+    int recon_slices;        /* reconSlices (the z_required_range slices actually loaded) */
+    double recon_step;       /* reconStep, z voxel pitch [cm] */
+    int proj_height;         /* projImgHeight */
+    int proj_width;          /* projImgWidth */
+    int n_angles;            /* totalProjNumber */
+    double x_reg, y_reg, z_reg;
+    double det_pitch_cm;     /* DETECTOR_PITCH_CM */
+    double det_pitch_iso_cm; /* DETECTOR_PITCH_CM_AT_ISO_CENT */
+    double sad_cm;           /* DIST_BTWN_SRC_AND_ISOCENT_CM */
+    double det_pixel_num;    /* DETECTOR_PIXEL_NUM */
+    int num_material;        /* 1..6 */
+} vcbct_geometry;
+
+typedef struct vcbct_projector vcbct_projector;
+
+/* Replaces initializeGPU (P/reprojectionOnGPU.cu:720-825): geometry and the per-angle tables
+ * (cos, sin, X/Y centre shift from prepareTrigonometricTable, P/misc.cpp:9-37) move to the device.
+ * Unlike the reference there is no 360-angle limit and no hard-coded device: `device` selects it. */
+int vcbct_projector_create(vcbct_projector** out, const vcbct_geometry* g, const double* cos_table,
+                           const double* sin_table, const double* xcenter_shift, const double* ycenter_shift,
+                           int device);
+/* Replaces terminateGPU (P/reprojectionOnGPU.cu:827-834). */
+void vcbct_projector_destroy(vcbct_projector* p);
+
+/* Volume upload, the first step of reprojectionOnGPU (P/reprojectionOnGPU.cu:867).  The volume is the
+ * reference's uint16 [z][mat][y][x]; it is re-laid out on the device to [y][x][z][8 x u16] (one 16-byte
+ * voxel holds all materials).  host_vol may be a host pointer (is_device = 0) or a device pointer. */
+int vcbct_projector_set_volume_u16(vcbct_projector* p, const uint16_t* vol, int is_device, void* stream);
+/* FP32 elemental maps, device pointer, layout [mat][z][y][x] (the differentiable path's maps). */
+int vcbct_projector_set_volume_f32(vcbct_projector* p, const float* dev_maps, void* stream);
+
+/* Spectrum upload (P/reprojectionOnGPU.cu:873-882): atten [mat][NE] (cm^2/g, as written by
+ * attenuation_coefficient), pdf [NE]. */
+int vcbct_projector_set_spectrum(vcbct_projector* p, const float* host_atten, const float* host_pdf, int NE);
+
+/* The compute of reprojectionOnGPU (P/reprojectionOnGPU.cu:1004-1056), energy-collapsed: one traversal
+ * per ray yields the material line integrals, then the in-kernel spectral sum
+ *   I = sum_e float(pdf_e * exp(-sum_m mu_me L_m / xfactor) * leng)   (FP32 accumulation, e ascending)
+ * is written to dev_out [n_angles][H][W] (row H-1-j for kernel row j, as the reference).
+ * Angles [a0, a1) are computed (a1 = -1: all). */
+int vcbct_projector_forward(vcbct_projector* p, float* dev_out, double xfactor, int a0, int a1, void* stream);
+
+/* Six line integrals per ray, dev_L [n_angles][H][W][num_material] FP32 (for checks and tools). */
+int vcbct_projector_line_integrals(vcbct_projector* p, float* dev_L, void* stream);
+
+/* Number of charged segments (in range, inside the inscribed circle) per ray, dev_counts [n_angles][H][W]
+ * FP32: the work unit of the roofline model (12 bytes of voxel data per segment). */
+int vcbct_projector_segment_counts(vcbct_projector* p, float* dev_counts, void* stream);
+
+/* Host drop-in for reprojectionOnGPU: uploads host_vol/atten/pdf, runs the forward projection and
+ * copies the result to host_out [n_angles][H][W]. */
+int vcbct_reprojection(vcbct_projector* p, const uint16_t* host_vol, const float* host_atten,
+                       const float* host_pdf, int NE, double xfactor, float* host_out);
+
+/* ------------------------------------------------------------------------------------------------
+ * Differentiable form (no reference counterpart; the reference's scatter U-Net, Unet_scatter.py, is the
+ * intended consumer).  Log projection p = -log( sum_e pdf_e exp(-sum_m mu_me L_m/xf) / sum_e pdf_e ).
+ */
+int vcbct_projector_logproj(vcbct_projector* p, float* dev_p, double xfactor, void* stream);
+/* Adjoint: dev_grad [mat][z][y][x] FP32 (overwritten) receives d/drho of
+ *   loss = sum_rays w_ray (p - target)^2  with w_ray = 1,
+ * by recomputing the per-ray line integrals (none stored) and scattering 2(p-t)*mubar_m/xf * rr along
+ * the same traversal (FP32 atomics).  dev_target [n_angles][H][W].  dev_loss (optional, 1 double)
+ * receives the loss.  Uses the volume set with vcbct_projector_set_volume_f32. */
+int vcbct_projector_adjoint(vcbct_projector* p, const float* dev_target, float* dev_grad, double* dev_loss,
+                            double xfactor, void* stream);
+/* Backward of logproj for an arbitrary upstream gradient dev_gp = dL/dp [n_angles][H][W]:
+ * dev_grad [mat][z][y][x] (overwritten) = sum_rays gp * mubar_m/xf * rr. */
+int vcbct_projector_logproj_backward(vcbct_projector* p, const float* dev_gp, float* dev_grad, double xfactor,
+                                     void* stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * FDK.
+ *
+ * Replaces prepareConstMap (F/filterSinogram.cpp:65-120): the W x W filter matrix h(m-n), FP64,
+ * host output [W][W]. */
+int vcbct_prepare_const_map(int proj_width, double det_pitch_iso_cm, double* host_const_map);
+
+/* Replaces filterSinogram (F/filterSinogram.cpp:122-186): q = -log(p), convolution with the reference's
+ * truncated kernel (centre tap coeff, off-centre h(m-k), |m-k| <= W-1) done as a batched FFT
+ * convolution, then the cosine weight Cm at the output pixel.  Device [n][H][W] in and out.
+ * xshift/yshift are host arrays of n_proj entries. */
+int vcbct_filter_sinogram(int proj_height, int proj_width, int n_proj, const float* dev_proj, float* dev_filtered,
+                          const double* host_xshift, const double* host_yshift, double det_pitch_iso_cm,
+                          void* stream);
+
+/* Replaces reconstructImage (F/reconstructImage.cpp:10-184): voxel-driven cone-beam backprojection
+ * with the reference's interpolation rule, angle weights and edge cases; accumulates (+=) into
+ * dev_recon [Z][R][R] FP32 (inside the (R-1)/2 circle mask).  Angles/xshift are host arrays. */
+int vcbct_reconstruct_image(int recon_size, int recon_slices, int zstart_cm, double recon_step,
+                            double recon_scale, float* dev_recon, int proj_height, int proj_width, int n_proj,
+                            const float* dev_filtered, int xfactor, const double* host_angles_rad,
+                            const double* host_xshift, double det_pitch_iso_cm, void* stream);
+
+/* Host-array forms of the two FDK calls for drop-in callers (F/main.cpp:133-137, 170-175): inputs are the
+ * contiguous new3DArray blocks; reconstruct_image_host accumulates (+=) into the caller's FP64 volume. */
+int vcbct_filter_sinogram_host(int proj_height, int proj_width, int n_proj, const float* host_proj,
+                               float* host_filtered, const double* host_xshift, const double* host_yshift,
+                               double det_pitch_iso_cm);
+int vcbct_reconstruct_image_host(int recon_size, int recon_slices, int zstart_cm, double recon_step,
+                                 double recon_scale, double* host_recon, int proj_height, int proj_width, int n_proj,
+                                 const float* host_filtered, int xfactor, const double* host_angles_rad,
+                                 const double* host_xshift, double det_pitch_iso_cm);
+
+/* ------------------------------------------------------------------------------------------------
+ * Host-side formats kept byte-compatible with the reference.
+ *
+ * attenuation_coefficient (P/prior_weight_production.cpp:200-523): spectrum CSV + the three xraydb
+ * CSVs under xraydb_dir -> atten [mat][NE], pdf [NE].  Returns NE (> 0) or a negative error. */
+int vcbct_attenuation_table(const char* spectrum_csv, const char* xraydb_dir, int num_material, int nthin,
+                            float* host_atten, float* host_pdf, int capacity);
+
+/* Projector input txt (P/projectionData.cpp:63-102, 15 fields after a 6-line header; format in
+ * P/txtfile/IR_waterphantom_36pro.txt).  Geometry constants come from physParams.h (P/physParams.h:47-51). */
+typedef struct vcbct_proj_config {
+    char spectrum[256];
+    int total_proj, proj_width, proj_height;
+    char flexmap[256];
+    char mat_prefix[256];
+    int zic;              /* IC z position (slice number) */
+    char outputname[256];
+    double x_scale, y_scale, z_scale;
+    int recon_size, recon_slices, mat_num;
+    char mat_type[16];
+    double xfactor;
+    double x_reg, y_reg, z_reg;
+    double det_pitch_cm, det_pitch_iso_cm, sad_cm, det_pixel_num, sdd_iso_cm;
+} vcbct_proj_config;
+int vcbct_read_proj_config(const char* txt_path, vcbct_proj_config* out);
+/* Flexmap rows "angle_deg xshift yshift class" -> radians (P/projectionData.cpp:152-171).  Returns the
+ * number of rows read (at most capacity). */
+int vcbct_read_flexmap(const char* path, double* angles_rad, double* xshift, double* yshift, int capacity);
+/* Slice window inside the cone (P/projectionData.cpp:173-185): [linemin, linemax). */
+int vcbct_proj_slice_window(const vcbct_proj_config* c, int* linemin, int* linemax);
+/* Stacks <prefix>_%03d.raw slices [linemin, linemax) into vol [z][mat][y][x] (P/projectionData.cpp:190-215).
+ * missing_policy 0: a missing or short file is an error (VCBCT_EIO).  1: the reference's behaviour — the
+ * previous slice's pixels are kept for it (P/projectionData.cpp:201, 246-259) and *n_missing counts it. */
+int vcbct_load_material_slices(const char* prefix, int linemin, int linemax, int recon_size, int num_material,
+                               int missing_policy, uint16_t* vol, int* n_missing);
+
+/* FDK input txt (F/projectionDataMVkV.cpp:51-97, 18 fields; reconOffset and xfactor are integers). */
+typedef struct vcbct_fdk_config {
+    char kvmv[16];
+    int total_proj, proj_width, proj_height, header_bytes;
+    char proj_path[256];
+    int num_digits;
+    char ext[16];
+    char flexmap[256];
+    int recon_size;
+    char tgt_class[16];
+    int recon_offset_cm;
+    double recon_step;
+    int recon_slices;
+    double recon_scale;
+    int xfactor;
+    char output[256];
+    double det_pitch_iso_cm;
+} vcbct_fdk_config;
+int vcbct_read_fdk_config(const char* txt_path, vcbct_fdk_config* out);
+/* FDK angle list (F/projectionDataMVkV.cpp:140-274): class filter, reversal when angle[row 1] - angle[row 30]
+ * < 0.  Returns the number of projections used (= total, the txt count) or a negative error. */
+int vcbct_read_fdk_angles(const char* path, const char* tgt_class, int total, double* angles_rad, double* xshift,
+                          double* yshift);
+
+/* prepareTrigonometricTable (P/misc.cpp:9-37). */
+int vcbct_trig_table(int proj_width, int proj_height, int n, const double* angles_rad, const double* xshift,
+                     const double* yshift, double det_pitch_cm, double det_pixel_num, double* cos_t,
+                     double* sin_t, double* xcenter_shift, double* ycenter_shift);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

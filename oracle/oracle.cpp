@@ -1,0 +1,751 @@
+/*
+ * FP64 CPU oracle for the virtual-CBCT chain.  TEST INFRASTRUCTURE ONLY: the product library never
+ * links or calls this file; tests, the smoke check and the benchmark's CPU-baseline leg load it as the
+ * checker.  Each function cites the reference routine it restates (paths relative to the reference
+ * repository; P/ = Release_CBCT_Projection_CM, F/ = Release_recon_from_forward_pro_float_CM).
+ *
+ * Build: oracle/Makefile (g++ -O2 -ffp-contract=off, no fast-math) so that FP64 expressions written in
+ * the reference's operation order round exactly as the reference's CPU code does.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+const double kPI = 3.141592653589793238462643383279;
+
+/* One charged segment of the compat traversal. */
+struct Seg {
+    int xmc, ymc, zmc;
+    double rr;
+    double ph; /* 1, or 0 when the midpoint radius is >= rmax */
+};
+
+struct Ray {
+    double sx, sy, sz, px, py, pz, leng;
+};
+
+/* Ray set-up: P/reprojectionOnGPU.cu:128-150.  j = detector row index of the kernel (before the
+ * output flip), jj = column, a = angle. */
+Ray make_ray(const orc_geom& g, const double* cos_t, const double* sin_t, const double* xc,
+             const double* yc, int a, int j, int jj) {
+    Ray r;
+    r.sx = -g.sad * sin_t[a] + g.x_reg;
+    r.sy = g.sad * cos_t[a] + g.y_reg;
+    r.sz = g.z_reg;
+    r.pz = -(yc[a] + j) * g.det_pitch_iso * (g.det_pixel_num / g.H) + g.z_reg * g.det_pitch_iso / g.det_pitch;
+    double pr = g.det_pitch_iso * (xc[a] + jj) * (g.det_pixel_num / g.H);
+    r.px = pr * cos_t[a] + g.x_reg * g.det_pitch_iso / g.det_pitch;
+    r.py = pr * sin_t[a] + g.y_reg * g.det_pitch_iso / g.det_pitch;
+    double leng = g.sad * g.sad / (g.sad * g.sad + (pr * pr + r.pz * r.pz));
+    r.leng = leng * sqrt(leng);
+    return r;
+}
+
+/* Literal restatement of the segment enumeration of reprojectionKernel
+ * (P/reprojectionOnGPU.cu:154-429 for |dx|>|dy|, :441-666 otherwise).  The five charge sites of each
+ * branch are folded into charge(); the operation order of every FP64 expression is the reference's.
+ * Behaviour the reference leaves undefined is defined here: a ray with fewer than two box hits has no
+ * segments, and a NaN midpoint is not charged. */
+struct Tracer {
+    const orc_geom& g;
+    int nxm, nym, nzm;
+    double rmax;
+    std::vector<Seg>* out;
+
+    explicit Tracer(const orc_geom& geom) : g(geom) {
+        nxm = (int)(g.N * 0.5);
+        nym = (int)(g.N * 0.5);
+        nzm = (int)(g.Nz * 0.5);
+        rmax = g.N * 0.5 * g.s;
+        out = nullptr;
+    }
+
+    /* P/reprojectionOnGPU.cu:255-269 (one of the 14 copies) */
+    void charge(double xb0, double yb0, double zb0, double xb1, double yb1, double zb1) {
+        double rr = sqrt((xb0 - xb1) * (xb0 - xb1) + (yb0 - yb1) * (yb0 - yb1) + (zb0 - zb1) * (zb0 - zb1));
+        double fx = (xb0 + xb1) * 0.5 / g.s + nxm;
+        double fy = (-yb0 - yb1) * 0.5 / g.s + nym;
+        double fz = (-zb0 - zb1) * 0.5 / g.t + nzm;
+        if (!(fabs(fx) < 1e9 && fabs(fy) < 1e9 && fabs(fz) < 1e9)) return; /* NaN/inf guard */
+        int xmc = (int)fx, ymc = (int)fy, zmc = (int)fz;
+        if (xmc >= 0 && ymc >= 0 && zmc >= 0 && xmc <= g.N - 1 && ymc <= g.N - 1 && zmc <= g.Nz - 1) {
+            double r = sqrt((xb0 + xb1) * (xb0 + xb1) + (-yb0 - yb1) * (-yb0 - yb1)) * 0.5;
+            Seg sg;
+            sg.xmc = xmc;
+            sg.ymc = ymc;
+            sg.zmc = zmc;
+            sg.rr = rr;
+            sg.ph = (r >= rmax) ? 0.0 : 1.0;
+            out->push_back(sg);
+        }
+    }
+
+    void trace(const Ray& R, std::vector<Seg>& segs) {
+        out = &segs;
+        segs.clear();
+        const double sx = R.sx, sy = R.sy, sz = R.sz, px = R.px, py = R.py, pz = R.pz;
+        double xx[4], yy[4], zz[4];
+        int m = 0;
+        yy[m] = double(nym) * g.s;
+        if (fabs((yy[m] - sy) / (py - sy) * (px - sx) + sx) < double(nxm) * g.s) {
+            xx[m] = (yy[m] - sy) / (py - sy) * (px - sx) + sx;
+            zz[m] = (yy[m] - sy) / (py - sy) * (pz - sz) + sz;
+            m++;
+        }
+        yy[m] = -double(nym) * g.s;
+        if (fabs((yy[m] - sy) / (py - sy) * (px - sx) + sx) < double(nxm) * g.s) {
+            xx[m] = (yy[m] - sy) / (py - sy) * (px - sx) + sx;
+            zz[m] = (yy[m] - sy) / (py - sy) * (pz - sz) + sz;
+            m++;
+        }
+        if (m < 4) {
+            xx[m] = double(nxm) * g.s;
+            if (fabs((xx[m] - sx) / (px - sx) * (py - sy) + sy) < double(nym) * g.s) {
+                yy[m] = (xx[m] - sx) / (px - sx) * (py - sy) + sy;
+                zz[m] = (xx[m] - sx) / (px - sx) * (pz - sz) + sz;
+                m++;
+            }
+        }
+        if (m < 4) {
+            xx[m] = -double(nxm) * g.s;
+            if (fabs((xx[m] - sx) / (px - sx) * (py - sy) + sy) < double(nym) * g.s) {
+                yy[m] = (xx[m] - sx) / (px - sx) * (py - sy) + sy;
+                zz[m] = (xx[m] - sx) / (px - sx) * (pz - sz) + sz;
+                m++;
+            }
+        }
+        if (m < 2) return; /* undefined in the reference (uninitialised hits); defined as a miss */
+        double xi, yi, zi, xf, yf, zf;
+        if ((xx[0] - sx) * (xx[0] - sx) + (yy[0] - sy) * (yy[0] - sy) + (zz[0] - sz) * (zz[0] - sz) <
+            (xx[1] - sx) * (xx[1] - sx) + (yy[1] - sy) * (yy[1] - sy) + (zz[1] - sz) * (zz[1] - sz)) {
+            xi = xx[0]; yi = yy[0]; zi = zz[0]; xf = xx[1]; yf = yy[1]; zf = zz[1];
+        } else {
+            xi = xx[1]; yi = yy[1]; zi = zz[1]; xf = xx[0]; yf = yy[0]; zf = zz[0];
+        }
+        double xb0 = xi, yb0 = yi, zb0 = zi;
+        double xii, yii, zii, xiii = 0, yiii = 0, ziii = 0, xiiii = 0, yiiii = 0, ziiii = 0, yj, xj, zj;
+        int xma = 0, yma = 0, zma = 0, xmb, ymb, zmb, icheck;
+        const double lim_x = double(nxm) * g.s, lim_y = double(nym) * g.s;
+        const long max_steps = 4L * (g.N + 8) + 64; /* the reference's loop is bounded by the box */
+
+        if (fabs(xi - xf) > fabs(yi - yf)) {
+            double pha = (xf - xi) / fabs(xi - xf);
+            int xin, ik = 0;
+            double chy, chz;
+            if ((xi < 0 && (xf - xi) < 0) || (xi > 0 && (xf - xi) > 0)) xin = int(xi) + (xf - xi) / fabs(xf - xi);
+            else xin = int(xi);
+            xii = xin;
+            yii = (xii - sx) / (px - sx) * (py - sy) + sy;
+            zii = (xii - sx) / (px - sx) * (pz - sz) + sz;
+            while (xii <= lim_x && xii >= -lim_x && yii <= lim_y && yii >= -lim_y && ik < max_steps) {
+                xii = xin + pha * ik * g.s;
+                ymb = yma;
+                yj = yii;
+                yii = (xii - sx) / (px - sx) * (py - sy) + sy;
+                yma = (int)(yii / g.s);
+                zmb = zma;
+                zj = zii;
+                zii = (xii - sx) / (px - sx) * (pz - sz) + sz;
+                zma = (int)(zii / g.t);
+                icheck = 0;
+                chy = 0;
+                chz = 0;
+                if ((ik != 0 && yma != ymb) || (ik != 0 && (yii / fabs(yii) != yj / fabs(yj)))) {
+                    if ((yii <= 0 && (yf - yi) < 0) || (yii >= 0 && (yf - yi) > 0)) yiii = (double)(yma * g.s);
+                    else yiii = (double)(yma * g.s) - (yf - yi) / fabs(yf - yi) * g.s;
+                    xiii = (yiii - sy) / (py - sy) * (px - sx) + sx;
+                    ziii = (yiii - sy) / (py - sy) * (pz - sz) + sz;
+                    chy = (xii - xiii) * (xii - xiii) + (yii - yiii) * (yii - yiii) + (zii - ziii) * (zii - ziii);
+                    icheck++;
+                }
+                if (ik != 0 && zma != zmb) {
+                    if ((zii < 0 && (zf - zi) < 0) || (zii > 0 && (zf - zi) > 0)) ziiii = (double)(zma * g.t);
+                    else ziiii = (double)(zma * g.t) - (zf - zi) / fabs(zf - zi) * g.t;
+                    xiiii = (ziiii - sz) / (pz - sz) * (px - sx) + sx;
+                    yiiii = (ziiii - sz) / (pz - sz) * (py - sy) + sy;
+                    chz = (xii - xiiii) * (xii - xiiii) + (yii - yiiii) * (yii - yiiii) + (zii - ziiii) * (zii - ziiii);
+                    icheck++;
+                }
+                step_tail(icheck, chy, chz, xb0, yb0, zb0, xiii, yiii, ziii, xiiii, yiiii, ziiii, xii, yii, zii);
+                ik++;
+            }
+        } else {
+            double pha = (yf - yi) / fabs(yi - yf);
+            int yin, ik = 0;
+            double chy, chz;
+            if ((yi < 0 && (yf - yi) < 0) || (yi > 0 && (yf - yi) > 0)) yin = int(yi) + (yf - yi) / fabs(yf - yi);
+            else yin = int(yi);
+            yii = yin;
+            xii = (yii - sy) / (py - sy) * (px - sx) + sx;
+            zii = (yii - sy) / (py - sy) * (pz - sz) + sz;
+            while (xii <= lim_x && xii >= -lim_x && yii <= lim_y && yii >= -lim_y && ik < max_steps) {
+                yii = yin + pha * ik * g.s;
+                xmb = xma;
+                xj = xii;
+                xii = (yii - sy) / (py - sy) * (px - sx) + sx;
+                xma = (int)(xii / g.s);
+                zmb = zma;
+                zj = zii;
+                zii = (yii - sy) / (py - sy) * (pz - sz) + sz;
+                zma = (int)(zii / g.t);
+                icheck = 0;
+                chy = 0;
+                chz = 0;
+                if ((ik != 0 && xma != xmb) || (ik != 0 && (xii / fabs(xii) != xj / fabs(xj)))) {
+                    if ((xii <= 0 && (xf - xi) < 0) || (xii >= 0 && (xf - xi) > 0)) xiii = (double)(xma * g.s);
+                    else xiii = (double)(xma * g.s) - (xf - xi) / fabs(xf - xi) * g.s;
+                    yiii = (xiii - sx) / (px - sx) * (py - sy) + sy;
+                    ziii = (xiii - sx) / (px - sx) * (pz - sz) + sz;
+                    chy = (xii - xiii) * (xii - xiii) + (yii - yiii) * (yii - yiii) + (zii - ziii) * (zii - ziii);
+                    icheck++;
+                }
+                if (ik != 0 && zma != zmb) {
+                    if ((zii < 0 && (zf - zi) < 0) || (zii > 0 && (zf - zi) > 0)) ziiii = (double)(zma * g.t);
+                    else ziiii = (double)(zma * g.t) - (zf - zi) / fabs(zf - zi) * g.t;
+                    xiiii = (ziiii - sz) / (pz - sz) * (px - sx) + sx;
+                    yiiii = (ziiii - sz) / (pz - sz) * (py - sy) + sy;
+                    chz = (xii - xiiii) * (xii - xiiii) + (yii - yiiii) * (yii - yiiii) + (zii - ziiii) * (zii - ziiii);
+                    icheck++;
+                }
+                step_tail(icheck, chy, chz, xb0, yb0, zb0, xiii, yiii, ziii, xiiii, yiiii, ziiii, xii, yii, zii);
+                ik++;
+            }
+        }
+    }
+
+    /* The segment emission of one step (P/reprojectionOnGPU.cu:249-427, identical in both branches). */
+    void step_tail(int icheck, double chy, double chz, double& xb0, double& yb0, double& zb0, double xiii,
+                   double yiii, double ziii, double xiiii, double yiiii, double ziiii, double xii, double yii,
+                   double zii) {
+        if (icheck == 2 && chz > chy) {
+            charge(xb0, yb0, zb0, xiiii, yiiii, ziiii);
+            xb0 = xiiii; yb0 = yiiii; zb0 = ziiii;
+            charge(xb0, yb0, zb0, xiii, yiii, ziii);
+            xb0 = xiii; yb0 = yiii; zb0 = ziii;
+        } else if (icheck == 2 && chz < chy) {
+            charge(xb0, yb0, zb0, xiii, yiii, ziii);
+            xb0 = xiii; yb0 = yiii; zb0 = ziii;
+            charge(xb0, yb0, zb0, xiiii, yiiii, ziiii);
+            xb0 = xiiii; yb0 = yiiii; zb0 = ziiii;
+        } else if (icheck == 1 && chz != 0) {
+            charge(xb0, yb0, zb0, xiiii, yiiii, ziiii);
+            xb0 = xiiii; yb0 = yiiii; zb0 = ziiii;
+        } else if (icheck == 1 && chy != 0) {
+            charge(xb0, yb0, zb0, xiii, yiii, ziii);
+            xb0 = xiii; yb0 = yiii; zb0 = ziii;
+        }
+        charge(xb0, yb0, zb0, xii, yii, zii);
+        xb0 = xii; yb0 = yii; zb0 = zii;
+    }
+};
+
+inline size_t out_index(const orc_geom& g, int a, int j, int jj) {
+    /* P/reprojectionOnGPU.cu:106: row H-1-j */
+    return (size_t)a * g.H * g.W + (size_t)(g.H - 1 - j) * g.W + jj;
+}
+
+inline double vox(const uint16_t* vu, const float* vf, size_t idx) {
+    return vu ? (double)vu[idx] : (double)vf[idx];
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_trig_table_proj(int W, int H, int n, const double* angles_rad, const double* xshift,
+                         const double* yshift, double det_pitch, double det_pixel_num, double* cos_t,
+                         double* sin_t, double* xc, double* yc) {
+    (void)W;
+    /* P/misc.cpp:25-34: both shifts use the projection HEIGHT. */
+    for (int i = 0; i < n; i++) {
+        cos_t[i] = cos(angles_rad[i]);
+        sin_t[i] = sin(angles_rad[i]);
+        xc[i] = -H / 2.0 - xshift[i] * 10.0 / (det_pitch * (det_pixel_num / H));
+        yc[i] = -H / 2.0 - yshift[i] * 10.0 / (det_pitch * (det_pixel_num / H));
+    }
+}
+
+void orc_trig_table_fdk(int n, const double* angles_rad, double* cos_t, double* sin_t) {
+    for (int i = 0; i < n; i++) {
+        cos_t[i] = cos(angles_rad[i]);
+        sin_t[i] = sin(angles_rad[i]);
+    }
+}
+
+int orc_attenuation_table(const char* spectrum_csv, const char* xraydb_dir, int nmat, int nthin, float* atten,
+                          float* pdf, int cap) {
+    /* P/prior_weight_production.cpp:200-270: spectrum read, thinning, normalisation. */
+    FILE* fp = fopen(spectrum_csv, "r");
+    if (!fp) return -1;
+    char row[1024];
+    float d0 = 0, d1 = 0, d2 = 0, sum = 0;
+    int nne = 0;
+    std::vector<double> energy(1024);
+    std::vector<float> pdf_raw(1024);
+    if (fgets(row, sizeof(row), fp) == NULL) { fclose(fp); return -2; }
+    while (fgets(row, sizeof(row), fp) != NULL) {
+        if (nne >= 1024) { fclose(fp); return -3; }
+        /* the reference does not check the conversion count: an unparsable row repeats the previous
+         * row's values (P/prior_weight_production.cpp:220-228) */
+        sscanf(row, "%f,%f,%f", &d0, &d1, &d2);
+        energy[nne] = (d1 + d0) * 0.5;
+        pdf_raw[nne] = d2;
+        sum += d2;
+        nne++;
+    }
+    fclose(fp);
+    if (nne == 0 || nthin < 1) return -4;
+    double psum = 0.0, esum = 0.0, sum_thin = 0.0;
+    int nne_thin = 0;
+    float pdf_thin[1024], pe_thin[1024];
+    for (int i = 0; i < nne; i++) {
+        psum += pdf_raw[i];
+        esum += energy[i];
+        if (i - nthin + 1 == (i - nthin + 1) / nthin * nthin) {
+            int k = (i - nthin + 1) / nthin;
+            if (k < 0) continue;
+            pdf_thin[k] = psum;
+            pe_thin[k] = esum / nthin;
+            sum_thin += psum;
+            psum = 0.0;
+            esum = 0.0;
+            nne_thin = k + 1;
+        }
+    }
+    nne = nne_thin;
+    sum = sum_thin;
+    if (nne > cap) return -5;
+    for (int i = 0; i < nne; i++) {
+        pdf[i] = pdf_thin[i];
+        energy[i] = pe_thin[i];
+    }
+    for (int i = 0; i < nne; i++) pdf[i] = pdf[i] / sum;
+
+    /* P/prior_weight_production.cpp:318-507: xraydb rows, 5 eV grid, match |dE| < 30e-9 MeV,
+     * mu/rho = photo + compton + rayleigh.  A bin with no match keeps the previous bin's values. */
+    const int ndb = 29001;
+    if (nmat < 1 || nmat > 6) return -6;
+    std::vector<double> edb(ndb), ph(ndb * 6), co(ndb * 6), ra(ndb * 6);
+    const char* names[3] = {"photo_5keVto150keV.csv", "comp_5keVto150keV.csv", "rayl_5keVto150keV.csv"};
+    std::vector<double>* dst[3] = {&ph, &co, &ra};
+    for (int f = 0; f < 3; f++) {
+        char path[2048];
+        snprintf(path, sizeof(path), "%s/%s", xraydb_dir, names[f]);
+        FILE* fd = fopen(path, "rb");
+        if (!fd) return -7;
+        char lb[512];
+        if (!fgets(lb, sizeof(lb), fd)) { fclose(fd); return -8; }
+        for (int e = 0; e < ndb; e++) {
+            double v[6], en;
+            if (!fgets(lb, sizeof(lb), fd)) { fclose(fd); return -9; }
+            sscanf(lb, "%lf,%lf,%lf,%lf,%lf,%lf,%lf", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &en);
+            if (f == 0) edb[e] = en / 1000 / 1000;
+            for (int k = 0; k < 6; k++) (*dst[f])[k * ndb + e] = v[k];
+        }
+        fclose(fd);
+    }
+    double photo = 0, compton = 0, rayleigh = 0;
+    for (int iz = 0; iz < nmat; iz++) {
+        for (int i = 0; i < nne; i++) {
+            for (int e = 0; e < ndb; e++) {
+                double aaa = edb[e] * 1000 * 1000 * 1000 - energy[i] * 1000 * 1000 * 1000;
+                if (-30 < aaa && aaa < 30) {
+                    photo = ph[iz * ndb + e];
+                    compton = co[iz * ndb + e];
+                    rayleigh = ra[iz * ndb + e];
+                }
+            }
+            atten[iz * nne + i] = (photo + compton + rayleigh + 0.0);
+        }
+    }
+    return nne;
+}
+
+int orc_forward_compat(const orc_geom* gp, const double* cos_t, const double* sin_t, const double* xc,
+                       const double* yc, const uint16_t* vol, const float* atten, const float* pdf, int NE,
+                       double xfactor, int a0, int a1, float* out) {
+    const orc_geom& g = *gp;
+    const size_t n2 = (size_t)g.N * g.N;
+#pragma omp parallel
+    {
+        Tracer tr(g);
+        std::vector<Seg> segs;
+        segs.reserve(8192);
+#pragma omp for collapse(2) schedule(dynamic, 4)
+        for (int a = a0; a < a1; a++)
+            for (int j = 0; j < g.H; j++)
+                for (int jj = 0; jj < g.W; jj++) {
+                    Ray R = make_ray(g, cos_t, sin_t, xc, yc, a, j, jj);
+                    tr.trace(R, segs);
+                    float acc = 0.0f; /* P/reprojectionOnGPU.cu:1038-1044 (FP32, ke order) */
+                    for (int ke = 0; ke < NE; ke++) {
+                        double transp = 0.0;
+                        for (const Seg& sg : segs)
+                            for (int mat = 0; mat < g.nmat; mat++) {
+                                size_t vnum = sg.xmc + (size_t)sg.ymc * g.N + ((size_t)sg.zmc * g.nmat + mat) * n2;
+                                /* u16 * float is an FP32 product in the reference (:268) */
+                                float vm = (float)vol[vnum] * atten[mat * NE + ke];
+                                transp += vm * sg.rr * sg.ph;
+                            }
+                        double intens = pdf[ke] * exp(-transp / xfactor) * R.leng;
+                        acc += (float)intens;
+                    }
+                    out[out_index(g, a, j, jj)] = acc;
+                }
+    }
+    return 0;
+}
+
+int orc_line_integrals(const orc_geom* gp, const double* cos_t, const double* sin_t, const double* xc,
+                       const double* yc, const uint16_t* vu, const float* vf, int a0, int a1, double* L,
+                       int32_t* nseg, double* leng) {
+    const orc_geom& g = *gp;
+    const size_t n2 = (size_t)g.N * g.N;
+#pragma omp parallel
+    {
+        Tracer tr(g);
+        std::vector<Seg> segs;
+        segs.reserve(8192);
+#pragma omp for collapse(2) schedule(dynamic, 4)
+        for (int a = a0; a < a1; a++)
+            for (int j = 0; j < g.H; j++)
+                for (int jj = 0; jj < g.W; jj++) {
+                    Ray R = make_ray(g, cos_t, sin_t, xc, yc, a, j, jj);
+                    tr.trace(R, segs);
+                    size_t o = out_index(g, a, j, jj);
+                    double Lm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                    for (const Seg& sg : segs)
+                        for (int mat = 0; mat < g.nmat; mat++) {
+                            size_t vnum = sg.xmc + (size_t)sg.ymc * g.N + ((size_t)sg.zmc * g.nmat + mat) * n2;
+                            Lm[mat] += vox(vu, vf, vnum) * sg.rr * sg.ph;
+                        }
+                    for (int mat = 0; mat < g.nmat; mat++) L[o * g.nmat + mat] = Lm[mat];
+                    if (nseg) nseg[o] = (int32_t)segs.size();
+                    if (leng) leng[o] = R.leng;
+                }
+    }
+    return 0;
+}
+
+int orc_forward_collapsed(const orc_geom* gp, const double* cos_t, const double* sin_t, const double* xc,
+                          const double* yc, const uint16_t* vu, const float* vf, const float* atten,
+                          const float* pdf, int NE, double xfactor, int a0, int a1, float* out) {
+    const orc_geom& g = *gp;
+    size_t nray = (size_t)(a1 - a0) * g.H * g.W;
+    std::vector<double> L(nray * g.nmat), lg(nray);
+    /* L and leng are indexed from angle a0 */
+    std::vector<double> cs(cos_t + a0, cos_t + a1), sn(sin_t + a0, sin_t + a1), xs(xc + a0, xc + a1),
+        ys(yc + a0, yc + a1);
+    orc_line_integrals(gp, cs.data(), sn.data(), xs.data(), ys.data(), vu, vf, 0, a1 - a0, L.data(), nullptr,
+                       lg.data());
+    float* o = out + (size_t)a0 * g.H * g.W;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)nray; i++) {
+        float acc = 0.0f;
+        for (int ke = 0; ke < NE; ke++) {
+            double transp = 0.0;
+            for (int mat = 0; mat < g.nmat; mat++) transp += (double)atten[mat * NE + ke] * L[i * g.nmat + mat];
+            double intens = pdf[ke] * exp(-transp / xfactor) * lg[i];
+            acc += (float)intens;
+        }
+        o[i] = acc;
+    }
+    return 0;
+}
+
+int orc_logproj(const orc_geom* gp, const double* cos_t, const double* sin_t, const double* xc, const double* yc,
+                const float* vf, const float* atten, const float* pdf, int NE, double xfactor, double* p) {
+    const orc_geom& g = *gp;
+    size_t nray = (size_t)g.n_ang * g.H * g.W;
+    std::vector<double> L(nray * g.nmat);
+    orc_line_integrals(gp, cos_t, sin_t, xc, yc, nullptr, vf, 0, g.n_ang, L.data(), nullptr, nullptr);
+    double wsum = 0.0;
+    for (int e = 0; e < NE; e++) wsum += pdf[e];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)nray; i++) {
+        double s = 0.0;
+        for (int e = 0; e < NE; e++) {
+            double tr = 0.0;
+            for (int m = 0; m < g.nmat; m++) tr += (double)atten[m * NE + e] * L[i * g.nmat + m];
+            s += pdf[e] * exp(-tr / xfactor);
+        }
+        p[i] = -log(s / wsum);
+    }
+    return 0;
+}
+
+int orc_adjoint(const orc_geom* gp, const double* cos_t, const double* sin_t, const double* xc, const double* yc,
+                const float* vf, const float* atten, const float* pdf, int NE, double xfactor, const double* target,
+                double* grad, double* loss) {
+    const orc_geom& g = *gp;
+    const size_t n2 = (size_t)g.N * g.N;
+    const size_t nvox = n2 * g.Nz * g.nmat;
+    memset(grad, 0, nvox * sizeof(double));
+    double wsum = 0.0;
+    for (int e = 0; e < NE; e++) wsum += pdf[e];
+    double total = 0.0;
+    int nthr = 1;
+#ifdef _OPENMP
+    nthr = omp_get_max_threads();
+#endif
+    std::vector<std::vector<double>> part(nthr);
+#pragma omp parallel reduction(+ : total)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        std::vector<double>& gl = part[tid];
+        gl.assign(nvox, 0.0);
+        Tracer tr(g);
+        std::vector<Seg> segs;
+        segs.reserve(8192);
+#pragma omp for collapse(2) schedule(dynamic, 4)
+        for (int a = 0; a < g.n_ang; a++)
+            for (int j = 0; j < g.H; j++)
+                for (int jj = 0; jj < g.W; jj++) {
+                    Ray R = make_ray(g, cos_t, sin_t, xc, yc, a, j, jj);
+                    tr.trace(R, segs);
+                    double Lm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                    for (const Seg& sg : segs)
+                        for (int m = 0; m < g.nmat; m++) {
+                            size_t vnum = sg.xmc + (size_t)sg.ymc * g.N + ((size_t)sg.zmc * g.nmat + m) * n2;
+                            Lm[m] += (double)vf[vnum] * sg.rr * sg.ph;
+                        }
+                    double s = 0.0, mub[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                    for (int e = 0; e < NE; e++) {
+                        double trn = 0.0;
+                        for (int m = 0; m < g.nmat; m++) trn += (double)atten[m * NE + e] * Lm[m];
+                        double we = pdf[e] * exp(-trn / xfactor);
+                        s += we;
+                        for (int m = 0; m < g.nmat; m++) mub[m] += we * atten[m * NE + e];
+                    }
+                    double p = -log(s / wsum);
+                    double r = p - target[out_index(g, a, j, jj)];
+                    total += r * r;
+                    /* dp/dL_m = mubar_m / xf with mubar the exp-weighted spectral mean of mu */
+                    double coef[8];
+                    for (int m = 0; m < g.nmat; m++) coef[m] = 2.0 * r * (mub[m] / s) / xfactor;
+                    for (const Seg& sg : segs)
+                        for (int m = 0; m < g.nmat; m++) {
+                            size_t vnum = sg.xmc + (size_t)sg.ymc * g.N + ((size_t)sg.zmc * g.nmat + m) * n2;
+                            gl[vnum] += coef[m] * sg.rr * sg.ph;
+                        }
+                }
+    }
+    for (int t = 0; t < nthr; t++)
+        if (!part[t].empty())
+            for (size_t i = 0; i < nvox; i++) grad[i] += part[t][i];
+    if (loss) *loss = total;
+    return 0;
+}
+
+void orc_const_map(int W, double pitch_iso, double* cmap) {
+    /* F/filterSinogram.cpp:65-120 with MAX_PIC_NUM 800, CUTOFF_FREQ 1.6, GAMMA 1, ALPHA 0.5 */
+    const double CUT = 1.60, GAM = 1.00, ALP = 0.50;
+    double theConst = 1.0 / (8.0 * kPI * kPI * pitch_iso * 800 / W);
+    double gamma_x_cutoffFreq = GAM * CUT;
+    double PI_x_cutoffFreq = kPI * CUT;
+    double one_minus_alpha = 1.0 - ALP;
+    double omega = kPI / GAM;
+    double cos_omega = cos(omega);
+    double sin_omega = sin(omega);
+    for (int m = 0; m < W; m++)
+        for (int n = 0; n < W; n++) {
+            if (m == n) {
+                cmap[(size_t)m * W + n] = 0.0;
+            } else {
+                double MN = (double)(m - n);
+                double theta = PI_x_cutoffFreq * MN;
+                double delta = gamma_x_cutoffFreq * MN;
+                double eta = 1.0 / (1.0 - 1.0 / delta);
+                double xi = 1.0 / (1.0 + 1.0 / delta);
+                double cos_theta = cos(theta);
+                double sin_theta = sin(theta);
+                cmap[(size_t)m * W + n] =
+                    theConst / MN / MN *
+                    (2.0 * ALP * theta * sin_theta + 2.0 * ALP * (cos_theta - 1.0) +
+                     xi * one_minus_alpha * theta * (sin_theta * cos_omega + cos_theta * sin_omega) +
+                     eta * one_minus_alpha * theta * (sin_theta * cos_omega - cos_theta * sin_omega) +
+                     xi * xi * one_minus_alpha * (cos_theta * cos_omega - sin_theta * sin_omega - 1.0) +
+                     eta * eta * one_minus_alpha * (cos_theta * cos_omega + sin_theta * sin_omega - 1.0));
+            }
+        }
+}
+
+static double filter_coeff(int W, double pitch_iso) {
+    const double CUT = 1.60, GAM = 1.00, ALP = 0.50;
+    double omega = kPI / GAM;
+    return CUT * CUT / 4.0 / (pitch_iso * 800 / W) *
+           (ALP * 0.5 + (1.0 - ALP) * (1.0 / omega * sin(omega) + 1.0 / omega / omega * (cos(omega) - 1.0)));
+}
+
+static inline double filter_cm(int W, int n, int m, double xs, double ys, double pitch_iso) {
+    /* F/filterSinogram.cpp:141-154: both centres use the projection WIDTH */
+    double XcenterShift = -W / 2.0 - xs / (0.05 * 800 / W) - 1;
+    double YcenterShift = -W / 2.0 - ys / (0.05 * 800 / W) - 1;
+    double pos2 = fabs((YcenterShift + n)) * (pitch_iso * 800 / W);
+    double pos1 = fabs((XcenterShift + m)) * (pitch_iso * 800 / W);
+    return 100.0 / sqrt(100.0 * 100.0 + pos1 * pos1 + pos2 * pos2);
+}
+
+void orc_filter_sinogram(int H, int W, int n_ang, const float* proj, float* out, const double* cmap,
+                         const double* xshift, const double* yshift, double pitch_iso) {
+    /* F/filterSinogram.cpp:122-173: FP64 terms accumulated into an FP32 lvalue, k ascending */
+    double coeff = filter_coeff(W, pitch_iso);
+#pragma omp parallel for
+    for (int th = 0; th < n_ang; th++)
+        for (int n = 0; n < H; n++) {
+            const float* q = proj + ((size_t)th * H + n) * W;
+            float* o = out + ((size_t)th * H + n) * W;
+            for (int m = 0; m < W; m++) {
+                double Cm = filter_cm(W, n, m, xshift[th], yshift[th], pitch_iso);
+                o[m] = -log(q[m]) * Cm * coeff;
+                for (int k = 0; k < W; k++) o[m] += cmap[(size_t)m * W + k] * (-log(q[k])) * Cm;
+            }
+        }
+}
+
+void orc_filter_sinogram64(int H, int W, int n_ang, const float* proj, double* out, const double* cmap,
+                           const double* xshift, const double* yshift, double pitch_iso) {
+    double coeff = filter_coeff(W, pitch_iso);
+#pragma omp parallel for
+    for (int th = 0; th < n_ang; th++)
+        for (int n = 0; n < H; n++) {
+            const float* q = proj + ((size_t)th * H + n) * W;
+            double* o = out + ((size_t)th * H + n) * W;
+            for (int m = 0; m < W; m++) {
+                double Cm = filter_cm(W, n, m, xshift[th], yshift[th], pitch_iso);
+                double acc = -log(q[m]) * coeff;
+                for (int k = 0; k < W; k++) acc += cmap[(size_t)m * W + k] * (-log(q[k]));
+                o[m] = acc * Cm;
+            }
+        }
+}
+
+void orc_mask(int R, char* mask) {
+    /* F/misc.cpp:20-35 */
+    double rad = (double)(R - 1) / 2.0, sq = rad * rad;
+    for (int j = 0; j < R; j++)
+        for (int i = 0; i < R; i++)
+            mask[(size_t)j * R + i] =
+                (((double)j - rad) * ((double)j - rad) + ((double)i - rad) * ((double)i - rad) <= sq) ? 1 : 0;
+}
+
+void orc_reconstruct_image(int R, int Z, int zstart, double step, double scale, double* recon, const char* mask,
+                           int H, int W, int n_ang, const float* sino, int xfactor, const double* angles,
+                           const double* cos_t, const double* sin_t, const double* xshift, double pitch_iso) {
+    /* F/reconstructImage.cpp:10-184, accumulating into the caller's buffer (+=) */
+    double centerPos = (double)R / 2.0 - 0.5;
+#pragma omp parallel for
+    for (int z = 0; z < Z; z++) {
+        double zz = zstart + z * step;
+        for (int y = 0; y < R; y++)
+            for (int x = 0; x < R; x++) {
+                if (mask[(size_t)y * R + x] == 0) continue;
+                double xx = (x - centerPos) * scale;
+                double yy = -(y - centerPos) * scale;
+                double& acc = recon[((size_t)z * R + y) * R + x];
+                for (int th = 0; th < n_ang; th++) {
+                    double dbeta, thdb0, thdb1, thdb2;
+                    if (th == 0) {
+                        thdb1 = angles[th];
+                        thdb2 = angles[th + 1];
+                        dbeta = fabs(thdb2 - thdb1) / 2;
+                    } else if (th == n_ang - 1) {
+                        thdb0 = angles[th - 1];
+                        thdb1 = angles[th];
+                        dbeta = fabs(thdb1 - thdb0) / 2;
+                    } else {
+                        thdb0 = angles[th - 1];
+                        thdb2 = angles[th + 1];
+                        thdb1 = angles[th + 1] - 2 * kPI;
+                        if (thdb2 < 0) thdb1 = angles[th + 1] + 2 * kPI;
+                        dbeta = fabs(thdb2 - thdb0) / 2;
+                        if (dbeta > kPI / 2) dbeta = fabs(thdb1 - thdb0) / 2;
+                    }
+                    double c = cos_t[th], s = sin_t[th];
+                    double X = xx * c + yy * s;
+                    double Y = -xx * s + yy * c;
+                    double L2 = 1.0 - Y / 100.0;
+                    double U = X / L2 / (pitch_iso * 800 / W);
+                    double w = fabs(fabs(U) - fabs(floor(U)));
+                    double UUU = (double)(W / 2 + U + xshift[th] / (0.05 * 800 / W) - 0.5);
+                    int UU = (int)floor(UUU);
+                    double ZETA = (double)(H / 2) + zz / L2 / (pitch_iso * 800 / W);
+                    int NZ = (int)floor(ZETA);
+                    double wz = fabs(fabs(ZETA) - fabs(floor(ZETA)));
+                    const float* P = sino + (size_t)th * H * W;
+                    double sum;
+                    bool hit = false;
+                    if ((UU >= 0 && UU < W - 1) && (NZ >= 0 && NZ < H - 1)) {
+                        sum = P[(size_t)NZ * W + UU] * (1.0 - w) * (1.0 - wz) + P[(size_t)NZ * W + UU + 1] * w * (1.0 - wz) +
+                              P[(size_t)(NZ + 1) * W + UU] * (1.0 - w) * wz + P[(size_t)(NZ + 1) * W + UU + 1] * w * wz;
+                        hit = true;
+                    } else if ((UU == W - 1) && (NZ >= 0 && NZ < H - 1)) {
+                        sum = P[(size_t)NZ * W + UU] * (1.0 - wz) + P[(size_t)(NZ + 1) * W + UU] * wz;
+                        hit = true;
+                    } else if ((UU >= 0 && UU < W - 1) && (NZ == H - 1)) {
+                        sum = P[(size_t)NZ * W + UU] * (1.0 - w) + P[(size_t)NZ * W + UU + 1] * w;
+                        hit = true;
+                    } else if ((UU == W - 1) && (NZ == H - 1)) {
+                        sum = P[(size_t)NZ * W + UU];
+                        hit = true;
+                    }
+                    if (hit) acc += (1.0 / L2 / L2 * sum * dbeta * xfactor);
+                }
+            }
+    }
+}
+
+int orc_fdk_angles(const char* flexmap, const char* tgt_class, int total, double* angles_rad, double* xshift,
+                   double* yshift, int cap) {
+    /* F/projectionDataMVkV.cpp:155-274 */
+    FILE* fp = fopen(flexmap, "r");
+    if (!fp) return -1;
+    std::vector<double> pa, px, py;
+    std::vector<int> ph;
+    double angle, x, y, cw0 = 0, cw = 0;
+    char cls[64];
+    int i = 0;
+    while (fscanf(fp, "%lf %lf %lf %63s", &angle, &x, &y, cls) == 4) {
+        i++;
+        if (i == 1) cw0 = angle;
+        if (i == 30) cw = (cw0 - angle);
+        ph.push_back((!strcmp(tgt_class, "*") || !strcmp(cls, tgt_class)) ? 1 : 0);
+        pa.push_back(angle);
+        px.push_back(x);
+        py.push_back(y);
+    }
+    fclose(fp);
+    int nl = (int)pa.size();
+    /* a flexmap shorter than 30 rows leaves cwccw uninitialised in the reference; here it is 0 (CCW) */
+    bool rev = cw < 0;
+    int used = total;
+    if (used > cap) return -2;
+    int cnt = 0;
+    for (int k = 0; k < nl; k++) {
+        int src = rev ? (nl - 1 - k) : k;
+        if (ph[src] == 1) {
+            if (cnt >= used) break; /* the reference writes past its arrays here */
+            angles_rad[cnt] = pa[src] * kPI / 180.0;
+            xshift[cnt] = px[src];
+            yshift[cnt] = py[src];
+            cnt++;
+        }
+    }
+    return cnt;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// CPU build of the GPU ray traversal (virtualcbct_amd/csrc/traverse.h) for tests: each 64-row wave is
+// emulated on the host (the wave-wide step range is the min/max over its lanes, as the kernel's shuffle
+// reduction computes it), so the lockstep walk, the step culling and the quirk handling can be checked
+// against the oracle without a GPU.  Test infrastructure only.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <vector>
+
+#include "traverse.h"
+
+namespace {
+struct HostAcc {
+    const uint16_t* vol;  // reference layout [z][mat][y][x]
+    int N, Nz, nmat;
+    double L[VCBCT_MAX_MAT];
+    int nseg;
+    void seg(int xmc, int ymc, int zmc, double rr) {
+        const size_t n2 = (size_t)N * N;
+        for (int m = 0; m < nmat; ++m) L[m] += (double)vol[((size_t)zmc * nmat + m) * n2 + (size_t)ymc * N + xmc] * rr;
+        ++nseg;
+    }
+};
+}  // namespace
+
+extern "C" int ht_line_integrals(int N, double s, int Nz, double t, int H, int W, int nAng, double x_reg, double y_reg,
+                                 double z_reg, double pitch, double pitch_iso, double sad, double pixnum, int nmat,
+                                 const double* tab, const uint16_t* vol, double* L, int32_t* nseg) {
+    vcbct::ProjGeom g;
+    g.N = N; g.Nz = Nz; g.H = H; g.W = W; g.nAng = nAng; g.nmat = nmat;
+    g.s = s; g.t = t; g.x_reg = x_reg; g.y_reg = y_reg; g.z_reg = z_reg;
+    g.pitch = pitch; g.pitch_iso = pitch_iso; g.sad = sad; g.pixnum = pixnum;
+#pragma omp parallel for collapse(2) schedule(dynamic)
+    for (int a = 0; a < nAng; ++a)
+        for (int jj = 0; jj < W; ++jj)
+            for (int rg = 0; rg < (H + 63) / 64; ++rg) {
+                vcbct::RayCtx r[64];
+                bool act[64];
+                int kmin = INT_MAX, kmax = INT_MIN;
+                for (int l = 0; l < 64; ++l) {
+                    int j = rg * 64 + l;
+                    act[l] = j < H;
+                    vcbct::ray_setup(g, tab[a], tab[nAng + a], tab[2 * nAng + a], tab[3 * nAng + a], act[l] ? j : 0, jj, r[l]);
+                    int lo, hi;
+                    vcbct::z_step_range(g, r[l], act[l], lo, hi);
+                    kmin = std::min(kmin, lo);
+                    kmax = std::max(kmax, hi);
+                }
+                for (int l = 0; l < 64; ++l) {
+                    if (!act[l]) continue;
+                    int j = rg * 64 + l;
+                    HostAcc acc{vol, N, Nz, nmat, {0, 0, 0, 0, 0, 0}, 0};
+                    vcbct::traverse_lane(g, r[l], act[l], kmin, kmax, acc);
+                    size_t o = (size_t)a * H * W + (size_t)(H - 1 - j) * W + jj;
+                    for (int m = 0; m < nmat; ++m) L[o * nmat + m] = acc.L[m];
+                    if (nseg) nseg[o] = acc.nseg;
+                }
+            }
+    return 0;
+}

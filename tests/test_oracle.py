@@ -1,0 +1,322 @@
+"""Pins the FP64 oracle: against the reference's own CPU code (golden fixtures written by
+tests/golden/gen_golden.py from oracle/_ref), against closed forms, and against invariants chosen so that a
+dropped term, a wrong sign or a transposed index fails."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from virtualcbct_amd import phantoms
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+# ------------------------------------------------------------------------------------------- FDK pieces
+def test_const_map_bitwise():
+    d = gold("fdk_small.npz")
+    assert np.array_equal(O.const_map(16), d["const_map16"])
+    cm = O.const_map(400)
+    assert np.array_equal(cm[0], d["const_map400_row0"])
+    assert np.array_equal(cm[199], d["const_map400_row199"])
+
+
+def test_filter_sinogram_bitwise():
+    d = gold("fdk_small.npz")
+    out = O.filter_sinogram(d["proj"], d["xshift"], d["yshift"])
+    assert np.array_equal(out, d["filtered"])
+    out2 = O.filter_sinogram(d["proj2"], d["xshift12"], d["yshift12"])
+    assert np.array_equal(out2, d["filtered2"])
+
+
+def test_filter_fp64_close_to_fp32_accumulation():
+    d = gold("fdk_small.npz")
+    f64 = O.filter_sinogram(d["proj2"], d["xshift12"], d["yshift12"], acc64=True)
+    assert np.abs(f64 - d["filtered2"]).max() <= 2e-5 * np.abs(f64).max()
+
+
+def test_reconstruct_image_bitwise():
+    d = gold("fdk_small.npz")
+    rec = O.reconstruct_image(16, 4, -1, 0.5, 0.3, d["sino"], 1000, d["angles"], d["xshift8"])
+    assert np.array_equal(rec, d["recon"])
+    rec2 = O.reconstruct_image(24, 6, -1, 0.35, 0.5, d["filtered2"], 1000, d["angles12"], d["xshift12"])
+    assert np.array_equal(rec2, d["recon2"])
+
+
+def test_mask_bitwise():
+    d = gold("fdk_small.npz")
+    assert np.array_equal(O.mask(16), d["mask16"])
+    assert np.array_equal(O.mask(270), d["mask270"])
+
+
+def test_filter_dc_response_and_impulse():
+    # The reference kernel is a band-limited ramp: its DC response coeff + sum_{d != 0} h(d) is ~0 for a
+    # central output pixel (a dropped or sign-flipped tap breaks this), and filtering one impulse returns
+    # the kernel column times the cosine weight.
+    # The only DC left is the truncation tail sum_{|d| > W/2} h(d) ~ 1/W, so (dc / coeff) * W is constant.
+    scaled = []
+    for W in (400, 1000):
+        cm = O.const_map(W)
+        coeff = 1.6 * 1.6 / 4.0 / (0.032509752 * 800 / W) * (0.5 * 0.5 + 0.5 * (0.0 + (-2.0) / np.pi ** 2))
+        dc = coeff + cm[W // 2].sum()
+        scaled.append(dc / coeff * W)
+    assert scaled[0] == pytest.approx(scaled[1], rel=1e-3)
+    assert 0.5 < scaled[0] < 2.0
+    W = 400
+    cm = O.const_map(W)
+    coeff = 1.6 * 1.6 / 4.0 / (0.032509752 * 800 / W) * (0.5 * 0.5 + 0.5 * (0.0 + (-2.0) / np.pi ** 2))
+    q = np.zeros((1, 1, W))
+    q[0, 0, 150] = 0.5
+    proj = np.exp(-q).astype(np.float32)
+    out = O.filter_sinogram(proj, [0.0], [0.0], acc64=True)[0, 0]
+    Xc = -W / 2.0 - 1
+    pos2 = abs(Xc + 0) * 0.032509752 * 800 / W
+    for mm in (150, 151, 170, 10):
+        pos1 = abs(Xc + mm) * 0.032509752 * 800 / W
+        Cm = 100.0 / np.sqrt(100.0 ** 2 + pos1 ** 2 + pos2 ** 2)
+        # -log() of a float resolves to the C++ float overload in the reference: q is an FP32 logarithm
+        lq = float(-np.log(np.float32(np.exp(-0.5))))
+        expect = (coeff if mm == 150 else cm[mm, 150]) * lq * Cm
+        assert out[mm] == pytest.approx(expect, rel=2e-7, abs=1e-12)
+
+
+def test_backprojection_constant_sinogram_closed_form():
+    # A constant filtered sinogram c backprojects to c * xfactor * sum_th dbeta_th / L2_th^2 wherever every
+    # sample is an interior bilinear one (weights sum to one).
+    n, H, W, R = 12, 64, 64, 16
+    ang = np.deg2rad(np.arange(n) * 30.0 - 170.0001)
+    xs = np.zeros(n)
+    c = 0.25
+    sino = np.full((n, H, W), c, np.float32)
+    rec = O.reconstruct_image(R, 2, 0, 0.5, 0.2, sino, 1000, ang, xs)
+    msk = O.mask(R)
+    dbeta = np.empty(n)
+    for th in range(n):
+        if th == 0:
+            dbeta[th] = abs(ang[1] - ang[0]) / 2
+        elif th == n - 1:
+            dbeta[th] = abs(ang[-1] - ang[-2]) / 2
+        else:
+            dbeta[th] = abs(ang[th + 1] - ang[th - 1]) / 2
+    cen = R / 2 - 0.5
+    for (y, x) in ((8, 8), (3, 9), (12, 4)):
+        if not msk[y, x]:
+            continue
+        xx, yy = (x - cen) * 0.2, -(y - cen) * 0.2
+        Y = -xx * np.sin(ang) + yy * np.cos(ang)
+        L2 = 1 - Y / 100.0
+        expect = np.sum(c * 1000 * dbeta / L2 ** 2)
+        assert rec[1, y, x] == pytest.approx(expect, rel=1e-12)
+
+
+# ------------------------------------------------------------------------------------------- projector
+def _small_geom(d):
+    return O.Geometry(N=int(d["N"]) if "N" in d else 32, s=0.5, Nz=8, t=0.5, H=24, W=24, cos_t=d["cos"],
+                      sin_t=d["sin"], xc=d["xc"], yc=d["yc"], z_reg=0.3)
+
+
+def test_trig_table_matches_reference_fixture():
+    d = gold("reproj_small.npz")
+    cs, sn, xc, yc = O.trig_table_proj(24, 24, d["angles"], d["xshift"], d["yshift"])
+    assert np.array_equal(cs, d["cos"]) and np.array_equal(sn, d["sin"])
+    assert np.array_equal(xc, d["xc"]) and np.array_equal(yc, d["yc"])
+
+
+def test_forward_compat_matches_reference_cpu_twin():
+    # The CPU twin (P/reprojection.cpp) computes the same traversal as the live kernel except: no cos^3
+    # factor, rows not flipped, and two tie rules that only act on rays with exact ties (none here: the
+    # detector is shifted off the rotation axis).  Per (angle, bin) the restatement must agree to FP32.
+    d = gold("reproj_small.npz")
+    g = _small_geom(d)
+    _, _, leng = O.line_integrals(g, d["vol"])
+    for ke in range(d["atten"].shape[1]):
+        one = O.forward_compat(g, d["vol"], np.ascontiguousarray(d["atten"][:, ke:ke + 1]), d["pdf"][ke:ke + 1],
+                               1000.0)
+        for a in range(len(d["angles"])):
+            mine = (one[a] / leng[a])[::-1]
+            twin = d["twin"][a, ke]
+            np.testing.assert_allclose(mine, twin, rtol=2e-6, atol=1e-30)
+
+
+def test_forward_collapsed_equals_per_energy_recompute():
+    # One traversal + spectral sum vs the reference's per-energy recompute.  They differ only by the
+    # reference's FP32 product vol*mu (P/reprojectionOnGPU.cu:268), far below 1e-6.
+    d = gold("reproj_small.npz")
+    g = _small_geom(d)
+    a = O.forward_compat(g, d["vol"], d["atten"], d["pdf"], 1000.0)
+    b = O.forward_collapsed(g, d["vol"], d["atten"], d["pdf"], 1000.0)
+    np.testing.assert_allclose(b, a, rtol=1e-6)
+
+
+def _geom(N, Nz, s, t, H, W, angles_deg, xs=0.0, z_reg=0.0):
+    ang = np.deg2rad(np.asarray(angles_deg, float))
+    cs, sn, xc, yc = O.trig_table_proj(W, H, ang, np.full(len(ang), xs), np.zeros(len(ang)))
+    return O.Geometry(N=N, s=s, Nz=Nz, t=t, H=H, W=W, cos_t=cs, sin_t=sn, xc=xc, yc=yc, z_reg=z_reg)
+
+
+def test_zero_volume_gives_air_intensity():
+    # I = sum_e float(pdf_e * leng): pins leng = (SAD^2 / (SAD^2 + r^2))^1.5, the row flip and the FP32 sum.
+    g = _geom(16, 4, 1.0, 1.0, 10, 12, [-170.0001, 33.3])
+    vol = np.zeros((4, 6, 16, 16), np.uint16)
+    atten = np.ones((6, 3), np.float32)
+    pdf = np.array([0.2, 0.5, 0.3], np.float32)
+    out = O.forward_compat(g, vol, atten, pdf, 1000.0)
+    for a in range(2):
+        for j in range(10):
+            for jj in range(12):
+                pz = -(g.yc[a] + j) * 0.032509752 * (800 / 10)
+                pr = 0.032509752 * (g.xc[a] + jj) * (800 / 10)
+                leng = 100.0 ** 2 / (100.0 ** 2 + (pr * pr + pz * pz))
+                leng = leng * np.sqrt(leng)
+                acc = np.float32(0)
+                for e in range(3):
+                    acc = np.float32(acc + np.float32(pdf[e] * 1.0 * leng))
+                assert out[a, 10 - 1 - j, jj] == acc
+
+
+def test_uniform_material_chord_length():
+    # One material at unit density filling the volume: L = chord of the ray through the inscribed circle
+    # (the mask), up to one segment at each end, for rays whose lumped entry segment lies outside the circle
+    # and whose z stays inside the slab.  A wrong segment length, a missing K factor or a transposed axis
+    # fails this by far more than the tolerance.
+    N, s = 64, 0.25
+    g = _geom(N, 40, s, 0.5, 30, 40, [-170.0001, -63.2, 9.99999, 47.0, 120.5], xs=0.01)
+    vol = np.zeros((40, 6, N, N), np.uint16)
+    vol[:, 3] = 1
+    L, _, _ = O.line_integrals(g, vol)
+    rmax = N * s / 2
+    checked = 0
+    for a in range(g.n_ang):
+        c, sn = g.cos_t[a], g.sin_t[a]
+        for j in range(12, 18):
+            for jj in range(0, 40, 3):
+                sx, sy, sz = -100 * sn, 100 * c, 0.0
+                pz = -(g.yc[a] + j) * 0.032509752 * (800 / 30)
+                pr = 0.032509752 * (g.xc[a] + jj) * (800 / 30)
+                px, py = pr * c, pr * sn
+                d = np.array([px - sx, py - sy, pz - sz])
+                dxy = np.hypot(d[0], d[1])
+                # distance of the xy line from the origin
+                dist = abs(sx * d[1] - sy * d[0]) / dxy
+                if dist > rmax - 2:
+                    continue
+                chord_xy = 2 * np.sqrt(rmax ** 2 - dist ** 2)
+                chord = chord_xy * np.linalg.norm(d) / dxy
+                K = np.linalg.norm(d) / max(abs(d[0]), abs(d[1]))
+                assert abs(L[a, 30 - 1 - j, jj, 3] - chord) <= 2 * s * K + 1e-9
+                checked += 1
+    assert checked > 50
+
+
+def test_lumped_entry_segment_quirk():
+    # The first segment, from the box entry to the whole-cm plane int(xi), is charged to its midpoint voxel
+    # (P/reprojectionOnGPU.cu:205-209, 405-428).  Box half-width 5.75 cm (N = 230, s = 0.05), rays along -x
+    # enter at x = 5.75; plane 0 is x = 5, so [5, 5.75] (15 voxels) is charged to the voxel holding x = 5.375
+    # (index 222) and the outermost voxel (229) is never charged.
+    N, s = 230, 0.05
+    g = _geom(N, 8, s, 1.0, 10, 9, [-90.00001], xs=0.0123)
+    r, c = 10 - 1 - 5, 5  # kernel row 5 (pz ~ 0), column 5 (y ~ 0)
+    vol = np.zeros((8, 6, N, N), np.uint16)
+    vol[:, 0, :, 222] = 1000
+    L, _, _ = O.line_integrals(g, vol)
+    assert L[0, r, c, 0] == pytest.approx(1000 * 0.75, rel=2e-3)
+    vol[:, 0, :, 222] = 0
+    vol[:, 0, :, 229] = 1000
+    L, _, _ = O.line_integrals(g, vol)
+    assert L[0, r, c, 0] == 0.0
+    vol[:, 0, :, 229] = 0
+    vol[:, 0, :, 200] = 1000  # an ordinary voxel: its own chord 0.05
+    L, _, _ = O.line_integrals(g, vol)
+    assert L[0, r, c, 0] == pytest.approx(1000 * 0.05, rel=2e-3)
+
+
+def test_z0_plane_not_split_and_slice0_extension():
+    # z crossings are detected by truncation, so the z = 0 plane is never a breakpoint; midpoints up to one
+    # slice above the top map to slice 0 (P/reprojectionOnGPU.cu:222, 238, 256-259).
+    N, s, Nz, t = 32, 0.5, 4, 1.0  # slices: 0 -> z in (1, 2], 1 -> (0, 1], 2 -> (-1, 0], 3 -> (-2, -1]
+    g = _geom(N, Nz, s, t, 40, 40, [-90.00001])
+    vol = np.zeros((Nz, 6, N, N), np.uint16)
+    vol[0, 1] = 1   # slice 0 only, material C
+    vol[2, 2] = 1   # slice 2 (z in (-1, 0]) only, material N
+    L, _, _ = O.line_integrals(g, vol)
+    # inside the box |x| <= 8 the ray's z stays within [0.92, 1.08] x pz
+    above, straddle = [], []
+    for j in range(40):
+        pz = -(g.yc[0] + j) * 0.032509752 * (800 / 40)
+        if 2.05 < 0.92 * pz and 1.08 * pz < 2.95:
+            above.append(40 - 1 - j)
+        if -0.5 < 0.92 * pz < 0 and pz != 0:
+            straddle.append((40 - 1 - j, pz))
+    assert above, "geometry must have rays in the extension band"
+    for r in above:  # z in (2, 3): one slice above the top, charged to slice 0
+        assert L[0, r, 21, 1] > 10.0
+    # a ray with z in (-0.5, 0) never crosses z = 0 or z = -1: all of it is charged to slice 2
+    for r, pz in straddle:
+        assert L[0, r, 21, 2] > 10.0
+
+
+def test_adjoint_identity_and_gradient():
+    # <A x, y> = <x, A^T y> for the linear line-integral operator (through the adjoint kernel's math with a
+    # linearised loss), and the loss gradient against central differences.
+    rng = np.random.default_rng(5)
+    N, Nz = 12, 4
+    g = _geom(N, Nz, 1.0, 1.0, 8, 10, [-170.0001, -40.00001, 59.99999], xs=0.02)
+    g.nmat = 2
+    x = rng.uniform(0.5, 1.5, (Nz, 2, N, N)).astype(np.float32)
+    NE = 4
+    atten = rng.uniform(0.1, 0.4, (2, NE)).astype(np.float32)
+    pdf = np.array([0.1, 0.4, 0.3, 0.2], np.float32)
+    p = O.logproj(g, x, atten, pdf, 1.0)
+    target = p + rng.normal(0, 0.05, p.shape)
+    grad, loss = O.adjoint(g, x, atten, pdf, 1.0, target)
+    assert loss == pytest.approx(np.sum((p - target) ** 2), rel=1e-12)
+    for (z, m, yy, xx) in [(1, 0, 6, 6), (2, 1, 5, 7), (0, 0, 6, 5)]:
+        h = 1e-3
+        xp = x.astype(np.float64).copy()
+        xm = xp.copy()
+        xp[z, m, yy, xx] += h
+        xm[z, m, yy, xx] -= h
+        lp = np.sum((O.logproj(g, xp.astype(np.float32), atten, pdf, 1.0) - target) ** 2)
+        lm = np.sum((O.logproj(g, xm.astype(np.float32), atten, pdf, 1.0) - target) ** 2)
+        fd = (lp - lm) / (2 * h)
+        assert grad[z, m, yy, xx] == pytest.approx(fd, rel=2e-2, abs=1e-6)
+    # linear part: L = A x;  <A x, w> == <x, A^T w> with A^T w from the gradient of 0.5*sum(w*L)... checked by
+    # building A column-by-column on a few voxels
+    Lx, _, _ = O.line_integrals(g, x)
+    w = rng.normal(0, 1, Lx.shape)
+    lhs = np.sum(Lx * w)
+    rhs = 0.0
+    base = np.zeros_like(x)
+    for z in range(Nz):
+        for m in range(2):
+            e = base.copy()
+            e[z, m] = x[z, m]
+            Le, _, _ = O.line_integrals(g, e)
+            rhs += np.sum(Le * w)
+    assert rhs == pytest.approx(lhs, rel=1e-12)
+
+
+def test_fdk_angle_loader_matches_reference():
+    d = gold("loaders.npz")
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        for n in (36, 360):
+            fm = os.path.join(td, f"f{n}.flexmap")
+            phantoms.write_flexmap(fm, n)
+            a, _, _ = O.fdk_angles(fm, n)
+            assert np.array_equal(a, d[f"fdk{n}_angles"])
+
+
+def test_reference_build_agrees_when_present():
+    if not O.ref_available():
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(9)
+    proj = rng.uniform(0.1, 1.0, (3, 5, 30)).astype(np.float32)
+    xs, ys = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+    a = O.filter_sinogram(proj, xs, ys)
+    b, _ = O.ref_filter_sinogram(proj, xs, ys)
+    assert np.array_equal(a, b)

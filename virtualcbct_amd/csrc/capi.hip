@@ -1,0 +1,307 @@
+// C ABI of the projector: context lifetime, uploads, and the forward / log-projection / adjoint calls.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <atomic>
+#include <vector>
+
+#include "vcbct_internal.h"
+
+namespace vcbct {
+
+static thread_local char g_err[1024] = "";
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+static std::atomic<long> g_alloc_count{0};
+
+hipError_t dev_malloc(void** p, size_t bytes) {
+    // VCBCT_FAULT_ALLOC=n makes the n-th device allocation of the process fail (host-side fault
+    // injection for the error paths; no kernel is involved).
+    static long fault_at = [] {
+        const char* e = getenv("VCBCT_FAULT_ALLOC");
+        return e ? atol(e) : 0L;
+    }();
+    long k = ++g_alloc_count;
+    if (fault_at > 0 && k == fault_at) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
+
+static int check_geometry(const vcbct_geometry* g) {
+    if (!g) return fail(VCBCT_EINVAL, "geometry is NULL");
+    if (g->recon_size < 2 || g->recon_size > 8192) return fail(VCBCT_EINVAL, "recon_size %d out of range [2, 8192]", g->recon_size);
+    if (g->recon_slices < 1 || g->recon_slices > 8192) return fail(VCBCT_EINVAL, "recon_slices %d out of range", g->recon_slices);
+    if (g->proj_height < 1 || g->proj_height > 16384 || g->proj_width < 1 || g->proj_width > 16384)
+        return fail(VCBCT_EINVAL, "projection size %dx%d out of range", g->proj_width, g->proj_height);
+    if (g->n_angles < 1 || g->n_angles > 100000) return fail(VCBCT_EINVAL, "n_angles %d out of range", g->n_angles);
+    if (g->num_material < 1 || g->num_material > VCBCT_MAX_MAT)
+        return fail(VCBCT_EINVAL, "num_material %d out of range [1, %d]", g->num_material, VCBCT_MAX_MAT);
+    if (!(g->recon_scale > 0) || !(g->recon_step > 0) || !(g->sad_cm > 0) || !(g->det_pitch_cm > 0) ||
+        !(g->det_pitch_iso_cm > 0) || !(g->det_pixel_num > 0))
+        return fail(VCBCT_EINVAL, "geometry pitches and distances must be positive");
+    double vox = (double)g->recon_size * g->recon_size * g->recon_slices;
+    if (vox * 32.0 > 200e9) return fail(VCBCT_EINVAL, "volume too large for one device");
+    return 0;
+}
+
+}  // namespace vcbct
+
+using namespace vcbct;
+
+extern "C" const char* vcbct_last_error(void) { return g_err; }
+
+extern "C" const char* vcbct_version(void) { return "vcbct 0.1 (gfx950)"; }
+
+extern "C" int vcbct_projector_create(vcbct_projector** out, const vcbct_geometry* g, const double* cos_t,
+                                      const double* sin_t, const double* xc, const double* yc, int device) {
+    if (!out) return fail(VCBCT_EINVAL, "out is NULL");
+    *out = nullptr;
+    int rc = check_geometry(g);
+    if (rc) return rc;
+    if (!cos_t || !sin_t || !xc || !yc) return fail(VCBCT_EINVAL, "angle tables are NULL");
+    int ndev = 0;
+    VCBCT_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(VCBCT_EINVAL, "device %d not present (%d devices)", device, ndev);
+    VCBCT_HIP(hipSetDevice(device));
+    vcbct_projector* p = (vcbct_projector*)calloc(1, sizeof(vcbct_projector));
+    if (!p) return fail(VCBCT_ENOMEM, "host allocation failed");
+    p->device = device;
+    ProjGeom& G = p->g;
+    G.N = g->recon_size;
+    G.Nz = g->recon_slices;
+    G.H = g->proj_height;
+    G.W = g->proj_width;
+    G.nAng = g->n_angles;
+    G.nmat = g->num_material;
+    G.s = g->recon_scale;
+    G.t = g->recon_step;
+    G.x_reg = g->x_reg;
+    G.y_reg = g->y_reg;
+    G.z_reg = g->z_reg;
+    G.pitch_iso = g->det_pitch_iso_cm;
+    G.pitch = g->det_pitch_cm;
+    G.sad = g->sad_cm;
+    G.pixnum = g->det_pixel_num;
+    std::vector<double> tab(4 * (size_t)G.nAng);
+    memcpy(tab.data(), cos_t, G.nAng * sizeof(double));
+    memcpy(tab.data() + G.nAng, sin_t, G.nAng * sizeof(double));
+    memcpy(tab.data() + 2 * G.nAng, xc, G.nAng * sizeof(double));
+    memcpy(tab.data() + 3 * G.nAng, yc, G.nAng * sizeof(double));
+    if (dev_malloc((void**)&p->d_tables, tab.size() * sizeof(double)) != hipSuccess) {
+        free(p);
+        return fail(VCBCT_ENOMEM, "device allocation of angle tables failed");
+    }
+    if (hipMemcpy(p->d_tables, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p->d_tables);
+        free(p);
+        return fail(VCBCT_EHIP, "copy of angle tables failed");
+    }
+    *out = p;
+    return 0;
+}
+
+extern "C" void vcbct_projector_destroy(vcbct_projector* p) {
+    if (!p) return;
+    (void)hipSetDevice(p->device);
+    (void)hipFree(p->d_tables);
+    (void)hipFree(p->d_vox_u16);
+    (void)hipFree(p->d_vox_f32);
+    (void)hipFree(p->d_grad_vox);
+    (void)hipFree(p->d_mu);
+    (void)hipFree(p->d_pdf);
+    free(p);
+}
+
+static size_t vox_count(const ProjGeom& g) { return (size_t)g.N * g.N * g.Nz; }
+
+extern "C" int vcbct_projector_set_volume_u16(vcbct_projector* p, const uint16_t* vol, int is_device, void* stream) {
+    if (!p || !vol) return fail(VCBCT_EINVAL, "set_volume_u16: NULL argument");
+    VCBCT_HIP(hipSetDevice(p->device));
+    hipStream_t st = (hipStream_t)stream;
+    const size_t nv = vox_count(p->g);
+    if (!p->d_vox_u16 && dev_malloc((void**)&p->d_vox_u16, nv * 16) != hipSuccess)
+        return fail(VCBCT_ENOMEM, "device allocation of the volume (%zu bytes) failed", nv * 16);
+    const uint16_t* src = vol;
+    uint16_t* stage = nullptr;
+    const size_t raw = nv * p->g.nmat * sizeof(uint16_t);
+    if (!is_device) {
+        if (dev_malloc((void**)&stage, raw) != hipSuccess)
+            return fail(VCBCT_ENOMEM, "device allocation of the upload stage (%zu bytes) failed", raw);
+        hipError_t e = hipMemcpyAsync(stage, vol, raw, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) {
+            (void)hipFree(stage);
+            return fail(VCBCT_EHIP, "volume upload failed: %s", hipGetErrorString(e));
+        }
+        src = stage;
+    }
+    int rc = launch_transpose_u16(src, p->d_vox_u16, p->g, st);
+    if (stage) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(stage);
+    }
+    if (rc) return rc;
+    p->have_u16 = 1;
+    return 0;
+}
+
+extern "C" int vcbct_projector_set_volume_f32(vcbct_projector* p, const float* dev_maps, void* stream) {
+    if (!p || !dev_maps) return fail(VCBCT_EINVAL, "set_volume_f32: NULL argument");
+    VCBCT_HIP(hipSetDevice(p->device));
+    const size_t nv = vox_count(p->g);
+    if (!p->d_vox_f32 && dev_malloc((void**)&p->d_vox_f32, nv * 32) != hipSuccess)
+        return fail(VCBCT_ENOMEM, "device allocation of the FP32 volume (%zu bytes) failed", nv * 32);
+    int rc = launch_transpose_f32(dev_maps, p->d_vox_f32, p->g, (hipStream_t)stream);
+    if (rc) return rc;
+    p->have_f32 = 1;
+    return 0;
+}
+
+extern "C" int vcbct_projector_set_spectrum(vcbct_projector* p, const float* atten, const float* pdf, int NE) {
+    if (!p || !atten || !pdf) return fail(VCBCT_EINVAL, "set_spectrum: NULL argument");
+    if (NE < 1 || NE > VCBCT_MAX_NE) return fail(VCBCT_EINVAL, "NE %d out of range [1, %d]", NE, VCBCT_MAX_NE);
+    VCBCT_HIP(hipSetDevice(p->device));
+    const int nm = p->g.nmat;
+    std::vector<double> mu((size_t)nm * NE), pd(NE);
+    for (size_t i = 0; i < mu.size(); ++i) mu[i] = atten[i];
+    for (int e = 0; e < NE; ++e) pd[e] = pdf[e];
+    if (NE != p->NE) {
+        (void)hipFree(p->d_mu);
+        (void)hipFree(p->d_pdf);
+        p->d_mu = nullptr;
+        p->d_pdf = nullptr;
+        if (dev_malloc((void**)&p->d_mu, mu.size() * sizeof(double)) != hipSuccess ||
+            dev_malloc((void**)&p->d_pdf, pd.size() * sizeof(double)) != hipSuccess)
+            return fail(VCBCT_ENOMEM, "device allocation of the spectrum failed");
+    }
+    VCBCT_HIP(hipMemcpy(p->d_mu, mu.data(), mu.size() * sizeof(double), hipMemcpyHostToDevice));
+    VCBCT_HIP(hipMemcpy(p->d_pdf, pd.data(), pd.size() * sizeof(double), hipMemcpyHostToDevice));
+    p->NE = NE;
+    return 0;
+}
+
+static int need(const vcbct_projector* p, bool u16, bool f32, bool spec) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    if (u16 && !p->have_u16) return fail(VCBCT_ESTATE, "no uint16 volume set (vcbct_projector_set_volume_u16)");
+    if (f32 && !p->have_f32) return fail(VCBCT_ESTATE, "no FP32 volume set (vcbct_projector_set_volume_f32)");
+    if (spec && p->NE < 1) return fail(VCBCT_ESTATE, "no spectrum set (vcbct_projector_set_spectrum)");
+    return 0;
+}
+
+static Spectrum spec_of(const vcbct_projector* p) {
+    Spectrum s;
+    s.mu = p->d_mu;
+    s.pdf = p->d_pdf;
+    s.NE = p->NE;
+    return s;
+}
+
+extern "C" int vcbct_projector_forward(vcbct_projector* p, float* dev_out, double xfactor, int a0, int a1, void* stream) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    const bool use_f32 = !p->have_u16 && p->have_f32;
+    int rc = need(p, !use_f32, use_f32, true);
+    if (rc) return rc;
+    if (!dev_out) return fail(VCBCT_EINVAL, "forward: output is NULL");
+    if (!(xfactor > 0) || !std::isfinite(xfactor)) return fail(VCBCT_EINVAL, "xfactor must be positive");
+    if (a1 < 0) a1 = p->g.nAng;
+    if (a0 < 0 || a0 > a1 || a1 > p->g.nAng) return fail(VCBCT_EINVAL, "angle range [%d, %d) invalid", a0, a1);
+    VCBCT_HIP(hipSetDevice(p->device));
+    return launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
+                          spec_of(p), xfactor, a0, a1, dev_out, VCBCT_MODE_INTENSITY, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int vcbct_projector_line_integrals(vcbct_projector* p, float* dev_L, void* stream) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    const bool use_f32 = !p->have_u16 && p->have_f32;
+    int rc = need(p, !use_f32, use_f32, false);
+    if (rc) return rc;
+    if (!dev_L) return fail(VCBCT_EINVAL, "line_integrals: output is NULL");
+    VCBCT_HIP(hipSetDevice(p->device));
+    Spectrum sp{nullptr, nullptr, 0};
+    return launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
+                          sp, 1.0, 0, p->g.nAng, nullptr, VCBCT_MODE_LINEINT, dev_L, (hipStream_t)stream);
+}
+
+extern "C" int vcbct_reprojection(vcbct_projector* p, const uint16_t* host_vol, const float* atten, const float* pdf,
+                                  int NE, double xfactor, float* host_out) {
+    if (!p || !host_vol || !host_out) return fail(VCBCT_EINVAL, "reprojection: NULL argument");
+    int rc = vcbct_projector_set_volume_u16(p, host_vol, 0, nullptr);
+    if (rc) return rc;
+    rc = vcbct_projector_set_spectrum(p, atten, pdf, NE);
+    if (rc) return rc;
+    const size_t n = (size_t)p->g.nAng * p->g.H * p->g.W;
+    float* d_out = nullptr;
+    if (dev_malloc((void**)&d_out, n * sizeof(float)) != hipSuccess)
+        return fail(VCBCT_ENOMEM, "device allocation of the projections (%zu bytes) failed", n * sizeof(float));
+    rc = vcbct_projector_forward(p, d_out, xfactor, 0, -1, nullptr);
+    if (rc == 0) {
+        hipError_t e = hipMemcpy(host_out, d_out, n * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(VCBCT_EHIP, "projection download failed: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(d_out);
+    return rc;
+}
+
+extern "C" int vcbct_projector_logproj(vcbct_projector* p, float* dev_p, double xfactor, void* stream) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    const bool use_f32 = p->have_f32;
+    int rc = need(p, !use_f32, use_f32, true);
+    if (rc) return rc;
+    if (!dev_p) return fail(VCBCT_EINVAL, "logproj: output is NULL");
+    if (!(xfactor > 0)) return fail(VCBCT_EINVAL, "xfactor must be positive");
+    VCBCT_HIP(hipSetDevice(p->device));
+    return launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
+                          spec_of(p), xfactor, 0, p->g.nAng, dev_p, VCBCT_MODE_LOGPROJ, nullptr, (hipStream_t)stream);
+}
+
+static int run_adjoint(vcbct_projector* p, const float* tg, float* dev_grad, double* dev_loss, double xfactor, int mode,
+                       void* stream) {
+    int rc = need(p, false, true, true);
+    if (rc) return rc;
+    if (!tg || !dev_grad) return fail(VCBCT_EINVAL, "adjoint: NULL argument");
+    if (!(xfactor > 0)) return fail(VCBCT_EINVAL, "xfactor must be positive");
+    VCBCT_HIP(hipSetDevice(p->device));
+    hipStream_t st = (hipStream_t)stream;
+    const size_t nv = vox_count(p->g);
+    if (!p->d_grad_vox && dev_malloc((void**)&p->d_grad_vox, nv * 32) != hipSuccess)
+        return fail(VCBCT_ENOMEM, "device allocation of the gradient volume failed");
+    // zero with a fill kernel, not hipMemsetAsync, so the call stays safe under stream capture
+    rc = launch_zero(p->d_grad_vox, nv * 32, st);
+    if (rc) return rc;
+    if (dev_loss && (rc = launch_zero(dev_loss, sizeof(double), st))) return rc;
+    rc = launch_adjoint(p->g, p->d_tables, p->d_vox_f32, spec_of(p), xfactor, tg, mode, p->d_grad_vox, dev_loss, st);
+    if (rc) return rc;
+    return launch_untranspose_f32(p->d_grad_vox, dev_grad, p->g, st);
+}
+
+extern "C" int vcbct_projector_adjoint(vcbct_projector* p, const float* dev_target, float* dev_grad, double* dev_loss,
+                                       double xfactor, void* stream) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    return run_adjoint(p, dev_target, dev_grad, dev_loss, xfactor, VCBCT_ADJ_LOSS, stream);
+}
+
+extern "C" int vcbct_projector_logproj_backward(vcbct_projector* p, const float* dev_gp, float* dev_grad, double xfactor,
+                                                void* stream) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    return run_adjoint(p, dev_gp, dev_grad, nullptr, xfactor, VCBCT_ADJ_UPSTREAM, stream);
+}
+
+extern "C" int vcbct_projector_segment_counts(vcbct_projector* p, float* dev_counts, void* stream) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    if (!dev_counts) return fail(VCBCT_EINVAL, "segment_counts: output is NULL");
+    VCBCT_HIP(hipSetDevice(p->device));
+    Spectrum sp{nullptr, nullptr, 0};
+    return launch_forward(p->g, p->d_tables, nullptr, 0, sp, 1.0, 0, p->g.nAng, nullptr, VCBCT_MODE_SEGCOUNT,
+                          dev_counts, (hipStream_t)stream);
+}

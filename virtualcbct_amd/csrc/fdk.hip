@@ -1,0 +1,385 @@
+// FDK on gfx950: ramp filter as a batched FFT convolution and voxel-driven cone-beam backprojection.
+//
+// Replaces filterSinogram (F/filterSinogram.cpp:122-186, a direct O(W^2) spatial convolution with
+// -log() inside the inner loop) and reconstructImage (F/reconstructImage.cpp:10-184, OpenMP over z).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <climits>
+#include <complex>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "vcbct_internal.h"
+
+namespace vcbct {
+
+static const double kPI = 3.141592653589793238462643383279;
+
+// ------------------------------------------------------------------------------------------------
+// Filter kernel h(d), F/filterSinogram.cpp:65-120 (off-centre taps) and :133-135 (centre coefficient).
+// FP64, the reference's operation order.
+static double ramp_offcentre(int md, double pitch_iso, int W) {
+    const double CUT = 1.60, GAM = 1.00, ALP = 0.50;
+    double theConst = 1.0 / (8.0 * kPI * kPI * pitch_iso * 800 / W);
+    double gamma_x_cutoffFreq = GAM * CUT;
+    double PI_x_cutoffFreq = kPI * CUT;
+    double one_minus_alpha = 1.0 - ALP;
+    double omega = kPI / GAM;
+    double cos_omega = cos(omega), sin_omega = sin(omega);
+    double MN = (double)md;
+    double theta = PI_x_cutoffFreq * MN;
+    double delta = gamma_x_cutoffFreq * MN;
+    double eta = 1.0 / (1.0 - 1.0 / delta);
+    double xi = 1.0 / (1.0 + 1.0 / delta);
+    double ct = cos(theta), st = sin(theta);
+    return theConst / MN / MN *
+           (2.0 * ALP * theta * st + 2.0 * ALP * (ct - 1.0) + xi * one_minus_alpha * theta * (st * cos_omega + ct * sin_omega) +
+            eta * one_minus_alpha * theta * (st * cos_omega - ct * sin_omega) +
+            xi * xi * one_minus_alpha * (ct * cos_omega - st * sin_omega - 1.0) +
+            eta * eta * one_minus_alpha * (ct * cos_omega + st * sin_omega - 1.0));
+}
+static double ramp_centre(int W, double pitch_iso) {
+    const double CUT = 1.60, GAM = 1.00, ALP = 0.50;
+    double omega = kPI / GAM;
+    return CUT * CUT / 4.0 / (pitch_iso * 800 / W) *
+           (ALP * 0.5 + (1.0 - ALP) * (1.0 / omega * sin(omega) + 1.0 / omega / omega * (cos(omega) - 1.0)));
+}
+
+// Host FP64 radix-2 FFT (used once per (W, pitch) to build the filter's frequency response).
+static void fft64(std::vector<std::complex<double>>& a) {
+    const size_t n = a.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        double ang = -2.0 * kPI / (double)len;
+        for (size_t i = 0; i < n; i += len)
+            for (size_t k = 0; k < len / 2; ++k) {
+                std::complex<double> w(cos(ang * k), sin(ang * k));
+                std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+                a[i + k] = u + v;
+                a[i + k + len / 2] = u - v;
+            }
+    }
+}
+
+struct FilterPlan {
+    int W, L, logL;
+    float2* d_H;    // [L] frequency response of the circularly wrapped h, scaled by 1/L
+    float2* d_tw;   // [L/2] exp(-2 pi i k / L)
+};
+
+static std::mutex g_plan_mu;
+static std::unordered_map<unsigned long long, FilterPlan> g_plans;  // key: device, W, pitch bits
+
+static int get_plan(int W, double pitch_iso, FilterPlan** out) {
+    int dev = 0;
+    VCBCT_HIP(hipGetDevice(&dev));
+    unsigned long long pb;
+    memcpy(&pb, &pitch_iso, 8);
+    unsigned long long key = ((unsigned long long)dev << 56) ^ ((unsigned long long)W << 40) ^ (pb * 0x9E3779B97F4A7C15ull);
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find(key);
+    if (it != g_plans.end() && it->second.W == W) {
+        *out = &it->second;
+        return 0;
+    }
+    FilterPlan P;
+    P.W = W;
+    P.L = 64;
+    P.logL = 6;
+    while (P.L < 2 * W - 1) { P.L <<= 1; P.logL++; }
+    if (P.L > 4096) return fail(VCBCT_EINVAL, "filter: projection width %d too large (max 2048)", W);
+    std::vector<std::complex<double>> h(P.L, 0.0);
+    h[0] = ramp_centre(W, pitch_iso);
+    for (int d = 1; d <= W - 1; ++d) {
+        h[d] = ramp_offcentre(d, pitch_iso, W);           // out[m] gets h(d) q[m-d]
+        h[P.L - d] = ramp_offcentre(-d, pitch_iso, W);
+    }
+    fft64(h);
+    std::vector<float2> Hf(P.L), tw(P.L / 2);
+    for (int k = 0; k < P.L; ++k) Hf[k] = make_float2((float)(h[k].real() / P.L), (float)(h[k].imag() / P.L));
+    for (int k = 0; k < P.L / 2; ++k) tw[k] = make_float2((float)cos(-2.0 * kPI * k / P.L), (float)sin(-2.0 * kPI * k / P.L));
+    VCBCT_HIP(dev_malloc((void**)&P.d_H, P.L * sizeof(float2)));
+    VCBCT_HIP(dev_malloc((void**)&P.d_tw, P.L / 2 * sizeof(float2)));
+    VCBCT_HIP(hipMemcpy(P.d_H, Hf.data(), P.L * sizeof(float2), hipMemcpyHostToDevice));
+    VCBCT_HIP(hipMemcpy(P.d_tw, tw.data(), P.L / 2 * sizeof(float2), hipMemcpyHostToDevice));
+    g_plans[key] = P;
+    *out = &g_plans[key];
+    return 0;
+}
+
+// One block filters two detector rows at once: z = q1 + i q2 (real kernel, so the two convolutions
+// separate into the real and imaginary parts), radix-2 Stockham FFT in LDS, product with H, inverse
+// FFT, cosine weight Cm at the output pixel.
+constexpr int FT = 256;
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+template <bool INV>
+__device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* __restrict__ tw, int L, int logL) {
+    // Stockham autosort, radix 2: stage s has stride Ns = 2^s
+    const int half = L >> 1;
+    for (int s = 0, Ns = 1; s < logL; ++s, Ns <<= 1) {
+        for (int j = threadIdx.x; j < half; j += FT) {
+            int k = j & (Ns - 1);                  // position inside the current sub-transform
+            float2 a = x[j], b = x[j + half];
+            float2 w = tw[k * (half / Ns)];        // exp(-2 pi i k / (2 Ns))
+            if (INV) w.y = -w.y;
+            float2 bw = cmul(b, w);
+            int o = (j - k) * 2 + k;               // (j / Ns) * 2 Ns + k
+            y[o] = make_float2(a.x + bw.x, a.y + bw.y);
+            y[o + Ns] = make_float2(a.x - bw.x, a.y - bw.y);
+        }
+        __syncthreads();
+        float2* t = x; x = y; y = t;
+    }
+    return x;
+}
+
+__global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, float* __restrict__ out, int H, int W,
+                                               long nrows, int L, int logL, const float2* __restrict__ Hf,
+                                               const float2* __restrict__ tw, const float* __restrict__ xc_tab,
+                                               const float* __restrict__ yc_tab, float pitchW) {
+    extern __shared__ float2 lds[];
+    float2* A = lds;
+    float2* B = lds + L;
+    const long r1 = 2 * (long)blockIdx.x, r2 = r1 + 1;
+    const bool has2 = r2 < nrows;
+    const float* p1 = proj + r1 * W;
+    const float* p2 = proj + r2 * W;
+    for (int k = threadIdx.x; k < L; k += FT) {
+        float a = 0.f, b = 0.f;
+        if (k < W) {
+            a = -logf(p1[k]);
+            if (has2) b = -logf(p2[k]);
+        }
+        A[k] = make_float2(a, b);
+    }
+    __syncthreads();
+    float2* X = fft_lds<false>(A, B, tw, L, logL);
+    float2* Y = (X == A) ? B : A;
+    for (int k = threadIdx.x; k < L; k += FT) X[k] = cmul(X[k], Hf[k]);
+    __syncthreads();
+    X = fft_lds<true>(X, Y, tw, L, logL);
+    // cosine weight at the output pixel (F/filterSinogram.cpp:141-154)
+    for (int s = 0; s < 2; ++s) {
+        const long r = r1 + s;
+        if (s == 1 && !has2) break;
+        const int th = (int)(r / H), n = (int)(r % H);
+        const float pos2 = fabsf(yc_tab[th] + (float)n) * pitchW;
+        for (int m = threadIdx.x; m < W; m += FT) {
+            float pos1 = fabsf(xc_tab[th] + (float)m) * pitchW;
+            float cm = 100.0f / sqrtf(10000.0f + pos1 * pos1 + pos2 * pos2);
+            float v = (s == 0) ? X[m].x : X[m].y;
+            out[r * W + m] = v * cm;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Backprojection.  Thread = one (x, y) voxel column, ZB consecutive slices; block = 16 x 16 columns.
+// Per angle the FP64 geometry (U, its interpolation weight and base column, 1/L2) is computed once
+// per column and reused for the ZB slices.  The interpolation follows F/reconstructImage.cpp:76-175:
+// the horizontal weight is frac(U) but the base column is floor(W/2 + U + xshift_px - 0.5), and the
+// three edge cases at the last column / row sample one tap pair or one tap.
+constexpr int ZB = 4;
+
+__global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, const float* __restrict__ sino, int R,
+                                                     int Z, double zstart, double step, double scale, int H, int W,
+                                                     int n, const double* __restrict__ atab, double xfactor,
+                                                     double pitchW) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int z0 = blockIdx.z * ZB;
+    if (x >= R || y >= R) return;
+    const double rad = (double)(R - 1) / 2.0;
+    if (((double)y - rad) * ((double)y - rad) + ((double)x - rad) * ((double)x - rad) > rad * rad) return;
+    const double centerPos = (double)R / 2.0 - 0.5;
+    const double xx = (x - centerPos) * scale;
+    const double yy = -(y - centerPos) * scale;
+    double zz[ZB];
+    float acc[ZB];
+#pragma unroll
+    for (int i = 0; i < ZB; ++i) {
+        zz[i] = zstart + (double)(z0 + i) * step;
+        acc[i] = 0.f;
+    }
+    const double hW = (double)(W / 2), hH = (double)(H / 2);
+    for (int th = 0; th < n; ++th) {
+        const double c = atab[th], s = atab[n + th], dbeta = atab[2 * n + th], xs = atab[3 * n + th];
+        double X = xx * c + yy * s;
+        double Y = -xx * s + yy * c;
+        double L2 = 1.0 - Y / 100.0;
+        double U = X / L2 / pitchW;
+        double fU = floor(U);
+        double w = U - fU;
+        int UU = (int)floor(hW + U + xs - 0.5);
+        double iL = 1.0 / L2 / pitchW;
+        float wgt = (float)(1.0 / L2 / L2 * dbeta * xfactor);
+        const bool ucol = (UU >= 0 && UU < W - 1), uedge = (UU == W - 1);
+        if (!ucol && !uedge) continue;
+        const float* P = sino + (size_t)th * H * W;
+        const float wf = (float)w;
+#pragma unroll
+        for (int i = 0; i < ZB; ++i) {
+            double ZETA = hH + zz[i] * iL;
+            double fz = floor(ZETA);
+            int NZ = (int)fz;
+            float wz = (float)(ZETA - fz);
+            float sum;
+            if (NZ >= 0 && NZ < H - 1) {
+                const float* r0 = P + (size_t)NZ * W + UU;
+                if (ucol) {
+                    float a0 = r0[0], a1 = r0[1], b0 = r0[W], b1 = r0[W + 1];
+                    sum = (a0 * (1.f - wf) + a1 * wf) * (1.f - wz) + (b0 * (1.f - wf) + b1 * wf) * wz;
+                } else {
+                    sum = r0[0] * (1.f - wz) + r0[W] * wz;
+                }
+            } else if (NZ == H - 1) {
+                const float* r0 = P + (size_t)NZ * W + UU;
+                sum = ucol ? (r0[0] * (1.f - wf) + r0[1] * wf) : r0[0];
+            } else {
+                continue;
+            }
+            acc[i] += sum * wgt;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < ZB; ++i)
+        if (z0 + i < Z) recon[((size_t)(z0 + i) * R + y) * R + x] += acc[i];
+}
+
+}  // namespace vcbct
+
+using namespace vcbct;
+
+extern "C" int vcbct_prepare_const_map(int W, double pitch_iso, double* out) {
+    if (W < 1 || !out) return fail(VCBCT_EINVAL, "prepare_const_map: bad arguments");
+    for (int m = 0; m < W; ++m)
+        for (int n = 0; n < W; ++n) out[(size_t)m * W + n] = (m == n) ? 0.0 : ramp_offcentre(m - n, pitch_iso, W);
+    return 0;
+}
+
+extern "C" int vcbct_filter_sinogram(int H, int W, int n, const float* dev_proj, float* dev_out, const double* xshift,
+                                     const double* yshift, double pitch_iso, void* stream) {
+    if (H < 1 || W < 2 || n < 1 || !dev_proj || !dev_out || !xshift || !yshift)
+        return fail(VCBCT_EINVAL, "filter_sinogram: bad arguments");
+    FilterPlan* P;
+    int rc = get_plan(W, pitch_iso, &P);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    // per-projection centre offsets, F/filterSinogram.cpp:141-142 (both with the projection WIDTH)
+    std::vector<float> tab(2 * n);
+    for (int th = 0; th < n; ++th) {
+        tab[th] = (float)(-W / 2.0 - xshift[th] / (0.05 * 800 / W) - 1);
+        tab[n + th] = (float)(-W / 2.0 - yshift[th] / (0.05 * 800 / W) - 1);
+    }
+    float* d_tab;
+    VCBCT_HIP(dev_malloc((void**)&d_tab, 2 * n * sizeof(float)));
+    VCBCT_HIP(hipMemcpyAsync(d_tab, tab.data(), 2 * n * sizeof(float), hipMemcpyHostToDevice, st));
+    const long nrows = (long)n * H;
+    const long nblk = (nrows + 1) / 2;
+    if (nblk > INT_MAX) return fail(VCBCT_EINVAL, "filter_sinogram: too many rows");
+    const float pitchW = (float)(pitch_iso * 800 / W);
+    hipLaunchKernelGGL(k_filter, dim3((unsigned)nblk), dim3(FT), 2 * P->L * sizeof(float2), st, dev_proj, dev_out, H, W,
+                       nrows, P->L, P->logL, P->d_H, P->d_tw, d_tab, d_tab + n, pitchW);
+    VCBCT_HIP(hipGetLastError());
+    VCBCT_HIP(hipStreamSynchronize(st));
+    VCBCT_HIP(hipFree(d_tab));
+    return 0;
+}
+
+extern "C" int vcbct_reconstruct_image(int R, int Z, int zstart, double step, double scale, float* dev_recon, int H,
+                                       int W, int n, const float* dev_sino, int xfactor, const double* angles,
+                                       const double* xshift, double pitch_iso, void* stream) {
+    if (R < 1 || Z < 1 || H < 2 || W < 2 || n < 2 || !dev_recon || !dev_sino || !angles || !xshift)
+        return fail(VCBCT_EINVAL, "reconstruct_image: bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    // per-angle FP64 table: cos, sin, dbeta (F/reconstructImage.cpp:44-71), xshift in detector pixels
+    std::vector<double> tab(4 * (size_t)n);
+    for (int th = 0; th < n; ++th) {
+        double dbeta, thdb0, thdb1, thdb2;
+        if (th == 0) {
+            thdb1 = angles[th];
+            thdb2 = angles[th + 1];
+            dbeta = fabs(thdb2 - thdb1) / 2;
+        } else if (th == n - 1) {
+            thdb0 = angles[th - 1];
+            thdb1 = angles[th];
+            dbeta = fabs(thdb1 - thdb0) / 2;
+        } else {
+            thdb0 = angles[th - 1];
+            thdb2 = angles[th + 1];
+            thdb1 = angles[th + 1] - 2 * kPI;
+            if (thdb2 < 0) thdb1 = angles[th + 1] + 2 * kPI;
+            dbeta = fabs(thdb2 - thdb0) / 2;
+            if (dbeta > kPI / 2) dbeta = fabs(thdb1 - thdb0) / 2;
+        }
+        tab[th] = cos(angles[th]);
+        tab[n + th] = sin(angles[th]);
+        tab[2 * n + th] = dbeta;
+        tab[3 * n + th] = xshift[th] / (0.05 * 800 / W);
+    }
+    double* d_tab;
+    VCBCT_HIP(dev_malloc((void**)&d_tab, tab.size() * sizeof(double)));
+    VCBCT_HIP(hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    dim3 grid((R + 15) / 16, (R + 15) / 16, (Z + ZB - 1) / ZB);
+    hipLaunchKernelGGL(k_backproject, grid, dim3(256), 0, st, dev_recon, dev_sino, R, Z, (double)zstart, step, scale, H,
+                       W, n, d_tab, (double)xfactor, pitch_iso * 800 / W);
+    VCBCT_HIP(hipGetLastError());
+    VCBCT_HIP(hipStreamSynchronize(st));
+    VCBCT_HIP(hipFree(d_tab));
+    return 0;
+}
+
+extern "C" int vcbct_filter_sinogram_host(int H, int W, int n, const float* hp, float* ho, const double* xs,
+                                          const double* ys, double pitch_iso) {
+    if (!hp || !ho) return fail(VCBCT_EINVAL, "filter_sinogram_host: NULL argument");
+    const size_t bytes = (size_t)n * H * W * sizeof(float);
+    float *dp = nullptr, *dq = nullptr;
+    if (dev_malloc((void**)&dp, bytes) != hipSuccess || dev_malloc((void**)&dq, bytes) != hipSuccess) {
+        (void)hipFree(dp);
+        return fail(VCBCT_ENOMEM, "device allocation of %zu bytes failed", 2 * bytes);
+    }
+    int rc = 0;
+    if (hipMemcpy(dp, hp, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = fail(VCBCT_EHIP, "upload failed");
+    if (!rc) rc = vcbct_filter_sinogram(H, W, n, dp, dq, xs, ys, pitch_iso, nullptr);
+    if (!rc && hipMemcpy(ho, dq, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(VCBCT_EHIP, "download failed");
+    (void)hipFree(dp);
+    (void)hipFree(dq);
+    return rc;
+}
+
+extern "C" int vcbct_reconstruct_image_host(int R, int Z, int zstart, double step, double scale, double* hrec, int H,
+                                            int W, int n, const float* hs, int xfactor, const double* angles,
+                                            const double* xs, double pitch_iso) {
+    if (!hrec || !hs) return fail(VCBCT_EINVAL, "reconstruct_image_host: NULL argument");
+    const size_t sb = (size_t)n * H * W * sizeof(float), vn = (size_t)Z * R * R;
+    float *ds = nullptr, *dr = nullptr;
+    if (dev_malloc((void**)&ds, sb) != hipSuccess || dev_malloc((void**)&dr, vn * sizeof(float)) != hipSuccess) {
+        (void)hipFree(ds);
+        return fail(VCBCT_ENOMEM, "device allocation failed");
+    }
+    int rc = 0;
+    if (hipMemcpy(ds, hs, sb, hipMemcpyHostToDevice) != hipSuccess) rc = fail(VCBCT_EHIP, "upload failed");
+    if (!rc) rc = launch_zero(dr, vn * sizeof(float), nullptr);
+    if (!rc) rc = vcbct_reconstruct_image(R, Z, zstart, step, scale, dr, H, W, n, ds, xfactor, angles, xs, pitch_iso, nullptr);
+    std::vector<float> tmp;
+    if (!rc) {
+        tmp.resize(vn);
+        if (hipMemcpy(tmp.data(), dr, vn * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(VCBCT_EHIP, "download failed");
+    }
+    if (!rc)
+        for (size_t i = 0; i < vn; ++i) hrec[i] += tmp[i];
+    (void)hipFree(ds);
+    (void)hipFree(dr);
+    return rc;
+}

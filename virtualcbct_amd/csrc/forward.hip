@@ -1,0 +1,410 @@
+// Forward projector kernels: volume re-layout, energy-collapsed polychromatic projection, line
+// integrals, log projection, and the adjoint (transposed traversal) — gfx950.
+//
+// Replaces reprojectionKernel and the host energy loop of reprojectionOnGPU
+// (P/reprojectionOnGPU.cu:56-690, 1004-1056): the reference recomputes the whole traversal once per
+// energy bin and copies every projection to the host after each bin; here one traversal per ray yields
+// the material line integrals L_m, and the spectral sum runs in the same kernel.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+
+#include "traverse.h"
+#include "vcbct_internal.h"
+
+namespace vcbct {
+
+// ------------------------------------------------------------------------------------------------
+// Volume re-layout.  Reference u16 [z][mat][y][x] -> device [y][x][z][8] (16-byte voxel, all materials;
+// the 64 rows of a wave read one contiguous run of a z column).  Tiled through LDS: a block reads a
+// 32(x) x 32(z) tile of every material plane coalesced along x and writes 32 z-runs of 16-byte voxels.
+constexpr int TT = 32;
+
+__global__ __launch_bounds__(256) void k_transpose_u16(const uint16_t* __restrict__ src, uint4* __restrict__ dst,
+                                                       int N, int Nz, int nmat) {
+    __shared__ uint16_t tile[VCBCT_MAX_MAT][TT][TT + 2];
+    const int x0 = blockIdx.x * TT, z0 = blockIdx.y * TT, y = blockIdx.z;
+    const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;  // 32 x 8
+    const size_t n2 = (size_t)N * N;
+    for (int m = 0; m < nmat; ++m)
+        for (int zz = ty; zz < TT; zz += 8) {
+            int z = z0 + zz, x = x0 + tx;
+            uint16_t v = 0;
+            if (z < Nz && x < N) v = src[((size_t)z * nmat + m) * n2 + (size_t)y * N + x];
+            tile[m][zz][tx] = v;
+        }
+    __syncthreads();
+    // write: thread -> (x = x0 + i / TT, z = z0 + i % TT), z fastest
+    for (int i = threadIdx.x; i < TT * TT; i += 256) {
+        int xx = i / TT, zz = i % TT;
+        int x = x0 + xx, z = z0 + zz;
+        if (x >= N || z >= Nz) continue;
+        uint16_t w[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) w[m] = (m < nmat) ? tile[m][zz][xx] : (uint16_t)0;
+        uint4 o;
+        o.x = (uint32_t)w[0] | ((uint32_t)w[1] << 16);
+        o.y = (uint32_t)w[2] | ((uint32_t)w[3] << 16);
+        o.z = (uint32_t)w[4] | ((uint32_t)w[5] << 16);
+        o.w = (uint32_t)w[6] | ((uint32_t)w[7] << 16);
+        dst[((size_t)y * N + x) * Nz + z] = o;
+    }
+}
+
+// FP32 maps [mat][z][y][x] -> [y][x][z][8] float (two float4 per voxel).
+__global__ __launch_bounds__(256) void k_transpose_f32(const float* __restrict__ src, float4* __restrict__ dst, int N,
+                                                       int Nz, int nmat) {
+    __shared__ float tile[VCBCT_MAX_MAT][TT][TT + 1];
+    const int x0 = blockIdx.x * TT, z0 = blockIdx.y * TT, y = blockIdx.z;
+    const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
+    const size_t n2 = (size_t)N * N, n3 = n2 * Nz;
+    for (int m = 0; m < nmat; ++m)
+        for (int zz = ty; zz < TT; zz += 8) {
+            int z = z0 + zz, x = x0 + tx;
+            float v = 0.f;
+            if (z < Nz && x < N) v = src[(size_t)m * n3 + (size_t)z * n2 + (size_t)y * N + x];
+            tile[m][zz][tx] = v;
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TT * TT; i += 256) {
+        int xx = i / TT, zz = i % TT;
+        int x = x0 + xx, z = z0 + zz;
+        if (x >= N || z >= Nz) continue;
+        float w[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) w[m] = (m < nmat) ? tile[m][zz][xx] : 0.f;
+        size_t o = (((size_t)y * N + x) * Nz + z) * 2;
+        dst[o] = make_float4(w[0], w[1], w[2], w[3]);
+        dst[o + 1] = make_float4(w[4], w[5], w[6], w[7]);
+    }
+}
+
+// [y][x][z][8] float -> [mat][z][y][x]
+__global__ __launch_bounds__(256) void k_untranspose_f32(const float4* __restrict__ src, float* __restrict__ dst, int N,
+                                                         int Nz, int nmat) {
+    __shared__ float tile[VCBCT_MAX_MAT][TT][TT + 1];
+    const int x0 = blockIdx.x * TT, z0 = blockIdx.y * TT, y = blockIdx.z;
+    const size_t n2 = (size_t)N * N, n3 = n2 * Nz;
+    for (int i = threadIdx.x; i < TT * TT; i += 256) {
+        int xx = i / TT, zz = i % TT;
+        int x = x0 + xx, z = z0 + zz;
+        float w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (x < N && z < Nz) {
+            size_t o = (((size_t)y * N + x) * Nz + z) * 2;
+            float4 a = src[o], b = src[o + 1];
+            w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w; w[4] = b.x; w[5] = b.y;
+        }
+#pragma unroll
+        for (int m = 0; m < VCBCT_MAX_MAT; ++m) tile[m][zz][xx] = w[m];
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
+    for (int m = 0; m < nmat; ++m)
+        for (int zz = ty; zz < TT; zz += 8) {
+            int z = z0 + zz, x = x0 + tx;
+            if (z < Nz && x < N) dst[(size_t)m * n3 + (size_t)z * n2 + (size_t)y * N + x] = tile[m][zz][tx];
+        }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Voxel access
+
+__device__ __forceinline__ void load_vox(const uint4* __restrict__ v, size_t idx, double out[VCBCT_MAX_MAT]) {
+    uint4 q = v[idx];
+    out[0] = (double)(q.x & 0xffffu);
+    out[1] = (double)(q.x >> 16);
+    out[2] = (double)(q.y & 0xffffu);
+    out[3] = (double)(q.y >> 16);
+    out[4] = (double)(q.z & 0xffffu);
+    out[5] = (double)(q.z >> 16);
+}
+__device__ __forceinline__ void load_vox(const float4* __restrict__ v, size_t idx, double out[VCBCT_MAX_MAT]) {
+    float4 a = v[2 * idx], b = v[2 * idx + 1];
+    out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w; out[4] = b.x; out[5] = b.y;
+}
+
+template <class VoxT>
+struct AccumL {
+    const VoxT* __restrict__ vox;
+    int N, Nz, nmat;
+    double L[VCBCT_MAX_MAT];
+    __device__ __forceinline__ void seg(int xmc, int ymc, int zmc, double rr) {
+        double w[VCBCT_MAX_MAT];
+        load_vox(vox, ((size_t)ymc * N + xmc) * Nz + zmc, w);
+#pragma unroll
+        for (int m = 0; m < VCBCT_MAX_MAT; ++m) L[m] += w[m] * rr;  // padded materials are zero
+    }
+};
+
+// Counts the charged segments of a ray (the algorithmic work unit of the roofline model).
+struct CountSeg {
+    int n = 0;
+    __device__ __forceinline__ void seg(int, int, int, double) { ++n; }
+};
+
+// Work decomposition shared by the ray kernels.  A block = 4 waves = 4 adjacent detector columns of one
+// (row group, angle); row group g covers rows [64 g, 64 g + 64).  Blocks are dealt round-robin to the 8
+// XCDs; block b runs on XCD b % 8, so the column range of every (row group, angle) is split into 8
+// contiguous chunks, one per XCD: all XCDs work on the same angle (one z band of the volume in the
+// shared last-level cache) and each XCD's L2 sees adjacent columns.
+struct Tile {
+    int rg, a, col;
+};
+__device__ __forceinline__ Tile tile_of(int nA, int W, int quads_per_xcd) {
+    const int b = blockIdx.x;
+    const int xcd = b & 7, local = b >> 3;
+    const int q = local % quads_per_xcd;
+    const int ra = local / quads_per_xcd;  // rg * nA + a
+    Tile t;
+    t.a = ra % nA;
+    t.rg = ra / nA;
+    t.col = ((xcd * quads_per_xcd + q) << 2) + (threadIdx.x >> 6);
+    return t;
+}
+
+template <int MODE, class VoxT>
+__global__ __launch_bounds__(256) void k_forward(ProjGeom g, const double* __restrict__ tab, const VoxT* __restrict__ vox,
+                                                 const double* __restrict__ mu, const double* __restrict__ pdf, int NE,
+                                                 double inv_xf, int a0, int nA, int quads_per_xcd,
+                                                 float* __restrict__ out, float* __restrict__ aux) {
+    const Tile t = tile_of(nA, g.W, quads_per_xcd);
+    const int lane = threadIdx.x & 63;
+    const int j = t.rg * 64 + lane;
+    const int jj = t.col;
+    const int a = a0 + t.a;
+    if (jj >= g.W) return;  // whole wave (column) out of range: uniform exit
+    const bool active = j < g.H;
+
+    RayCtx r;
+    ray_setup(g, tab[a], tab[g.nAng + a], tab[2 * g.nAng + a], tab[3 * g.nAng + a], active ? j : 0, jj, r);
+
+    const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - j) * g.W + jj;
+    if (MODE == VCBCT_MODE_SEGCOUNT) {
+        CountSeg cnt;
+        traverse(g, r, active, cnt);
+        if (active) aux[o] = (float)cnt.n;
+        return;
+    }
+    AccumL<VoxT> acc;
+    acc.vox = vox;
+    acc.N = g.N;
+    acc.Nz = g.Nz;
+    acc.nmat = g.nmat;
+#pragma unroll
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0;
+    traverse(g, r, active, acc);
+    if (!active) return;
+
+    if (MODE == VCBCT_MODE_LINEINT) {
+        for (int m = 0; m < g.nmat; ++m) aux[o * g.nmat + m] = (float)acc.L[m];
+        return;
+    }
+    bool zero = true;
+#pragma unroll
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m) zero = zero && (acc.L[m] == 0.0);
+    if (MODE == VCBCT_MODE_INTENSITY) {
+        // sum_e float(pdf_e * exp(-transp_e / xf) * leng), FP32 accumulation in e order
+        // (P/reprojectionOnGPU.cu:432, 1038-1044)
+        float s = 0.0f;
+        if (zero) {
+            for (int e = 0; e < NE; ++e) s += (float)(pdf[e] * 1.0 * r.leng);
+        } else {
+            for (int e = 0; e < NE; ++e) {
+                double tr = 0.0;
+#pragma unroll
+                for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+                    if (m < g.nmat) tr += mu[m * NE + e] * acc.L[m];
+                s += (float)(pdf[e] * exp(-tr * inv_xf) * r.leng);
+            }
+        }
+        out[o] = s;
+    } else {  // VCBCT_MODE_LOGPROJ: p = -log(sum_e pdf_e e^{-mu.L/xf} / sum_e pdf_e), FP64
+        double sw = 0.0, se = 0.0;
+        for (int e = 0; e < NE; ++e) {
+            double tr = 0.0;
+#pragma unroll
+            for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+                if (m < g.nmat) tr += mu[m * NE + e] * acc.L[m];
+            sw += pdf[e];
+            se += pdf[e] * exp(-tr * inv_xf);
+        }
+        out[o] = (float)(-log(se / sw));
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Adjoint: recompute L_m for the ray (first traversal), form the per-material coefficient
+//   c_m = w_ray * mubar_m / xf,  mubar_m = sum_e pdf_e e_e mu_me / sum_e pdf_e e_e,
+// with w_ray = 2 (p - target) (loss mode) or the upstream gradient dL/dp (upstream mode), then scatter
+// c_m * rr into the gradient volume along the same traversal (second traversal, FP32 atomics into the
+// [y][x][z][8] layout, so the six materials of a voxel share one 32-byte sector).
+struct ScatterG {
+    float* __restrict__ gvox;
+    int N, Nz, nmat;
+    float c[VCBCT_MAX_MAT];
+    __device__ __forceinline__ void seg(int xmc, int ymc, int zmc, double rr) {
+        float* p = gvox + (((size_t)ymc * N + xmc) * Nz + zmc) * 8;
+        float rf = (float)rr;
+#pragma unroll
+        for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+            if (m < nmat && c[m] != 0.f) atomicAdd(p + m, c[m] * rf);
+    }
+};
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __restrict__ tab, const float4* __restrict__ vox,
+                                                 const double* __restrict__ mu, const double* __restrict__ pdf, int NE,
+                                                 double inv_xf, int quads_per_xcd, const float* __restrict__ tg,
+                                                 float* __restrict__ gvox, double* __restrict__ loss) {
+    const Tile t = tile_of(g.nAng, g.W, quads_per_xcd);
+    const int lane = threadIdx.x & 63;
+    const int j = t.rg * 64 + lane;
+    const int jj = t.col;
+    const int a = t.a;
+    if (jj >= g.W) return;
+    const bool active = j < g.H;
+    RayCtx r;
+    ray_setup(g, tab[a], tab[g.nAng + a], tab[2 * g.nAng + a], tab[3 * g.nAng + a], active ? j : 0, jj, r);
+    AccumL<float4> acc;
+    acc.vox = vox;
+    acc.N = g.N;
+    acc.Nz = g.Nz;
+    acc.nmat = g.nmat;
+#pragma unroll
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0;
+    traverse(g, r, active, acc);
+
+    ScatterG sc;
+    sc.gvox = gvox;
+    sc.N = g.N;
+    sc.Nz = g.Nz;
+    sc.nmat = g.nmat;
+    double lossc = 0.0;
+#pragma unroll
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m) sc.c[m] = 0.f;
+    if (active) {
+        double sw = 0.0, se = 0.0, mub[VCBCT_MAX_MAT] = {0, 0, 0, 0, 0, 0};
+        for (int e = 0; e < NE; ++e) {
+            double tr = 0.0;
+#pragma unroll
+            for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+                if (m < g.nmat) tr += mu[m * NE + e] * acc.L[m];
+            double we = pdf[e] * exp(-tr * inv_xf);
+            sw += pdf[e];
+            se += we;
+#pragma unroll
+            for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+                if (m < g.nmat) mub[m] += we * mu[m * NE + e];
+        }
+        const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - j) * g.W + jj;
+        double w;
+        if (MODE == VCBCT_ADJ_LOSS) {
+            double p = -log(se / sw);
+            double res = p - (double)tg[o];
+            lossc = res * res;
+            w = 2.0 * res;
+        } else {
+            w = (double)tg[o];
+        }
+#pragma unroll
+        for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+            if (m < g.nmat) sc.c[m] = (float)(w * (mub[m] / se) * inv_xf);
+    }
+    traverse(g, r, active, sc);
+    if (MODE == VCBCT_ADJ_LOSS && loss) {
+        // wave reduction of the loss, one atomic per wave
+        double v = lossc;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) atomicAdd(loss, v);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Launchers
+
+__global__ void k_zero(uint32_t* __restrict__ p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+int launch_zero(void* p, size_t bytes, hipStream_t st) {
+    size_t n = bytes / 4;
+    unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    if (nb == 0) return 0;
+    hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, st, (uint32_t*)p, n);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+
+static inline int n_blocks(const ProjGeom& g, int nA, int& qpx) {
+    const int quads = (g.W + 3) / 4;
+    qpx = (quads + 7) / 8;
+    const int rgs = (g.H + 63) / 64;
+    long nb = (long)rgs * nA * qpx * 8;
+    if (nb > INT_MAX) return -1;
+    return (int)nb;
+}
+
+int launch_transpose_u16(const uint16_t* src, uint16_t* dst, const ProjGeom& g, hipStream_t st) {
+    dim3 grid((g.N + TT - 1) / TT, (g.Nz + TT - 1) / TT, g.N);
+    hipLaunchKernelGGL(k_transpose_u16, grid, dim3(256), 0, st, src, (uint4*)dst, g.N, g.Nz, g.nmat);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+int launch_transpose_f32(const float* src, float* dst, const ProjGeom& g, hipStream_t st) {
+    dim3 grid((g.N + TT - 1) / TT, (g.Nz + TT - 1) / TT, g.N);
+    hipLaunchKernelGGL(k_transpose_f32, grid, dim3(256), 0, st, src, (float4*)dst, g.N, g.Nz, g.nmat);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+int launch_untranspose_f32(const float* src, float* dst, const ProjGeom& g, hipStream_t st) {
+    dim3 grid((g.N + TT - 1) / TT, (g.Nz + TT - 1) / TT, g.N);
+    hipLaunchKernelGGL(k_untranspose_f32, grid, dim3(256), 0, st, (const float4*)src, dst, g.N, g.Nz, g.nmat);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int vox_is_f32, const Spectrum& sp,
+                   double xfactor, int a0, int a1, float* out, int mode, float* aux, hipStream_t st) {
+    const int nA = a1 - a0;
+    if (nA <= 0) return 0;
+    int qpx;
+    int nb = n_blocks(g, nA, qpx);
+    if (nb < 0) return fail(VCBCT_EINVAL, "forward: launch too large");
+    const double inv_xf = 1.0 / xfactor;
+#define VCBCT_LAUNCH_FWD(M, T)                                                                              \
+    hipLaunchKernelGGL((k_forward<M, T>), dim3(nb), dim3(256), 0, st, g, tables, (const T*)vox, sp.mu, sp.pdf, \
+                       sp.NE, inv_xf, a0, nA, qpx, out, aux)
+    if (!vox_is_f32) {
+        if (mode == VCBCT_MODE_INTENSITY) VCBCT_LAUNCH_FWD(VCBCT_MODE_INTENSITY, uint4);
+        else if (mode == VCBCT_MODE_LINEINT) VCBCT_LAUNCH_FWD(VCBCT_MODE_LINEINT, uint4);
+        else if (mode == VCBCT_MODE_SEGCOUNT) VCBCT_LAUNCH_FWD(VCBCT_MODE_SEGCOUNT, uint4);
+        else VCBCT_LAUNCH_FWD(VCBCT_MODE_LOGPROJ, uint4);
+    } else {
+        if (mode == VCBCT_MODE_INTENSITY) VCBCT_LAUNCH_FWD(VCBCT_MODE_INTENSITY, float4);
+        else if (mode == VCBCT_MODE_LINEINT) VCBCT_LAUNCH_FWD(VCBCT_MODE_LINEINT, float4);
+        else VCBCT_LAUNCH_FWD(VCBCT_MODE_LOGPROJ, float4);
+    }
+#undef VCBCT_LAUNCH_FWD
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double xfactor,
+                   const float* target_or_gp, int mode, float* grad_vox, double* loss, hipStream_t st) {
+    int qpx;
+    int nb = n_blocks(g, g.nAng, qpx);
+    if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
+    const double inv_xf = 1.0 / xfactor;
+    if (mode == VCBCT_ADJ_LOSS)
+        hipLaunchKernelGGL(k_adjoint<VCBCT_ADJ_LOSS>, dim3(nb), dim3(256), 0, st, g, tables, (const float4*)vox, sp.mu,
+                           sp.pdf, sp.NE, inv_xf, qpx, target_or_gp, grad_vox, loss);
+    else
+        hipLaunchKernelGGL(k_adjoint<VCBCT_ADJ_UPSTREAM>, dim3(nb), dim3(256), 0, st, g, tables, (const float4*)vox,
+                           sp.mu, sp.pdf, sp.NE, inv_xf, qpx, target_or_gp, grad_vox, loss);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace vcbct

@@ -1,0 +1,263 @@
+// Ray traversal of the material-based polychromatic forward projector, shared by the forward,
+// line-integral, log-projection and adjoint kernels.
+//
+// Semantics follow reprojectionKernel (P/reprojectionOnGPU.cu:56-690): ray set-up (:128-150),
+// box entry/exit (:154-192), major axis (:198), lumped first segment to the whole-cm plane int(xi)
+// (:205-209), per-plane minor/z crossing detection by truncation (:212-247, z=0 never split), crossing
+// order by distance from the plane (:249-404), midpoint voxel by truncation (:256-259) and the
+// inscribed-circle mask (:261-267).
+//
+// Work shape: one wavefront = 64 consecutive detector rows of one column at one angle.  Such rays share
+// their xy path (source and detector point differ only in z), so the plane sequence, the minor-axis
+// crossings, the loop bounds and the major-axis choice are wave-uniform; only z crossings differ per
+// lane.  The wave walks the planes in lockstep and each lane charges 1-3 segments per plane step.
+// Steps where no lane of the wave can hit the volume's z band are skipped (their segments are not
+// charged by the reference either: the midpoint slice is out of range).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cmath>
+
+#include "vcbct_internal.h"
+
+namespace vcbct {
+
+struct RayCtx {
+    // literal ray set-up
+    double sx, sy, sz, px, py, pz, leng;
+    bool hit;  // at least two box hits
+    // u = major axis coordinate, v = minor
+    bool xmaj;
+    double su, sv, pu, pv;
+    double ui, vi, zi, uf, vf, zf;
+    double pha;       // +-1, direction along u
+    double uin;       // plane 0 position (whole cm)
+    double dv, dz;    // dv/du, dz/du
+    double duv, duz;  // du/dv, du/dz
+    double K;         // segment length per |du|
+    double dirv, dirz;
+    bool vmove;       // pv != sv
+};
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// P/reprojectionOnGPU.cu:128-198 (FP64, the reference's operation order; no contraction in this file's
+// users: they are compiled with -ffp-contract=off).
+__host__ __device__ __forceinline__ void ray_setup(const ProjGeom& g, double c, double sn, double xc, double yc, int j, int jj,
+                                          RayCtx& r) {
+    r.sx = -g.sad * sn + g.x_reg;
+    r.sy = g.sad * c + g.y_reg;
+    r.sz = g.z_reg;
+    r.pz = -(yc + j) * g.pitch_iso * (g.pixnum / g.H) + g.z_reg * g.pitch_iso / g.pitch;
+    double pr = g.pitch_iso * (xc + jj) * (g.pixnum / g.H);
+    r.px = pr * c + g.x_reg * g.pitch_iso / g.pitch;
+    r.py = pr * sn + g.y_reg * g.pitch_iso / g.pitch;
+    double leng = g.sad * g.sad / (g.sad * g.sad + (pr * pr + r.pz * r.pz));
+    r.leng = leng * sqrt(leng);
+
+    const int nxm = (int)(g.N * 0.5), nym = nxm;
+    const double sx = r.sx, sy = r.sy, sz = r.sz, px = r.px, py = r.py, pz = r.pz;
+    double xx[4], yy[4], zz[4];
+    int m = 0;
+    {
+        double y0 = double(nym) * g.s;
+        double x0 = (y0 - sy) / (py - sy) * (px - sx) + sx;
+        if (fabs(x0) < double(nxm) * g.s) { xx[m] = x0; yy[m] = y0; zz[m] = (y0 - sy) / (py - sy) * (pz - sz) + sz; m++; }
+    }
+    {
+        double y0 = -double(nym) * g.s;
+        double x0 = (y0 - sy) / (py - sy) * (px - sx) + sx;
+        if (fabs(x0) < double(nxm) * g.s) { xx[m] = x0; yy[m] = y0; zz[m] = (y0 - sy) / (py - sy) * (pz - sz) + sz; m++; }
+    }
+    {
+        double x0 = double(nxm) * g.s;
+        double y0 = (x0 - sx) / (px - sx) * (py - sy) + sy;
+        if (m < 4 && fabs(y0) < double(nym) * g.s) { xx[m] = x0; yy[m] = y0; zz[m] = (x0 - sx) / (px - sx) * (pz - sz) + sz; m++; }
+    }
+    {
+        double x0 = -double(nxm) * g.s;
+        double y0 = (x0 - sx) / (px - sx) * (py - sy) + sy;
+        if (m < 4 && fabs(y0) < double(nym) * g.s) { xx[m] = x0; yy[m] = y0; zz[m] = (x0 - sx) / (px - sx) * (pz - sz) + sz; m++; }
+    }
+    r.hit = m >= 2;
+    if (!r.hit) { xx[0] = xx[1] = yy[0] = yy[1] = zz[0] = zz[1] = 0.0; }
+    double xi, yi, zi, xf, yf, zf;
+    if ((xx[0] - sx) * (xx[0] - sx) + (yy[0] - sy) * (yy[0] - sy) + (zz[0] - sz) * (zz[0] - sz) <
+        (xx[1] - sx) * (xx[1] - sx) + (yy[1] - sy) * (yy[1] - sy) + (zz[1] - sz) * (zz[1] - sz)) {
+        xi = xx[0]; yi = yy[0]; zi = zz[0]; xf = xx[1]; yf = yy[1]; zf = zz[1];
+    } else {
+        xi = xx[1]; yi = yy[1]; zi = zz[1]; xf = xx[0]; yf = yy[0]; zf = zz[0];
+    }
+    r.xmaj = fabs(xi - xf) > fabs(yi - yf);
+    r.su = r.xmaj ? sx : sy;
+    r.sv = r.xmaj ? sy : sx;
+    r.pu = r.xmaj ? px : py;
+    r.pv = r.xmaj ? py : px;
+    r.ui = r.xmaj ? xi : yi;
+    r.uf = r.xmaj ? xf : yf;
+    r.vi = r.xmaj ? yi : xi;
+    r.vf = r.xmaj ? yf : xf;
+    r.zi = zi;
+    r.zf = zf;
+    r.pha = (r.uf - r.ui) / fabs(r.ui - r.uf);
+    int uin;
+    if ((r.ui < 0 && (r.uf - r.ui) < 0) || (r.ui > 0 && (r.uf - r.ui) > 0)) uin = int(r.ui) + (r.uf - r.ui) / fabs(r.uf - r.ui);
+    else uin = int(r.ui);
+    r.uin = (double)uin;
+    r.dv = (r.pv - r.sv) / (r.pu - r.su);
+    r.dz = (pz - sz) / (r.pu - r.su);
+    r.duv = (r.pu - r.su) / (r.pv - r.sv);
+    r.duz = (r.pu - r.su) / (pz - sz);
+    r.K = sqrt(1.0 + r.dv * r.dv + r.dz * r.dz);
+    r.dirv = (r.vf - r.vi) / fabs(r.vf - r.vi);
+    r.dirz = (zf - zi) / fabs(zf - zi);
+    r.vmove = (r.pv != r.sv) && (r.vf != r.vi);
+    if (!r.hit) r.pha = 1.0;
+}
+
+// Range of plane steps k (k >= 1) whose segments can have a midpoint slice inside [0, Nz-1].
+__host__ __device__ __forceinline__ void z_step_range(const ProjGeom& g, const RayCtx& r, bool active, int& klo, int& khi) {
+    klo = INT_MAX;
+    khi = INT_MIN;
+    if (!active || !r.hit) return;
+    const int nzm = (int)(g.Nz * 0.5);
+    const double zlo = (double)(nzm - g.Nz) * g.t, zhi = (double)(nzm + 1) * g.t;
+    const double z0 = (r.uin - r.su) * r.dz + r.sz;   // z at plane 0
+    const double dk = r.pha * g.s * r.dz;             // z change per plane step
+    if (fabs(dk) < 1e-300) {
+        if (z0 > zlo - g.t && z0 < zhi + g.t) { klo = 0; khi = INT_MAX - 8; }
+        return;
+    }
+    double k1 = (zlo - z0) / dk, k2 = (zhi - z0) / dk;
+    double a = fmin(k1, k2), b = fmax(k1, k2);
+    // steps whose end planes both lie beyond the band on one side have midpoints outside it;
+    // k = 0 stands for the lumped entry segment, which spans plane indices [ke, 0]
+    a = floor(a) - 1.0;
+    b = ceil(b) + 2.0;
+    const double ke = (r.ui - r.uin) * r.pha / g.s - 1.0;
+    if (b < ke) return;
+    if (a > 1.0e8) return;
+    klo = (int)fmax(a, 0.0);
+    khi = (int)fmin(fmax(b, 0.0), 1.0e8);
+}
+
+// Segment visitor interface: V::seg(xmc, ymc, zmc, rr) is called for every charged segment (in range and
+// inside the inscribed circle).  `on` predicates lanes.
+template <class V>
+__host__ __device__ __forceinline__ void charge(const ProjGeom& g, const RayCtx& r, double ua, double ub, bool on, V& vis) {
+    if (!on) return;
+    const int nxm = (int)(g.N * 0.5), nym = nxm, nzm = (int)(g.Nz * 0.5);
+    const double inv_s = 1.0 / g.s, inv_t = 1.0 / g.t;
+    const double rmax = g.N * 0.5 * g.s;
+    double um = (ua + ub) * 0.5;
+    double vm = (um - r.su) * r.dv + r.sv;
+    double zm = (um - r.su) * r.dz + r.sz;
+    double xm = r.xmaj ? um : vm;
+    double ym = r.xmaj ? vm : um;
+    int xmc = (int)(xm * inv_s + nxm);
+    int ymc = (int)(-ym * inv_s + nym);
+    int zmc = (int)(-zm * inv_t + nzm);
+    if ((unsigned)xmc < (unsigned)g.N && (unsigned)ymc < (unsigned)g.N && (unsigned)zmc < (unsigned)g.Nz &&
+        xm * xm + ym * ym < rmax * rmax) {
+        double rr = fabs(ub - ua) * r.K;
+        vis.seg(xmc, ymc, zmc, rr);
+    }
+}
+
+// Walks the compat traversal of one ray (lanes of a wave in lockstep over plane steps).
+// `active`: lane holds a real detector pixel.  Returns nothing; the visitor accumulates.
+// One lane's walk over plane steps [kmin, kmax] (the wave-wide union of the lanes' z ranges).
+template <class V>
+__host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
+                                                       int kmax, V& vis) {
+    if (kmin > kmax) return;  // no lane of this wave can reach the volume's z band
+
+    const double lim_u = double((int)(g.N * 0.5)) * g.s;  // nxm == nym
+    const double inv_s = 1.0 / g.s, inv_t = 1.0 / g.t;
+    const bool on = active && r.hit;
+
+    // plane 0 and the lumped entry segment (wave-uniform box test)
+    double u0 = r.uin;
+    double v0 = (u0 - r.su) * r.dv + r.sv;
+    bool inbox0 = (u0 <= lim_u && u0 >= -lim_u && v0 <= lim_u && v0 >= -lim_u);
+    if (!inbox0) return;
+    int kstart = kmin;
+    double uprev;
+    if (kstart <= 1) {  // kmin 0 or 1: the lumped entry segment is needed (and cheap)
+        charge(g, r, r.ui, u0, on, vis);
+        kstart = 1;
+        uprev = u0;
+    } else {
+        uprev = r.uin + r.pha * (double)(kstart - 1) * g.s;
+    }
+    double vprev = (uprev - r.su) * r.dv + r.sv;
+    double zprev = (uprev - r.su) * r.dz + r.sz;
+    int vmaprev = (int)(vprev * inv_s);
+    int zmaprev = (int)(zprev * inv_t);
+    double b0 = uprev;
+
+    for (int k = kstart; k <= kmax; ++k) {
+        // the reference continues while the previous plane lies in the box (inclusive)
+        if (!(uprev <= lim_u && uprev >= -lim_u && vprev <= lim_u && vprev >= -lim_u)) break;
+        double uk = r.uin + r.pha * (double)k * g.s;
+        double vk = (uk - r.su) * r.dv + r.sv;
+        double zk = (uk - r.su) * r.dz + r.sz;
+        int vma = (int)(vk * inv_s);
+        int zma = (int)(zk * inv_t);
+        bool same_sign = (vk > 0 && vprev > 0) || (vk < 0 && vprev < 0);
+        bool hv = r.vmove && ((vma != vmaprev) || !same_sign);
+        bool hz = (zma != zmaprev);
+        double uv = uk, uz = uk;
+        if (hv) {
+            double vb = ((vk <= 0 && r.dirv < 0) || (vk >= 0 && r.dirv > 0)) ? (double)(vma * g.s)
+                                                                            : (double)(vma * g.s) - r.dirv * g.s;
+            uv = (vb - r.sv) * r.duv + r.su;
+        }
+        if (hz) {
+            double zb = ((zk < 0 && r.dirz < 0) || (zk > 0 && r.dirz > 0)) ? (double)(zma * g.t)
+                                                                          : (double)(zma * g.t) - r.dirz * g.t;
+            uz = (zb - r.sz) * r.duz + r.su;
+        }
+        double chv = fabs(uk - uv), chz = fabs(uk - uz);
+        double e1 = uk, e2 = uk;
+        if (hv && hz) {
+            if (chz > chv) { e1 = uz; e2 = uv; }
+            else if (chz < chv) { e1 = uv; e2 = uz; }
+        } else if (hz && chz != 0) {
+            e1 = uz;
+        } else if (hv && chv != 0) {
+            e1 = uv;
+        }
+        charge(g, r, b0, e1, on, vis);
+        charge(g, r, e1, e2, on && (e2 != e1), vis);
+        charge(g, r, e2, uk, on && (uk != e2), vis);
+        b0 = uk;
+        uprev = uk;
+        vprev = vk;
+        vmaprev = vma;
+        zmaprev = zma;
+    }
+}
+
+}  // namespace vcbct
+
+namespace vcbct {
+template <class V>
+__device__ __forceinline__ void traverse(const ProjGeom& g, const RayCtx& r, bool active, V& vis) {
+    int klo, khi;
+    z_step_range(g, r, active, klo, khi);
+    const int kmin = wave_min(klo);
+    const int kmax = wave_max(khi);
+    traverse_lane(g, r, active, kmin, kmax, vis);
+}
+}  // namespace vcbct

@@ -1,0 +1,80 @@
+// Internal declarations shared by the vcbct translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+
+#include "vcbct.h"
+
+namespace vcbct {
+
+// Thread-local last-error message; returns `code` so call sites can `return fail(...)`.
+int fail(int code, const char* fmt, ...);
+
+#define VCBCT_HIP(call)                                                                          \
+    do {                                                                                         \
+        hipError_t _e = (call);                                                                  \
+        if (_e != hipSuccess)                                                                    \
+            return ::vcbct::fail(VCBCT_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,        \
+                                 hipGetErrorString(_e));                                         \
+    } while (0)
+
+// Device copy of the projector geometry: everything the traversal needs, by value in a kernel argument.
+struct ProjGeom {
+    int N, Nz, H, W, nAng, nmat;
+    double s, t;            // voxel pitch x/y and z [cm]
+    double x_reg, y_reg, z_reg;
+    double pitch_iso, pitch, sad, pixnum;
+};
+
+// Per-call spectrum view (device pointers to FP64 copies of the reference's float tables).
+struct Spectrum {
+    const double* mu;   // [nmat][NE]
+    const double* pdf;  // [NE]
+    int NE;
+};
+
+struct vcbct_projector_impl;
+
+// Kernels launchers (forward.hip)
+int launch_transpose_u16(const uint16_t* src, uint16_t* dst, const ProjGeom& g, hipStream_t st);
+int launch_transpose_f32(const float* src, float* dst, const ProjGeom& g, hipStream_t st);
+int launch_untranspose_f32(const float* src, float* dst, const ProjGeom& g, hipStream_t st);
+int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int vox_is_f32, const Spectrum& sp,
+                   double xfactor, int a0, int a1, float* out, int mode, float* aux, hipStream_t st);
+int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double xfactor,
+                   const float* target_or_gp, int mode, float* grad_vox, double* loss, hipStream_t st);
+
+// Zero-fill kernel (bytes must be a multiple of 4).
+int launch_zero(void* p, size_t bytes, hipStream_t st);
+
+// Fault injection for allocation failure (VCBCT_FAULT_ALLOC=<n>: the n-th device allocation fails).
+hipError_t dev_malloc(void** p, size_t bytes);
+
+}  // namespace vcbct
+
+// Forward/logproj kernel modes
+#define VCBCT_MODE_INTENSITY 0
+#define VCBCT_MODE_LINEINT 1
+#define VCBCT_MODE_LOGPROJ 2
+#define VCBCT_MODE_SEGCOUNT 3
+// Adjoint modes
+#define VCBCT_ADJ_LOSS 0
+#define VCBCT_ADJ_UPSTREAM 1
+
+struct vcbct_projector {
+    vcbct::ProjGeom g;
+    int device;
+    double* d_tables;      // [4][nAng]: cos, sin, Xc, Yc
+    uint16_t* d_vox_u16;   // [N][N][Nz][8] (y, x, z, mat)
+    float* d_vox_f32;      // [N][N][Nz][8]
+    float* d_grad_vox;     // adjoint scatter target [N][N][Nz][8]
+    double* d_mu;          // [nmat][NE]
+    double* d_pdf;         // [NE]
+    int NE;
+    int have_u16, have_f32;
+    uint16_t* d_stage_u16; // upload staging in the reference layout
+};

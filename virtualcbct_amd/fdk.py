@@ -1,0 +1,110 @@
+"""FDK: the reference's prepareConstMap / filterSinogram / reconstructImage interface
+(F/filterSinogram.h:2-6, F/reconstructImage.h:2-5) over the HIP kernels of libvcbct.so.
+
+The host-array functions take and return numpy arrays laid out like the reference's new3DArray blocks
+([th][n][m] projections, [z][y][x] volumes); the `*_device` functions work on torch device tensors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import P_D, check, lib, ptr
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream(stream):
+    if stream is not None:
+        return C.c_void_p(stream)
+    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def prepareConstMap(projWidth, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752):
+    """prepareConstMap (F/filterSinogram.cpp:65-120): the W x W FP64 filter matrix h(m - n)."""
+    out = np.empty((projWidth, projWidth), np.float64)
+    check(lib().vcbct_prepare_const_map(int(projWidth), float(DETECTOR_PITCH_CM_AT_ISO_CENT), ptr(out, P_D)))
+    return out
+
+
+def filter_sinogram_device(proj, xshift, yshift, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, out=None, stream=None):
+    """Device form of filterSinogram: proj, out are torch float32 [n][H][W] on the device."""
+    torch = _torch()
+    n, H, W = proj.shape
+    p = proj.contiguous()
+    if out is None:
+        out = torch.empty_like(p)
+    xs = np.ascontiguousarray(xshift, np.float64)
+    ys = np.ascontiguousarray(yshift, np.float64)
+    if xs.size < n or ys.size < n:
+        raise ValueError("xshift/yshift need one entry per projection")
+    check(lib().vcbct_filter_sinogram(H, W, n, C.c_void_p(p.data_ptr()), C.c_void_p(out.data_ptr()), ptr(xs, P_D),
+                                      ptr(ys, P_D), float(DETECTOR_PITCH_CM_AT_ISO_CENT), _stream(stream)))
+    return out
+
+
+def filterSinogram(projHeight, projWidth, sngrmHeight, projVolume, xshift, yshift,
+                   DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, device=0):
+    """filterSinogram (F/filterSinogram.cpp:122-186) on host arrays: returns fltdVoxData [th][n][m] FP32."""
+    torch = _torch()
+    pv = np.ascontiguousarray(projVolume, np.float32).reshape(sngrmHeight, projHeight, projWidth)
+    d = torch.from_numpy(pv).to(f"cuda:{device}")
+    out = filter_sinogram_device(d, xshift, yshift, DETECTOR_PITCH_CM_AT_ISO_CENT)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def reconstruct_image_device(recon, sino, zstart, reconStep, reconScale, xfactor, anglesRad, xshift,
+                             DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, stream=None):
+    """Device form of reconstructImage: accumulates (+=) into recon, a torch float32 [Z][R][R] tensor."""
+    Z, R, R2 = recon.shape
+    n, H, W = sino.shape
+    if R != R2 or not recon.is_contiguous():
+        raise ValueError("recon must be a contiguous [Z][R][R] tensor")
+    a = np.ascontiguousarray(anglesRad, np.float64)
+    xs = np.ascontiguousarray(xshift, np.float64)
+    if a.size < n or xs.size < n:
+        raise ValueError("angles/xshift need one entry per projection")
+    s = sino.contiguous()
+    check(lib().vcbct_reconstruct_image(R, Z, int(zstart), float(reconStep), float(reconScale),
+                                        C.c_void_p(recon.data_ptr()), H, W, n, C.c_void_p(s.data_ptr()), int(xfactor),
+                                        ptr(a, P_D), ptr(xs, P_D), float(DETECTOR_PITCH_CM_AT_ISO_CENT),
+                                        _stream(stream)))
+    return recon
+
+
+def reconstructImage(reconSize, reconSlices, zstart, reconStep, reconScale, reconVoxData, projHeight, sngrmWidth,
+                     sngrmHeight, sngrmVoxData, xfactor, anglesRad, xshift, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752,
+                     device=0):
+    """reconstructImage (F/reconstructImage.cpp:10-184): accumulates into reconVoxData (numpy FP64 [z][y][x],
+    caller-zeroed as in F/main.cpp:168) and returns it."""
+    torch = _torch()
+    sino = torch.from_numpy(np.ascontiguousarray(sngrmVoxData, np.float32).reshape(sngrmHeight, projHeight,
+                                                                                     sngrmWidth)).to(f"cuda:{device}")
+    rec = torch.zeros((reconSlices, reconSize, reconSize), dtype=torch.float32, device=f"cuda:{device}")
+    reconstruct_image_device(rec, sino, zstart, reconStep, reconScale, xfactor, anglesRad, xshift,
+                             DETECTOR_PITCH_CM_AT_ISO_CENT)
+    torch.cuda.synchronize()
+    reconVoxData += rec.cpu().numpy().astype(np.float64)
+    return reconVoxData
+
+
+def fdk_device(proj, zstart, reconStep, reconScale, reconSize, reconSlices, xfactor, anglesRad, xshift, yshift,
+               DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, stream=None):
+    """filter + backprojection on device tensors; returns the FP32 [Z][R][R] volume (before the int16 cast)."""
+    torch = _torch()
+    filt = filter_sinogram_device(proj, xshift, yshift, DETECTOR_PITCH_CM_AT_ISO_CENT, stream=stream)
+    rec = torch.zeros((reconSlices, reconSize, reconSize), dtype=torch.float32, device=proj.device)
+    reconstruct_image_device(rec, filt, zstart, reconStep, reconScale, xfactor, anglesRad, xshift,
+                             DETECTOR_PITCH_CM_AT_ISO_CENT, stream=stream)
+    return rec
+
+
+def to_short(recon):
+    """(short) truncation toward zero of the FP volume (F/main.cpp:179-185)."""
+    r = np.asarray(recon)
+    return np.trunc(r).astype(np.int16)

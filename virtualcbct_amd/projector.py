@@ -1,0 +1,210 @@
+"""Forward projector: the reference's initializeGPU / reprojectionOnGPU / terminateGPU interface
+(P/reprojectionOnGPU.h:76-93) over the HIP kernels of libvcbct.so.
+
+Device tensors are torch tensors on the ROCm device (PyTorch is the allocator and stream provider);
+the computation is the hand-written HIP code behind the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import P_D, P_F, P_U16, check, lib, ptr
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream(stream):
+    if stream is not None:
+        return C.c_void_p(stream)
+    torch = _torch()
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Projector:
+    """Device state of initializeGPU (P/reprojectionOnGPU.cu:720-825).
+
+    Arguments follow initializeGPU's (P/reprojectionOnGPU.h:76-82); `device` replaces the reference's
+    hard-coded device 1.
+    """
+
+    def __init__(self, reconSize, reconScale, reconSlices, reconStep, projImgHeight, projImgWidth,
+                 totalProjNumber, cosTable, sinTable, XcenterShift, YcenterShift, x_reg=0.0, y_reg=0.0,
+                 z_reg=0.0, DETECTOR_PITCH_CM=0.05, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752,
+                 DIST_BTWN_SRC_AND_ISOCENT_CM=100.0, DETECTOR_PIXEL_NUM=800.0, num_material=6, device=0):
+        self.g = _lib.vcbct_geometry(int(reconSize), float(reconScale), int(reconSlices), float(reconStep),
+                                     int(projImgHeight), int(projImgWidth), int(totalProjNumber), float(x_reg),
+                                     float(y_reg), float(z_reg), float(DETECTOR_PITCH_CM),
+                                     float(DETECTOR_PITCH_CM_AT_ISO_CENT), float(DIST_BTWN_SRC_AND_ISOCENT_CM),
+                                     float(DETECTOR_PIXEL_NUM), int(num_material))
+        tabs = [np.ascontiguousarray(t, np.float64) for t in (cosTable, sinTable, XcenterShift, YcenterShift)]
+        for t in tabs:
+            if t.shape != (int(totalProjNumber),):
+                raise ValueError("angle tables must have totalProjNumber entries")
+        self.device = int(device)
+        h = C.c_void_p()
+        check(lib().vcbct_projector_create(C.byref(h), C.byref(self.g), *[ptr(t, P_D) for t in tabs], self.device))
+        self.h = h
+        self.NE = 0
+        self._vol_keep = None
+
+    # shape helpers
+    @property
+    def shape_proj(self):
+        return (self.g.n_angles, self.g.proj_height, self.g.proj_width)
+
+    @property
+    def shape_maps(self):
+        return (self.g.num_material, self.g.recon_slices, self.g.recon_size, self.g.recon_size)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vcbct_projector_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ uploads
+    def set_volume(self, vol, stream=None):
+        """vol: uint16 [z][mat][y][x] (numpy host array or torch int16/uint16 device tensor, the reference
+        layout), or a torch float32 device tensor [mat][z][y][x] (elemental maps of the differentiable
+        path)."""
+        n = self.g
+        if isinstance(vol, np.ndarray):
+            v = np.ascontiguousarray(vol, np.uint16)
+            if v.size != n.recon_slices * n.num_material * n.recon_size ** 2:
+                raise ValueError(f"volume has {v.size} voxels, geometry needs "
+                                 f"{n.recon_slices}x{n.num_material}x{n.recon_size}^2")
+            check(lib().vcbct_projector_set_volume_u16(self.h, v.ctypes.data_as(C.c_void_p), 0, _stream(stream)))
+            return
+        torch = _torch()
+        if vol.dtype == torch.float32:
+            if tuple(vol.shape) != self.shape_maps or not vol.is_contiguous():
+                raise ValueError(f"FP32 maps must be contiguous {self.shape_maps}")
+            check(lib().vcbct_projector_set_volume_f32(self.h, C.c_void_p(vol.data_ptr()), _stream(stream)))
+            self._vol_keep = vol
+            return
+        if vol.dtype not in (torch.int16, torch.uint16) or not vol.is_contiguous():
+            raise ValueError("device volume must be a contiguous int16/uint16 tensor")
+        if vol.numel() != n.recon_slices * n.num_material * n.recon_size ** 2:
+            raise ValueError("device volume size does not match the geometry")
+        check(lib().vcbct_projector_set_volume_u16(self.h, C.c_void_p(vol.data_ptr()), 1, _stream(stream)))
+
+    def set_spectrum(self, Attenuation, pdf_pe):
+        """Attenuation [mat][NE] (or flat mat*NE) cm^2/g and pdf_pe [NE], as attenuation_coefficient writes."""
+        pdf = np.ascontiguousarray(pdf_pe, np.float32).ravel()
+        ne = pdf.size
+        at = np.ascontiguousarray(Attenuation, np.float32).ravel()
+        if at.size < self.g.num_material * ne:
+            raise ValueError("attenuation table smaller than num_material x NE")
+        at = np.ascontiguousarray(at[: self.g.num_material * ne])
+        check(lib().vcbct_projector_set_spectrum(self.h, ptr(at, P_F), ptr(pdf, P_F), ne))
+        self.NE = ne
+
+    # ------------------------------------------------------------------ compute
+    def forward(self, xfactor, out=None, a0=0, a1=None, stream=None):
+        """Polychromatic intensity projections [n_angles][H][W] (torch float32 on the device)."""
+        torch = _torch()
+        if out is None:
+            out = torch.empty(self.shape_proj, dtype=torch.float32, device=f"cuda:{self.device}")
+        a1 = self.g.n_angles if a1 is None else a1
+        check(lib().vcbct_projector_forward(self.h, C.c_void_p(out.data_ptr()), float(xfactor), int(a0), int(a1),
+                                            _stream(stream)))
+        return out
+
+    def line_integrals(self, stream=None):
+        torch = _torch()
+        L = torch.empty(self.shape_proj + (self.g.num_material,), dtype=torch.float32, device=f"cuda:{self.device}")
+        check(lib().vcbct_projector_line_integrals(self.h, C.c_void_p(L.data_ptr()), _stream(stream)))
+        return L
+
+    def segment_counts(self, stream=None):
+        """Charged segments per ray (FP32 [n_angles][H][W]); independent of the volume's values."""
+        torch = _torch()
+        c = torch.empty(self.shape_proj, dtype=torch.float32, device=f"cuda:{self.device}")
+        check(lib().vcbct_projector_segment_counts(self.h, C.c_void_p(c.data_ptr()), _stream(stream)))
+        return c
+
+    def logproj(self, xfactor, out=None, stream=None):
+        torch = _torch()
+        if out is None:
+            out = torch.empty(self.shape_proj, dtype=torch.float32, device=f"cuda:{self.device}")
+        check(lib().vcbct_projector_logproj(self.h, C.c_void_p(out.data_ptr()), float(xfactor), _stream(stream)))
+        return out
+
+    def adjoint(self, target, xfactor, stream=None):
+        """Gradient of sum (p - target)^2 w.r.t. the FP32 maps set with set_volume; returns (grad, loss)."""
+        torch = _torch()
+        t = target.contiguous().to(torch.float32)
+        grad = torch.empty(self.shape_maps, dtype=torch.float32, device=f"cuda:{self.device}")
+        loss = torch.zeros(1, dtype=torch.float64, device=f"cuda:{self.device}")
+        check(lib().vcbct_projector_adjoint(self.h, C.c_void_p(t.data_ptr()), C.c_void_p(grad.data_ptr()),
+                                            C.c_void_p(loss.data_ptr()), float(xfactor), _stream(stream)))
+        return grad, loss
+
+    def logproj_backward(self, grad_p, xfactor, stream=None):
+        torch = _torch()
+        gp = grad_p.contiguous().to(torch.float32)
+        grad = torch.empty(self.shape_maps, dtype=torch.float32, device=f"cuda:{self.device}")
+        check(lib().vcbct_projector_logproj_backward(self.h, C.c_void_p(gp.data_ptr()), C.c_void_p(grad.data_ptr()),
+                                                     float(xfactor), _stream(stream)))
+        return grad
+
+
+# ---------------------------------------------------------------------------------------------------
+# The reference's module-level interface (P/reprojectionOnGPU.h:76-93), one projector per process.
+
+_current: Projector | None = None
+
+
+def initializeGPU(reconSize, reconScale, reconSlices, reconStep, projImgHeight, projImgWidth, totalProjNumber,
+                  cosTable, sinTable, XcenterShift, YcenterShift, x_reg, y_reg, z_reg, DETECTOR_PITCH_CM,
+                  DETECTOR_PITCH_CM_AT_ISO_CENT, DIST_BTWN_SRC_AND_ISOCENT_CM, DETECTOR_PIXEL_NUM, num_material,
+                  device=0):
+    """initializeGPU (P/reprojectionOnGPU.cu:720-825)."""
+    global _current
+    if _current is not None:
+        _current.close()
+    _current = Projector(reconSize, reconScale, reconSlices, reconStep, projImgHeight, projImgWidth,
+                         totalProjNumber, cosTable, sinTable, XcenterShift, YcenterShift, x_reg, y_reg, z_reg,
+                         DETECTOR_PITCH_CM, DETECTOR_PITCH_CM_AT_ISO_CENT, DIST_BTWN_SRC_AND_ISOCENT_CM,
+                         DETECTOR_PIXEL_NUM, num_material, device)
+    return _current
+
+
+def terminateGPU():
+    """terminateGPU (P/reprojectionOnGPU.cu:827-834)."""
+    global _current
+    if _current is not None:
+        _current.close()
+        _current = None
+
+
+def reprojectionOnGPU(reconImageshort, xfactor, Attenuation, pdf_pe, NE, outputname=None, endProjNumber=None):
+    """reprojectionOnGPU (P/reprojectionOnGPU.cu:838-1100): projects the uint16 [z][mat][y][x] volume with the
+    spectrum and, like the reference, writes `<outputname>.raw` (FP32 [angle][row][col]) when a name is
+    given.  Returns the projections as a host numpy array."""
+    if _current is None:
+        raise RuntimeError("initializeGPU was not called")
+    p = _current
+    vol = np.ascontiguousarray(reconImageshort, np.uint16)
+    at = np.ascontiguousarray(np.asarray(Attenuation, np.float32).ravel()[: p.g.num_material * NE])
+    pdf = np.ascontiguousarray(np.asarray(pdf_pe, np.float32).ravel()[:NE])
+    out = np.empty(p.shape_proj, np.float32)
+    check(lib().vcbct_reprojection(p.h, ptr(vol, P_U16), ptr(at, P_F), ptr(pdf, P_F), int(NE), float(xfactor),
+                                   ptr(out, P_F)))
+    if endProjNumber is not None:
+        out = out[:endProjNumber]
+    if outputname:
+        out.tofile(f"{outputname}.raw")
+    return out
