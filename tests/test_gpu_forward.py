@@ -168,3 +168,29 @@ def test_logproj_and_adjoint_match_oracle(torch):
     gp = torch.from_numpy((2 * (p - target)).astype(np.float32)).cuda()
     g2 = P.logproj_backward(gp, 1.0).cpu().numpy()
     assert np.abs(g2 - gg).max() / scale < 1e-4
+
+
+def test_autograd_function_matches_oracle_gradient(torch):
+    from virtualcbct_amd.autograd import poly_logproj
+    rng = np.random.default_rng(12)
+    N, Nz = 16, 4
+    ang = np.deg2rad(np.array([-170.0001, -60.00001, 29.99999, 120.0]))
+    cs, sn, xc, yc = O.trig_table_proj(14, 14, ang, np.full(4, 0.01), np.zeros(4))
+    g = O.Geometry(N=N, s=0.8, Nz=Nz, t=1.2, H=14, W=14, cos_t=cs, sin_t=sn, xc=xc, yc=yc)
+    maps = rng.uniform(0.1, 1.0, (6, Nz, N, N)).astype(np.float32)
+    atten = rng.uniform(0.1, 0.5, (6, 5)).astype(np.float32)
+    pdf = np.array([0.1, 0.3, 0.3, 0.2, 0.1], np.float32)
+    P = make_projector(g)
+    P.set_spectrum(atten, pdf)
+    x = torch.from_numpy(maps).cuda().requires_grad_(True)
+    target = torch.from_numpy(O.logproj(g, np.ascontiguousarray(maps.transpose(1, 0, 2, 3)) * 1.03, atten, pdf,
+                                        1.0).astype(np.float32)).cuda()
+    p = poly_logproj(x, P, 1.0)
+    loss = ((p - target) ** 2).sum()
+    loss.backward()
+    go, _ = O.adjoint(g, np.ascontiguousarray(maps.transpose(1, 0, 2, 3)), atten, pdf, 1.0,
+                      target.cpu().numpy().astype(np.float64))
+    go = go.transpose(1, 0, 2, 3)
+    err = np.abs(x.grad.cpu().numpy() - go).max() / np.abs(go).max()
+    print(f"autograd grad vs FP64 adjoint: {err:.3e}")
+    assert err < 1e-4

@@ -1,0 +1,53 @@
+"""Differentiable polychromatic projector for PyTorch.
+
+`poly_logproj(maps, projector, xfactor)` returns the polychromatic log projection
+    p = -log( sum_e pdf_e exp(-sum_m mu_me L_m / xfactor) / sum_e pdf_e )
+of six FP32 elemental density maps [mat][z][y][x] (a torch tensor on the device).  Its backward runs the
+hand-written adjoint kernel: the per-ray line integrals are recomputed (not stored) and
+dL/dp * mubar_m / xfactor is scattered along the same ray traversal, with mubar_m the transmission-weighted
+spectral mean of mu_m.  There is no reference counterpart; the reference's scatter U-Net (Unet_scatter.py)
+consumes projections of this kind.
+"""
+from __future__ import annotations
+
+import torch
+
+from .projector import Projector
+
+
+class PolyLogProjection(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, maps, projector: Projector, xfactor: float):
+        if maps.dtype != torch.float32 or not maps.is_cuda:
+            raise TypeError("maps must be an FP32 tensor on the device")
+        m = maps.contiguous()
+        projector.set_volume(m)
+        p = projector.logproj(xfactor)
+        ctx.projector = projector
+        ctx.xfactor = xfactor
+        ctx.save_for_backward(m)
+        return p
+
+    @staticmethod
+    def backward(ctx, grad_p):
+        (m,) = ctx.saved_tensors
+        P = ctx.projector
+        P.set_volume(m)  # the projector may have been used with other maps since forward
+        grad = P.logproj_backward(grad_p.contiguous(), ctx.xfactor)
+        return grad, None, None
+
+
+def poly_logproj(maps, projector: Projector, xfactor: float = 1.0):
+    return PolyLogProjection.apply(maps, projector, xfactor)
+
+
+class DifferentiableProjector(torch.nn.Module):
+    """nn.Module wrapper: geometry and spectrum fixed at construction, maps as the input."""
+
+    def __init__(self, projector: Projector, xfactor: float = 1.0):
+        super().__init__()
+        self.projector = projector
+        self.xfactor = float(xfactor)
+
+    def forward(self, maps):
+        return poly_logproj(maps, self.projector, self.xfactor)
