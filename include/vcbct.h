@@ -151,6 +151,41 @@ int vcbct_reconstruct_image_host(int recon_size, int recon_slices, int zstart_cm
                                  const float* host_filtered, int xfactor, const double* host_angles_rad,
                                  const double* host_xshift, double det_pitch_iso_cm);
 
+/* FDK plan (no reference counterpart: the reference recomputes its tables per call): the per-projection
+ * tables and the filter's frequency response are built once; execution allocates nothing and does not
+ * synchronise, so it can be captured in a hipGraph.  Creating a plan also picks the backprojector's
+ * slices-per-thread by timing 2, 4 and 8 (autotuner; VCBCT_NO_AUTOTUNE=1 keeps 4). */
+typedef struct vcbct_fdk_plan vcbct_fdk_plan;
+int vcbct_fdk_plan_create(vcbct_fdk_plan** out, int proj_height, int proj_width, int n_proj, int recon_size,
+                          int recon_slices, int zstart_cm, double recon_step, double recon_scale, int xfactor,
+                          const double* host_angles_rad, const double* host_xshift, const double* host_yshift,
+                          double det_pitch_iso_cm, int device);
+void vcbct_fdk_plan_destroy(vcbct_fdk_plan* p);
+int vcbct_fdk_plan_filter(vcbct_fdk_plan* p, const float* dev_proj, float* dev_filtered, void* stream);
+int vcbct_fdk_plan_backproject(vcbct_fdk_plan* p, const float* dev_filtered, float* dev_recon, void* stream);
+int vcbct_fdk_plan_execute(vcbct_fdk_plan* p, const float* dev_proj, float* dev_filtered, float* dev_recon,
+                           void* stream);
+int vcbct_fdk_plan_slices_per_thread(const vcbct_fdk_plan* p);
+
+/* ------------------------------------------------------------------------------------------------
+ * Batched launch (no reference counterpart): many forward problems — volumes of different slice counts,
+ * angle counts and spectra, each set up as a vcbct_projector — in ONE persistent kernel launch with
+ * per-XCD work queues, optionally followed by per-problem FDK plans; the whole sequence can be captured
+ * in a hipGraph and replayed.  The persistent grid size is autotuned on the first run. */
+typedef struct vcbct_batch vcbct_batch;
+int vcbct_batch_create(vcbct_batch** out, int device);
+void vcbct_batch_destroy(vcbct_batch* b);
+int vcbct_batch_add_forward(vcbct_batch* b, vcbct_projector* p, float* dev_out, double xfactor);
+/* recon_bytes: size of dev_recon, which each run zeroes before backprojecting into it */
+int vcbct_batch_add_fdk(vcbct_batch* b, vcbct_fdk_plan* plan, const float* dev_proj, float* dev_filtered,
+                        float* dev_recon, long long recon_bytes);
+int vcbct_batch_run(vcbct_batch* b, void* stream);
+int vcbct_batch_capture_graph(vcbct_batch* b);
+int vcbct_batch_launch_graph(vcbct_batch* b, void* stream);
+int vcbct_batch_blocks(const vcbct_batch* b);
+/* Text log of the autotuner's decisions (candidates and timings); returns its length. */
+int vcbct_autotune_log(char* buf, int capacity);
+
 /* ------------------------------------------------------------------------------------------------
  * Host-side formats kept byte-compatible with the reference.
  *

@@ -76,6 +76,22 @@ SIGNATURES = {
     "vcbct_filter_sinogram_host": (c_int, [c_int, c_int, c_int, P_F, P_F, P_D, P_D, c_double]),
     "vcbct_reconstruct_image_host": (c_int, [c_int, c_int, c_int, c_double, c_double, P_D, c_int, c_int, c_int, P_F,
                                              c_int, P_D, P_D, c_double]),
+    "vcbct_fdk_plan_create": (c_int, [C.POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int, c_int, c_double,
+                                      c_double, c_int, P_D, P_D, P_D, c_double, c_int]),
+    "vcbct_fdk_plan_destroy": (None, [c_void_p]),
+    "vcbct_fdk_plan_filter": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "vcbct_fdk_plan_backproject": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "vcbct_fdk_plan_execute": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "vcbct_fdk_plan_slices_per_thread": (c_int, [c_void_p]),
+    "vcbct_batch_create": (c_int, [C.POINTER(c_void_p), c_int]),
+    "vcbct_batch_destroy": (None, [c_void_p]),
+    "vcbct_batch_add_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_double]),
+    "vcbct_batch_add_fdk": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.c_longlong]),
+    "vcbct_batch_run": (c_int, [c_void_p, c_void_p]),
+    "vcbct_batch_capture_graph": (c_int, [c_void_p]),
+    "vcbct_batch_launch_graph": (c_int, [c_void_p, c_void_p]),
+    "vcbct_batch_blocks": (c_int, [c_void_p]),
+    "vcbct_autotune_log": (c_int, [C.c_char_p, c_int]),
     "vcbct_read_proj_config": (c_int, [c_char_p, C.POINTER(vcbct_proj_config)]),
     "vcbct_read_flexmap": (c_int, [c_char_p, P_D, P_D, P_D, c_int]),
     "vcbct_proj_slice_window": (c_int, [C.POINTER(vcbct_proj_config), P_I, P_I]),

@@ -1,0 +1,62 @@
+"""Batched launch: many forward problems (phantoms of uneven slice counts, angle counts and energy bins) in one
+persistent kernel launch, optionally followed by per-problem FDK plans, capturable in a hipGraph."""
+from __future__ import annotations
+
+import ctypes as C
+
+from ._lib import check, lib
+
+
+def _stream(stream):
+    if stream is not None:
+        return C.c_void_p(stream)
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Batch:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().vcbct_batch_create(C.byref(h), int(device)))
+        self.h = h
+        self._keep = []
+
+    def add_forward(self, projector, out, xfactor):
+        """projector: a Projector with its volume and spectrum set; out: FP32 device tensor of its projections."""
+        check(lib().vcbct_batch_add_forward(self.h, projector.h, C.c_void_p(out.data_ptr()), float(xfactor)))
+        self._keep += [projector, out]
+
+    def add_fdk(self, plan, proj, filtered, recon):
+        check(lib().vcbct_batch_add_fdk(self.h, plan.h, C.c_void_p(proj.data_ptr()), C.c_void_p(filtered.data_ptr()),
+                                        C.c_void_p(recon.data_ptr()), recon.numel() * 4))
+        self._keep += [plan, proj, filtered, recon]
+
+    def run(self, stream=None):
+        check(lib().vcbct_batch_run(self.h, _stream(stream)))
+
+    def capture(self):
+        check(lib().vcbct_batch_capture_graph(self.h))
+
+    def replay(self, stream=None):
+        check(lib().vcbct_batch_launch_graph(self.h, _stream(stream)))
+
+    @property
+    def blocks(self):
+        return lib().vcbct_batch_blocks(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vcbct_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune_log():
+    buf = C.create_string_buffer(1 << 16)
+    lib().vcbct_autotune_log(buf, len(buf))
+    return buf.value.decode()

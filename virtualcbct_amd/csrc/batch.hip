@@ -1,0 +1,260 @@
+// Batched launch: many forward problems (phantoms of uneven slice counts, angle counts and spectra) in one
+// persistent launch, optionally followed by per-problem FDK, captured in a hipGraph.  The persistent grid
+// size is chosen by timing candidates on the first run (autotuner, cached per batch signature).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "vcbct_internal.h"
+
+namespace vcbct {
+const char* tune_log();
+static std::mutex g_btune_mu;
+static std::unordered_map<std::string, int> g_batch_blocks;
+static std::string g_btune_log;
+}  // namespace vcbct
+
+using namespace vcbct;
+
+struct vcbct_batch {
+    int device;
+    std::vector<BatchProblem> probs;
+    int totals[8];
+    BatchProblem* d_probs;
+    int* d_total;
+    int* d_counters;
+    int cap;
+    int nblocks;
+    bool dirty;
+    struct Fdk {
+        vcbct_fdk_plan* plan;
+        const float* proj;
+        float* filt;
+        float* rec;
+        size_t rec_bytes;
+    };
+    std::vector<Fdk> fdk;
+    hipStream_t cap_stream;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    bool have_graph;
+    std::string signature;
+};
+
+extern "C" int vcbct_fdk_plan_execute(vcbct_fdk_plan* p, const float* dev_proj, float* dev_filtered, float* dev_recon,
+                                      void* stream);
+
+extern "C" int vcbct_batch_create(vcbct_batch** out, int device) {
+    if (!out) return fail(VCBCT_EINVAL, "batch_create: out is NULL");
+    *out = nullptr;
+    VCBCT_HIP(hipSetDevice(device));
+    vcbct_batch* b = new (std::nothrow) vcbct_batch();
+    if (!b) return fail(VCBCT_ENOMEM, "host allocation failed");
+    b->device = device;
+    memset(b->totals, 0, sizeof b->totals);
+    b->d_probs = nullptr;
+    b->d_total = nullptr;
+    b->d_counters = nullptr;
+    b->cap = 0;
+    b->nblocks = 0;
+    b->dirty = true;
+    b->have_graph = false;
+    if (hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(VCBCT_EHIP, "stream creation failed");
+    }
+    *out = b;
+    return 0;
+}
+
+extern "C" void vcbct_batch_destroy(vcbct_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->have_graph) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+    }
+    (void)hipStreamDestroy(b->cap_stream);
+    (void)hipFree(b->d_probs);
+    (void)hipFree(b->d_total);
+    (void)hipFree(b->d_counters);
+    delete b;
+}
+
+extern "C" int vcbct_batch_add_forward(vcbct_batch* b, vcbct_projector* p, float* dev_out, double xfactor) {
+    if (!b || !p || !dev_out) return fail(VCBCT_EINVAL, "batch_add_forward: NULL argument");
+    if (p->device != b->device) return fail(VCBCT_EINVAL, "projector lives on another device");
+    if (!(p->have_u16 || p->have_f32)) return fail(VCBCT_ESTATE, "projector has no volume");
+    if (p->NE < 1) return fail(VCBCT_ESTATE, "projector has no spectrum");
+    if (!(xfactor > 0)) return fail(VCBCT_EINVAL, "xfactor must be positive");
+    if (b->have_graph) return fail(VCBCT_ESTATE, "batch already captured; create a new batch");
+    BatchProblem P;
+    memset(&P, 0, sizeof P);
+    P.g = p->g;
+    P.tab = p->d_tables;
+    P.vox_is_f32 = p->have_u16 ? 0 : 1;
+    P.vox = P.vox_is_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16;
+    P.mu = p->d_mu;
+    P.pdf = p->d_pdf;
+    P.NE = p->NE;
+    P.out = dev_out;
+    P.inv_xf = 1.0 / xfactor;
+    P.cols_per_xcd = (p->g.W + 7) / 8;
+    const long items = (long)((p->g.H + 63) / 64) * p->g.nAng * P.cols_per_xcd;
+    for (int q = 0; q < 8; ++q) {
+        if ((long)b->totals[q] + items > 0x7fffffffL - 1024) return fail(VCBCT_EINVAL, "batch too large");
+        P.xcd_start[q] = b->totals[q];
+        b->totals[q] += (int)items;
+    }
+    b->probs.push_back(P);
+    char sig[128];
+    snprintf(sig, sizeof sig, "[N%d Z%d A%d H%d W%d E%d]", p->g.N, p->g.Nz, p->g.nAng, p->g.H, p->g.W, p->NE);
+    b->signature += sig;
+    b->dirty = true;
+    return 0;
+}
+
+extern "C" int vcbct_batch_add_fdk(vcbct_batch* b, vcbct_fdk_plan* plan, const float* dev_proj, float* dev_filtered,
+                                   float* dev_recon, long long recon_bytes) {
+    if (!b || !plan || !dev_proj || !dev_filtered || !dev_recon || recon_bytes <= 0 || (recon_bytes & 3))
+        return fail(VCBCT_EINVAL, "batch_add_fdk: bad arguments");
+    if (b->have_graph) return fail(VCBCT_ESTATE, "batch already captured; create a new batch");
+    b->fdk.push_back({plan, dev_proj, dev_filtered, dev_recon, (size_t)recon_bytes});
+    return 0;
+}
+
+static int batch_launch(vcbct_batch* b, int nblocks, hipStream_t st) {
+    int rc = 0;
+    if (!b->probs.empty())
+        rc = launch_forward_batch(b->d_probs, (int)b->probs.size(), b->d_total, b->d_counters, nblocks, st);
+    for (size_t i = 0; !rc && i < b->fdk.size(); ++i) {
+        rc = launch_zero(b->fdk[i].rec, b->fdk[i].rec_bytes, st);
+        if (!rc) rc = vcbct_fdk_plan_execute(b->fdk[i].plan, b->fdk[i].proj, b->fdk[i].filt, b->fdk[i].rec, st);
+    }
+    return rc;
+}
+
+static int batch_finalize(vcbct_batch* b) {
+    if (!b->dirty) return 0;
+    const size_t np = b->probs.size();
+    if (np > (size_t)b->cap) {
+        (void)hipFree(b->d_probs);
+        b->d_probs = nullptr;
+        if (dev_malloc((void**)&b->d_probs, np * sizeof(BatchProblem)) != hipSuccess)
+            return fail(VCBCT_ENOMEM, "batch descriptors");
+        b->cap = (int)np;
+    }
+    if (!b->d_total && (dev_malloc((void**)&b->d_total, 8 * sizeof(int)) != hipSuccess ||
+                        dev_malloc((void**)&b->d_counters, 8 * sizeof(int)) != hipSuccess))
+        return fail(VCBCT_ENOMEM, "batch counters");
+    if (np) VCBCT_HIP(hipMemcpy(b->d_probs, b->probs.data(), np * sizeof(BatchProblem), hipMemcpyHostToDevice));
+    VCBCT_HIP(hipMemcpy(b->d_total, b->totals, 8 * sizeof(int), hipMemcpyHostToDevice));
+    // persistent grid: a multiple of 8 blocks (one queue per XCD), tuned among 2, 4, 6 blocks per CU
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+    cus = (cus + 7) / 8 * 8;
+    int chosen = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_btune_mu);
+        auto it = g_batch_blocks.find(b->signature);
+        if (it != g_batch_blocks.end()) chosen = it->second;
+    }
+    const char* off = getenv("VCBCT_NO_AUTOTUNE");
+    if (!chosen && np && !(off && *off == '1')) {
+        hipEvent_t e0, e1;
+        VCBCT_HIP(hipEventCreate(&e0));
+        VCBCT_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        std::string rep = "batch " + b->signature + ":";
+        for (int per : {2, 4, 6}) {
+            int nb = cus * per;
+            int rc = launch_forward_batch(b->d_probs, (int)np, b->d_total, b->d_counters, nb, nullptr);
+            if (rc) return rc;
+            (void)hipEventRecord(e0, nullptr);
+            rc = launch_forward_batch(b->d_probs, (int)np, b->d_total, b->d_counters, nb, nullptr);
+            if (rc) return rc;
+            (void)hipEventRecord(e1, nullptr);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            char s[64];
+            snprintf(s, sizeof s, " %dx%d=%.3fms", per, cus, ms);
+            rep += s;
+            if (ms < best) { best = ms; chosen = nb; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        std::lock_guard<std::mutex> lk(g_btune_mu);
+        g_batch_blocks[b->signature] = chosen;
+        g_btune_log += rep + " -> " + std::to_string(chosen) + " blocks\n";
+    }
+    b->nblocks = chosen ? chosen : cus * 4;
+    b->dirty = false;
+    return 0;
+}
+
+extern "C" int vcbct_batch_run(vcbct_batch* b, void* stream) {
+    if (!b) return fail(VCBCT_EINVAL, "batch is NULL");
+    VCBCT_HIP(hipSetDevice(b->device));
+    int rc = batch_finalize(b);
+    if (rc) return rc;
+    return batch_launch(b, b->nblocks, (hipStream_t)stream);
+}
+
+extern "C" int vcbct_batch_capture_graph(vcbct_batch* b) {
+    if (!b) return fail(VCBCT_EINVAL, "batch is NULL");
+    VCBCT_HIP(hipSetDevice(b->device));
+    int rc = batch_finalize(b);  // uploads and autotuning happen outside the capture
+    if (rc) return rc;
+    VCBCT_HIP(hipDeviceSynchronize());
+    if (b->have_graph) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->have_graph = false;
+    }
+    VCBCT_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    rc = batch_launch(b, b->nblocks, b->cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(VCBCT_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(VCBCT_EHIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    }
+    b->graph = g;
+    b->have_graph = true;
+    return 0;
+}
+
+extern "C" int vcbct_batch_launch_graph(vcbct_batch* b, void* stream) {
+    if (!b) return fail(VCBCT_EINVAL, "batch is NULL");
+    if (!b->have_graph) return fail(VCBCT_ESTATE, "no captured graph (vcbct_batch_capture_graph)");
+    VCBCT_HIP(hipSetDevice(b->device));
+    VCBCT_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return 0;
+}
+
+extern "C" int vcbct_batch_blocks(const vcbct_batch* b) { return b ? b->nblocks : -1; }
+
+extern "C" int vcbct_autotune_log(char* buf, int cap) {
+    if (!buf || cap < 1) return fail(VCBCT_EINVAL, "autotune_log: bad buffer");
+    std::string s;
+    {
+        std::lock_guard<std::mutex> lk(g_btune_mu);
+        s = g_btune_log;
+    }
+    s += tune_log();
+    snprintf(buf, (size_t)cap, "%s", s.c_str());
+    return (int)s.size();
+}

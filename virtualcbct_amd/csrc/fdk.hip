@@ -9,6 +9,7 @@
 #include <climits>
 #include <complex>
 #include <mutex>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -184,13 +185,13 @@ __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, f
 }
 
 // ------------------------------------------------------------------------------------------------
-// Backprojection.  Thread = one (x, y) voxel column, ZB consecutive slices; block = 16 x 16 columns.
+// Backprojection.  Thread = one (x, y) voxel column, ZB consecutive slices (ZB picked by the autotuner
+// among 2, 4, 8); block = 16 x 16 columns.
 // Per angle the FP64 geometry (U, its interpolation weight and base column, 1/L2) is computed once
 // per column and reused for the ZB slices.  The interpolation follows F/reconstructImage.cpp:76-175:
 // the horizontal weight is frac(U) but the base column is floor(W/2 + U + xshift_px - 0.5), and the
 // three edge cases at the last column / row sample one tap pair or one tap.
-constexpr int ZB = 4;
-
+template <int ZB>
 __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, const float* __restrict__ sino, int R,
                                                      int Z, double zstart, double step, double scale, int H, int W,
                                                      int n, const double* __restrict__ atab, double xfactor,
@@ -256,52 +257,12 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
         if (z0 + i < Z) recon[((size_t)(z0 + i) * R + y) * R + x] += acc[i];
 }
 
-}  // namespace vcbct
 
-using namespace vcbct;
+// ------------------------------------------------------------------------------------------------
+// Plans: per-geometry device tables built once, so that execution allocates nothing and synchronises
+// nothing (it can be captured in a hipGraph).
 
-extern "C" int vcbct_prepare_const_map(int W, double pitch_iso, double* out) {
-    if (W < 1 || !out) return fail(VCBCT_EINVAL, "prepare_const_map: bad arguments");
-    for (int m = 0; m < W; ++m)
-        for (int n = 0; n < W; ++n) out[(size_t)m * W + n] = (m == n) ? 0.0 : ramp_offcentre(m - n, pitch_iso, W);
-    return 0;
-}
-
-extern "C" int vcbct_filter_sinogram(int H, int W, int n, const float* dev_proj, float* dev_out, const double* xshift,
-                                     const double* yshift, double pitch_iso, void* stream) {
-    if (H < 1 || W < 2 || n < 1 || !dev_proj || !dev_out || !xshift || !yshift)
-        return fail(VCBCT_EINVAL, "filter_sinogram: bad arguments");
-    FilterPlan* P;
-    int rc = get_plan(W, pitch_iso, &P);
-    if (rc) return rc;
-    hipStream_t st = (hipStream_t)stream;
-    // per-projection centre offsets, F/filterSinogram.cpp:141-142 (both with the projection WIDTH)
-    std::vector<float> tab(2 * n);
-    for (int th = 0; th < n; ++th) {
-        tab[th] = (float)(-W / 2.0 - xshift[th] / (0.05 * 800 / W) - 1);
-        tab[n + th] = (float)(-W / 2.0 - yshift[th] / (0.05 * 800 / W) - 1);
-    }
-    float* d_tab;
-    VCBCT_HIP(dev_malloc((void**)&d_tab, 2 * n * sizeof(float)));
-    VCBCT_HIP(hipMemcpyAsync(d_tab, tab.data(), 2 * n * sizeof(float), hipMemcpyHostToDevice, st));
-    const long nrows = (long)n * H;
-    const long nblk = (nrows + 1) / 2;
-    if (nblk > INT_MAX) return fail(VCBCT_EINVAL, "filter_sinogram: too many rows");
-    const float pitchW = (float)(pitch_iso * 800 / W);
-    hipLaunchKernelGGL(k_filter, dim3((unsigned)nblk), dim3(FT), 2 * P->L * sizeof(float2), st, dev_proj, dev_out, H, W,
-                       nrows, P->L, P->logL, P->d_H, P->d_tw, d_tab, d_tab + n, pitchW);
-    VCBCT_HIP(hipGetLastError());
-    VCBCT_HIP(hipStreamSynchronize(st));
-    VCBCT_HIP(hipFree(d_tab));
-    return 0;
-}
-
-extern "C" int vcbct_reconstruct_image(int R, int Z, int zstart, double step, double scale, float* dev_recon, int H,
-                                       int W, int n, const float* dev_sino, int xfactor, const double* angles,
-                                       const double* xshift, double pitch_iso, void* stream) {
-    if (R < 1 || Z < 1 || H < 2 || W < 2 || n < 2 || !dev_recon || !dev_sino || !angles || !xshift)
-        return fail(VCBCT_EINVAL, "reconstruct_image: bad arguments");
-    hipStream_t st = (hipStream_t)stream;
+static std::vector<double> bp_table(int n, int W, const double* angles, const double* xshift) {
     // per-angle FP64 table: cos, sin, dbeta (F/reconstructImage.cpp:44-71), xshift in detector pixels
     std::vector<double> tab(4 * (size_t)n);
     for (int th = 0; th < n; ++th) {
@@ -327,16 +288,236 @@ extern "C" int vcbct_reconstruct_image(int R, int Z, int zstart, double step, do
         tab[2 * n + th] = dbeta;
         tab[3 * n + th] = xshift[th] / (0.05 * 800 / W);
     }
+    return tab;
+}
+
+static std::vector<float> filter_table(int n, int W, const double* xshift, const double* yshift) {
+    // per-projection centre offsets, F/filterSinogram.cpp:141-142 (both with the projection WIDTH)
+    std::vector<float> tab(2 * (size_t)n);
+    for (int th = 0; th < n; ++th) {
+        tab[th] = (float)(-W / 2.0 - xshift[th] / (0.05 * 800 / W) - 1);
+        tab[n + th] = (float)(-W / 2.0 - yshift[th] / (0.05 * 800 / W) - 1);
+    }
+    return tab;
+}
+
+static int launch_filter(const FilterPlan* P, int H, int W, int n, const float* dp, float* dq, const float* d_ctab,
+                         double pitch_iso, hipStream_t st) {
+    const long nrows = (long)n * H;
+    const long nblk = (nrows + 1) / 2;
+    if (nblk > INT_MAX) return fail(VCBCT_EINVAL, "filter_sinogram: too many rows");
+    const float pitchW = (float)(pitch_iso * 800 / W);
+    hipLaunchKernelGGL(k_filter, dim3((unsigned)nblk), dim3(FT), 2 * P->L * sizeof(float2), st, dp, dq, H, W, nrows,
+                       P->L, P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+
+static int launch_bp(int zb, int R, int Z, int zstart, double step, double scale, float* drec, int H, int W, int n,
+                     const float* ds, const double* d_atab, int xfactor, double pitch_iso, hipStream_t st) {
+    dim3 grid((R + 15) / 16, (R + 15) / 16, (Z + zb - 1) / zb);
+    const double pitchW = pitch_iso * 800 / W;
+#define VCBCT_BP(ZBV)                                                                                                  \
+    hipLaunchKernelGGL(k_backproject<ZBV>, grid, dim3(256), 0, st, drec, ds, R, Z, (double)zstart, step, scale, H, W, n, \
+                       d_atab, (double)xfactor, pitchW)
+    if (zb == 2) VCBCT_BP(2);
+    else if (zb == 8) VCBCT_BP(8);
+    else VCBCT_BP(4);
+#undef VCBCT_BP
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+
+// Autotuner cache: slices per thread of the backprojector, per (R, Z, n, H, W).
+static std::mutex g_tune_mu;
+static std::unordered_map<std::string, int> g_bp_zb;
+static std::string g_tune_log;
+
+}  // namespace vcbct
+
+struct vcbct_fdk_plan {
+    int device, H, W, n, R, Z, zstart, xfactor, zb;
+    double step, scale, pitch;
+    vcbct::FilterPlan* fp;
+    float* d_ctab;
+    double* d_atab;
+};
+
+namespace vcbct {
+
+static int tune_bp(vcbct_fdk_plan* p) {
+    char key[128];
+    snprintf(key, sizeof key, "bp R%d Z%d n%d H%d W%d", p->R, p->Z, p->n, p->H, p->W);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_bp_zb.find(key);
+        if (it != g_bp_zb.end()) {
+            p->zb = it->second;
+            return 0;
+        }
+    }
+    const char* off = getenv("VCBCT_NO_AUTOTUNE");
+    if (off && *off == '1') {
+        p->zb = 4;
+        return 0;
+    }
+    float *ds = nullptr, *dr = nullptr;
+    const size_t sb = (size_t)p->n * p->H * p->W * sizeof(float), rb = (size_t)p->Z * p->R * p->R * sizeof(float);
+    if (dev_malloc((void**)&ds, sb) != hipSuccess || dev_malloc((void**)&dr, rb) != hipSuccess) {
+        (void)hipFree(ds);
+        return fail(VCBCT_ENOMEM, "autotune buffers");
+    }
+    int rc = launch_zero(ds, sb, nullptr);
+    if (!rc) rc = launch_zero(dr, rb, nullptr);
+    hipEvent_t e0, e1;
+    VCBCT_HIP(hipEventCreate(&e0));
+    VCBCT_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_zb = 4;
+    std::string rep = std::string(key) + ":";
+    for (int zb : {2, 4, 8}) {
+        if (rc) break;
+        rc = launch_bp(zb, p->R, p->Z, p->zstart, p->step, p->scale, dr, p->H, p->W, p->n, ds, p->d_atab, p->xfactor,
+                       p->pitch, nullptr);  // warm
+        float ms = 0;
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < 2 && !rc; ++r)
+            rc = launch_bp(zb, p->R, p->Z, p->zstart, p->step, p->scale, dr, p->H, p->W, p->n, ds, p->d_atab, p->xfactor,
+                           p->pitch, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        char b[64];
+        snprintf(b, sizeof b, " zb%d=%.3fms", zb, ms / 2);
+        rep += b;
+        if (ms < best) { best = ms; best_zb = zb; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(ds);
+    (void)hipFree(dr);
+    if (rc) return rc;
+    p->zb = best_zb;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_bp_zb[key] = best_zb;
+    g_tune_log += rep + " -> zb" + std::to_string(best_zb) + "\n";
+    return 0;
+}
+
+const char* tune_log() { return g_tune_log.c_str(); }
+
+}  // namespace vcbct
+
+using namespace vcbct;
+
+extern "C" int vcbct_fdk_plan_create(vcbct_fdk_plan** out, int H, int W, int n, int R, int Z, int zstart, double step,
+                                     double scale, int xfactor, const double* angles, const double* xshift,
+                                     const double* yshift, double pitch_iso, int device) {
+    if (!out) return fail(VCBCT_EINVAL, "fdk_plan_create: out is NULL");
+    *out = nullptr;
+    if (H < 2 || W < 2 || n < 2 || R < 1 || Z < 1 || !angles || !xshift || !yshift)
+        return fail(VCBCT_EINVAL, "fdk_plan_create: bad arguments");
+    VCBCT_HIP(hipSetDevice(device));
+    vcbct_fdk_plan* p = (vcbct_fdk_plan*)calloc(1, sizeof(vcbct_fdk_plan));
+    if (!p) return fail(VCBCT_ENOMEM, "host allocation failed");
+    p->device = device; p->H = H; p->W = W; p->n = n; p->R = R; p->Z = Z; p->zstart = zstart; p->xfactor = xfactor;
+    p->step = step; p->scale = scale; p->pitch = pitch_iso; p->zb = 4;
+    int rc = get_plan(W, pitch_iso, &p->fp);
+    std::vector<float> ct = filter_table(n, W, xshift, yshift);
+    std::vector<double> at = bp_table(n, W, angles, xshift);
+    if (!rc && (dev_malloc((void**)&p->d_ctab, ct.size() * sizeof(float)) != hipSuccess ||
+                dev_malloc((void**)&p->d_atab, at.size() * sizeof(double)) != hipSuccess))
+        rc = fail(VCBCT_ENOMEM, "fdk plan tables");
+    if (!rc && (hipMemcpy(p->d_ctab, ct.data(), ct.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(p->d_atab, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(VCBCT_EHIP, "fdk plan upload");
+    if (!rc) rc = tune_bp(p);
+    if (rc) {
+        (void)hipFree(p->d_ctab);
+        (void)hipFree(p->d_atab);
+        free(p);
+        return rc;
+    }
+    *out = p;
+    return 0;
+}
+
+extern "C" void vcbct_fdk_plan_destroy(vcbct_fdk_plan* p) {
+    if (!p) return;
+    (void)hipSetDevice(p->device);
+    (void)hipFree(p->d_ctab);
+    (void)hipFree(p->d_atab);
+    free(p);
+}
+
+extern "C" int vcbct_fdk_plan_filter(vcbct_fdk_plan* p, const float* dev_proj, float* dev_filtered, void* stream) {
+    if (!p || !dev_proj || !dev_filtered) return fail(VCBCT_EINVAL, "fdk_plan_filter: NULL argument");
+    return launch_filter(p->fp, p->H, p->W, p->n, dev_proj, dev_filtered, p->d_ctab, p->pitch, (hipStream_t)stream);
+}
+
+extern "C" int vcbct_fdk_plan_backproject(vcbct_fdk_plan* p, const float* dev_filtered, float* dev_recon, void* stream) {
+    if (!p || !dev_filtered || !dev_recon) return fail(VCBCT_EINVAL, "fdk_plan_backproject: NULL argument");
+    return launch_bp(p->zb, p->R, p->Z, p->zstart, p->step, p->scale, dev_recon, p->H, p->W, p->n, dev_filtered,
+                     p->d_atab, p->xfactor, p->pitch, (hipStream_t)stream);
+}
+
+extern "C" int vcbct_fdk_plan_execute(vcbct_fdk_plan* p, const float* dev_proj, float* dev_filtered, float* dev_recon,
+                                      void* stream) {
+    int rc = vcbct_fdk_plan_filter(p, dev_proj, dev_filtered, stream);
+    if (rc) return rc;
+    return vcbct_fdk_plan_backproject(p, dev_filtered, dev_recon, stream);
+}
+
+extern "C" int vcbct_fdk_plan_slices_per_thread(const vcbct_fdk_plan* p) { return p ? p->zb : -1; }
+
+
+extern "C" int vcbct_prepare_const_map(int W, double pitch_iso, double* out) {
+    if (W < 1 || !out) return fail(VCBCT_EINVAL, "prepare_const_map: bad arguments");
+    for (int m = 0; m < W; ++m)
+        for (int n = 0; n < W; ++n) out[(size_t)m * W + n] = (m == n) ? 0.0 : ramp_offcentre(m - n, pitch_iso, W);
+    return 0;
+}
+
+extern "C" int vcbct_filter_sinogram(int H, int W, int n, const float* dev_proj, float* dev_out, const double* xshift,
+                                     const double* yshift, double pitch_iso, void* stream) {
+    if (H < 1 || W < 2 || n < 1 || !dev_proj || !dev_out || !xshift || !yshift)
+        return fail(VCBCT_EINVAL, "filter_sinogram: bad arguments");
+    FilterPlan* P;
+    int rc = get_plan(W, pitch_iso, &P);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<float> tab = filter_table(n, W, xshift, yshift);
+    float* d_tab;
+    VCBCT_HIP(dev_malloc((void**)&d_tab, tab.size() * sizeof(float)));
+    VCBCT_HIP(hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    rc = launch_filter(P, H, W, n, dev_proj, dev_out, d_tab, pitch_iso, st);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(d_tab);
+    return rc;
+}
+
+extern "C" int vcbct_reconstruct_image(int R, int Z, int zstart, double step, double scale, float* dev_recon, int H,
+                                       int W, int n, const float* dev_sino, int xfactor, const double* angles,
+                                       const double* xshift, double pitch_iso, void* stream) {
+    if (R < 1 || Z < 1 || H < 2 || W < 2 || n < 2 || !dev_recon || !dev_sino || !angles || !xshift)
+        return fail(VCBCT_EINVAL, "reconstruct_image: bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<double> tab = bp_table(n, W, angles, xshift);
     double* d_tab;
     VCBCT_HIP(dev_malloc((void**)&d_tab, tab.size() * sizeof(double)));
     VCBCT_HIP(hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    dim3 grid((R + 15) / 16, (R + 15) / 16, (Z + ZB - 1) / ZB);
-    hipLaunchKernelGGL(k_backproject, grid, dim3(256), 0, st, dev_recon, dev_sino, R, Z, (double)zstart, step, scale, H,
-                       W, n, d_tab, (double)xfactor, pitch_iso * 800 / W);
-    VCBCT_HIP(hipGetLastError());
-    VCBCT_HIP(hipStreamSynchronize(st));
-    VCBCT_HIP(hipFree(d_tab));
-    return 0;
+    int zb = 4;
+    {
+        char key[128];
+        snprintf(key, sizeof key, "bp R%d Z%d n%d H%d W%d", R, Z, n, H, W);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_bp_zb.find(key);
+        if (it != g_bp_zb.end()) zb = it->second;
+    }
+    int rc = launch_bp(zb, R, Z, zstart, step, scale, dev_recon, H, W, n, dev_sino, d_tab, xfactor, pitch_iso, st);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(d_tab);
+    return rc;
 }
 
 extern "C" int vcbct_filter_sinogram_host(int H, int W, int n, const float* hp, float* ho, const double* xs,

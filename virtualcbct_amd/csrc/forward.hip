@@ -163,17 +163,14 @@ __device__ __forceinline__ Tile tile_of(int nA, int W, int quads_per_xcd) {
     return t;
 }
 
+// One wave: 64 detector rows [64 rg, 64 rg + 64) of column jj at angle a.  Wave-uniform arguments.
 template <int MODE, class VoxT>
-__global__ __launch_bounds__(256) void k_forward(ProjGeom g, const double* __restrict__ tab, const VoxT* __restrict__ vox,
-                                                 const double* __restrict__ mu, const double* __restrict__ pdf, int NE,
-                                                 double inv_xf, int a0, int nA, int quads_per_xcd,
-                                                 float* __restrict__ out, float* __restrict__ aux) {
-    const Tile t = tile_of(nA, g.W, quads_per_xcd);
+__device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __restrict__ tab,
+                                             const VoxT* __restrict__ vox, const double* __restrict__ mu,
+                                             const double* __restrict__ pdf, int NE, double inv_xf, int a, int rg,
+                                             int jj, float* __restrict__ out, float* __restrict__ aux) {
     const int lane = threadIdx.x & 63;
-    const int j = t.rg * 64 + lane;
-    const int jj = t.col;
-    const int a = a0 + t.a;
-    if (jj >= g.W) return;  // whole wave (column) out of range: uniform exit
+    const int j = rg * 64 + lane;
     const bool active = j < g.H;
 
     RayCtx r;
@@ -231,6 +228,72 @@ __global__ __launch_bounds__(256) void k_forward(ProjGeom g, const double* __res
         }
         out[o] = (float)(-log(se / sw));
     }
+}
+
+template <int MODE, class VoxT>
+__global__ __launch_bounds__(256) void k_forward(ProjGeom g, const double* __restrict__ tab, const VoxT* __restrict__ vox,
+                                                 const double* __restrict__ mu, const double* __restrict__ pdf, int NE,
+                                                 double inv_xf, int a0, int nA, int quads_per_xcd,
+                                                 float* __restrict__ out, float* __restrict__ aux) {
+    const Tile t = tile_of(nA, g.W, quads_per_xcd);
+    if (t.col >= g.W) return;  // whole wave (column) out of range: uniform exit
+    forward_wave<MODE, VoxT>(g, tab, vox, mu, pdf, NE, inv_xf, a0 + t.a, t.rg, t.col, out, aux);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Batched forward: many problems (volumes of different slice counts, angle counts and spectra) in one
+// persistent launch.  Work item = one wave = (problem, row group, angle, column).  The column range of
+// every (problem, row group, angle) is split into 8 contiguous chunks, one per XCD; each XCD has its own
+// work counter, so the waves of an XCD walk neighbouring columns of one angle (L2 locality).  A wave
+// that finds its XCD's queue empty steals from the other queues, so every queue drains and every wave
+// exits.
+__device__ __forceinline__ int find_problem(const BatchProblem* __restrict__ pr, int np, int xcd, int item) {
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {  // last problem whose start <= item
+        int mid = (lo + hi + 1) >> 1;
+        if (pr[mid].xcd_start[xcd] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __restrict__ pr, int np,
+                                                       const int* __restrict__ xcd_total, int* __restrict__ counters) {
+    const int home = blockIdx.x & 7;
+    const int lane = threadIdx.x & 63;
+    for (int qi = 0; qi < 8; ++qi) {
+        const int q = (home + qi) & 7;
+        const int total = xcd_total[q];
+        for (;;) {
+            int item = 0;
+            if (lane == 0) item = atomicAdd(&counters[q], 1);
+            item = __shfl(item, 0, 64);
+            if (item >= total) break;
+            const int p = find_problem(pr, np, q, item);
+            const BatchProblem& P = pr[p];
+            const int local = item - P.xcd_start[q];
+            const int cl = local % P.cols_per_xcd;
+            const int ra = local / P.cols_per_xcd;  // rg * nA + a
+            const int a = ra % P.g.nAng, rg = ra / P.g.nAng;
+            const int jj = q * P.cols_per_xcd + cl;
+            if (jj >= P.g.W) continue;
+            if (P.vox_is_f32)
+                forward_wave<VCBCT_MODE_INTENSITY, float4>(P.g, P.tab, (const float4*)P.vox, P.mu, P.pdf, P.NE,
+                                                           P.inv_xf, a, rg, jj, P.out, nullptr);
+            else
+                forward_wave<VCBCT_MODE_INTENSITY, uint4>(P.g, P.tab, (const uint4*)P.vox, P.mu, P.pdf, P.NE,
+                                                          P.inv_xf, a, rg, jj, P.out, nullptr);
+        }
+    }
+}
+
+int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xcd_total, int* d_counters, int nblocks,
+                         hipStream_t st) {
+    int rc = launch_zero(d_counters, 8 * sizeof(int), st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_forward_batch, dim3(nblocks), dim3(256), 0, st, d_problems, np, d_xcd_total, d_counters);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -39,6 +39,22 @@ struct Spectrum {
 
 struct vcbct_projector_impl;
 
+// One problem of a batched forward launch (device-resident descriptor).
+struct BatchProblem {
+    ProjGeom g;
+    const double* tab;
+    const void* vox;
+    const double* mu;
+    const double* pdf;
+    float* out;
+    double inv_xf;
+    int NE, vox_is_f32;
+    int cols_per_xcd;   // ceil(W / 8)
+    int xcd_start[8];   // first work item of this problem in each XCD queue
+};
+int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xcd_total, int* d_counters, int nblocks,
+                         hipStream_t st);
+
 // Kernels launchers (forward.hip)
 int launch_transpose_u16(const uint16_t* src, uint16_t* dst, const ProjGeom& g, hipStream_t st);
 int launch_transpose_f32(const float* src, float* dst, const ProjGeom& g, hipStream_t st);

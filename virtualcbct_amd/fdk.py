@@ -108,3 +108,49 @@ def to_short(recon):
     """(short) truncation toward zero of the FP volume (F/main.cpp:179-185)."""
     r = np.asarray(recon)
     return np.trunc(r).astype(np.int16)
+
+
+class FdkPlan:
+    """Per-geometry FDK plan (tables and filter response built once; execution allocates nothing and can be
+    captured in a hipGraph).  The backprojector's slices-per-thread is autotuned at creation."""
+
+    def __init__(self, projHeight, projWidth, nProj, reconSize, reconSlices, zstart, reconStep, reconScale, xfactor,
+                 anglesRad, xshift, yshift, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, device=0):
+        a, xs, ys = [np.ascontiguousarray(v, np.float64) for v in (anglesRad, xshift, yshift)]
+        if min(a.size, xs.size, ys.size) < nProj:
+            raise ValueError("angles/xshift/yshift need one entry per projection")
+        h = C.c_void_p()
+        check(lib().vcbct_fdk_plan_create(C.byref(h), int(projHeight), int(projWidth), int(nProj), int(reconSize),
+                                          int(reconSlices), int(zstart), float(reconStep), float(reconScale),
+                                          int(xfactor), ptr(a, P_D), ptr(xs, P_D), ptr(ys, P_D),
+                                          float(DETECTOR_PITCH_CM_AT_ISO_CENT), int(device)))
+        self.h = h
+        self.shape_proj = (int(nProj), int(projHeight), int(projWidth))
+        self.shape_recon = (int(reconSlices), int(reconSize), int(reconSize))
+        self.device = int(device)
+
+    @property
+    def slices_per_thread(self):
+        return lib().vcbct_fdk_plan_slices_per_thread(self.h)
+
+    def execute(self, proj, filtered=None, recon=None, stream=None):
+        """filter + backprojection; recon (FP32 [Z][R][R]) is accumulated into (zeroed if created here)."""
+        torch = _torch()
+        if filtered is None:
+            filtered = torch.empty(self.shape_proj, dtype=torch.float32, device=proj.device)
+        if recon is None:
+            recon = torch.zeros(self.shape_recon, dtype=torch.float32, device=proj.device)
+        check(lib().vcbct_fdk_plan_execute(self.h, C.c_void_p(proj.data_ptr()), C.c_void_p(filtered.data_ptr()),
+                                           C.c_void_p(recon.data_ptr()), _stream(stream)))
+        return recon
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vcbct_fdk_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
