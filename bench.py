@@ -98,7 +98,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline forward only (used under the profiler)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (scale for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28  # 2^28 floats = 1 GiB
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
@@ -218,6 +230,71 @@ def main():
                            "roofline_frac": adj_bytes / (np.median(ta) * 1e-3) / 1e9 / HBM_PEAK_GBS}
         log(f"adjoint: {np.median(ta):.3f} ms/gradient")
         P4.close()
+
+    if not args.no_extras:
+        # batched: eight phantoms of uneven length (64 .. 256 slices), 360 angles, 91 bins — one persistent
+        # launch (and its hipGraph replay) against the plain per-phantom launches
+        from virtualcbct_amd import phantoms, io
+        from virtualcbct_amd.batch import Batch, autotune_log
+        zs = (64, 96, 128, 160, 192, 224, 240, 256)
+        ang3, xs3, ys3 = phantoms.flexmap_angles(360)
+        c3, s3, x3, y3 = io.trig_table(400, 400, ang3, xs3, ys3)
+        projs, outs3 = [], []
+        for i, Z in enumerate(zs):
+            volt = phantoms.ellipsoid_phantom_torch(512, Z, 0.05, 0.1, 3000 + i, device=dev)
+            Pi = Projector(512, 0.05, Z, 0.1, 400, 400, 360, c3, s3, x3, y3)
+            Pi.set_volume(volt)
+            Pi.set_spectrum(inp["atten"], inp["pdf"])
+            del volt
+            projs.append(Pi)
+            outs3.append(torch.empty(Pi.shape_proj, dtype=torch.float32, device=dev))
+        torch.cuda.synchronize()
+        for Pi, o in zip(projs, outs3):
+            Pi.forward(1000.0, out=o)
+        torch.cuda.synchronize()
+        plain_ms = []
+        for _ in range(2):
+            e[0].record(stream)
+            for Pi, o in zip(projs, outs3):
+                Pi.forward(1000.0, out=o)
+            e[1].record(stream)
+            torch.cuda.synchronize()
+            plain_ms.append(e[0].elapsed_time(e[1]))
+        ref0 = outs3[3].clone()
+        B = Batch()
+        for Pi, o in zip(projs, outs3):
+            B.add_forward(Pi, o, 1000.0)
+        B.run()  # finalises and autotunes the persistent grid
+        torch.cuda.synchronize()
+        batch_ms = []
+        for _ in range(2):
+            e[0].record(stream)
+            B.run()
+            e[1].record(stream)
+            torch.cuda.synchronize()
+            batch_ms.append(e[0].elapsed_time(e[1]))
+        same = bool(torch.equal(outs3[3], ref0))
+        B.capture()
+        graph_ms = []
+        for _ in range(2):
+            e[0].record(stream)
+            B.replay()
+            e[1].record(stream)
+            torch.cuda.synchronize()
+            graph_ms.append(e[0].elapsed_time(e[1]))
+        rays3 = 360 * 400 * 400 * len(zs)
+        line["batched"] = {"workload": "8 ellipsoid phantoms 512^2 x {64..256} slices, 360 angles, 400^2, 91 bins",
+                           "plain_ms": float(min(plain_ms)), "batched_ms": float(min(batch_ms)),
+                           "graph_ms": float(min(graph_ms)),
+                           "plain_rays_bins_per_s": rays3 * NE / (min(plain_ms) * 1e-3),
+                           "batched_rays_bins_per_s": rays3 * NE / (min(batch_ms) * 1e-3),
+                           "speedup": float(min(plain_ms) / min(batch_ms)), "identical_to_plain": same,
+                           "persistent_blocks": B.blocks, "autotune": autotune_log().strip().split("\n")}
+        log(f"batched: plain {min(plain_ms):.2f} ms, persistent {min(batch_ms):.2f} ms, graph {min(graph_ms):.2f} ms")
+        B.close()
+        for Pi in projs:
+            Pi.close()
+        del outs3
 
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

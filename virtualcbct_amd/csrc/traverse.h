@@ -155,6 +155,9 @@ __host__ __device__ __forceinline__ void z_step_range(const ProjGeom& g, const R
 // inside the inscribed circle).  `on` predicates lanes.
 template <class V>
 __host__ __device__ __forceinline__ void charge(const ProjGeom& g, const RayCtx& r, double ua, double ub, bool on, V& vis) {
+    // The per-segment arithmetic may fuse multiply-adds: it already differs from the reference's operation
+    // order (slopes instead of divisions), and the set-up above, whose roundings are restated, stays unfused.
+#pragma clang fp contract(fast)
     if (!on) return;
     const int nxm = (int)(g.N * 0.5), nym = nxm, nzm = (int)(g.Nz * 0.5);
     const double inv_s = 1.0 / g.s, inv_t = 1.0 / g.t;
@@ -180,6 +183,7 @@ __host__ __device__ __forceinline__ void charge(const ProjGeom& g, const RayCtx&
 template <class V>
 __host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
                                                        int kmax, V& vis) {
+#pragma clang fp contract(fast)
     if (kmin > kmax) return;  // no lane of this wave can reach the volume's z band
 
     const double lim_u = double((int)(g.N * 0.5)) * g.s;  // nxm == nym

@@ -80,6 +80,44 @@ def ellipsoid_phantom(N, Nz, scale_cm, step_cm, seed, n_ell=40, n_labels=8):
     return vol
 
 
+def ellipsoid_phantom_torch(N, Nz, scale_cm, step_cm, seed, n_ell=40, n_labels=8, device="cuda"):
+    """The same phantom as ellipsoid_phantom (same seeded parameters, same label rule), voxelised on the device
+    with torch for large volumes; returns an int16 device tensor [z][mat][y][x] holding the uint16 values."""
+    import torch
+    rng = np.random.Generator(np.random.PCG64(seed))
+    frac = rng.dirichlet(np.ones(6), size=n_labels)
+    frac[-2:, 4:] *= 10.0
+    frac /= frac.sum(1, keepdims=True)
+    dens = rng.uniform(0.9, 1.9, n_labels)
+    half_xy = N * scale_cm / 2
+    half_z = Nz * step_cm / 2
+    zc = torch.tensor((np.arange(Nz) - Nz / 2 + 0.5) * step_cm, dtype=torch.float64, device=device)
+    yc = torch.tensor((np.arange(N) - N / 2 + 0.5) * scale_cm, dtype=torch.float64, device=device)
+    label = torch.full((Nz, N, N), -1, dtype=torch.int16, device=device)
+    Zg, Yg, Xg = torch.meshgrid(zc, yc, yc, indexing="ij")
+    for k in range(n_ell):
+        c = rng.uniform([-0.5 * half_xy, -0.5 * half_xy, -0.7 * half_z], [0.5 * half_xy, 0.5 * half_xy, 0.7 * half_z])
+        ax = rng.uniform([0.1 * half_xy, 0.1 * half_xy, 0.1 * half_z], [0.45 * half_xy, 0.45 * half_xy, 0.6 * half_z])
+        th = rng.uniform(0, np.pi)
+        lab = int(rng.integers(0, n_labels))
+        ct, st = np.cos(th), np.sin(th)
+        dx, dy, dz = Xg - c[0], Yg - c[1], Zg - c[2]
+        u = (ct * dx + st * dy) / ax[0]
+        v = (-st * dx + ct * dy) / ax[1]
+        w = dz / ax[2]
+        label[(u * u + v * v + w * w) <= 1.0] = lab
+        del dx, dy, dz, u, v, w
+    vals = np.zeros((n_labels + 1, 6), np.int32)
+    for lab in range(n_labels):
+        for m in range(6):
+            vals[lab, m] = int(frac[lab, m] * dens[lab] * 1000.0)
+    vals[n_labels, 2] = 1  # air (label -1 maps to the last row)
+    lut = torch.tensor(vals, dtype=torch.int32, device=device)
+    idx = torch.where(label < 0, torch.full_like(label, n_labels), label).long()
+    vol = lut[idx]                       # [z][y][x][mat]
+    return vol.permute(0, 3, 1, 2).contiguous().to(torch.int16)
+
+
 def write_slices(vol, prefix):
     """Writes <prefix>_%03d.raw, one [mat][y][x] uint16 slice per file (P/projectionData.cpp:197)."""
     os.makedirs(os.path.dirname(prefix) or ".", exist_ok=True)
