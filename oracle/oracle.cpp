@@ -551,6 +551,24 @@ int orc_adjoint(const orc_geom* gp, const double* cos_t, const double* sin_t, co
     return 0;
 }
 
+/* Debug aid: the charged segments of one ray (xmc, ymc, zmc, rr, ph per row of `out`, capacity cap). */
+int orc_trace_ray(const orc_geom* gp, const double* cos_t, const double* sin_t, const double* xc, const double* yc,
+                  int a, int j, int jj, double* out, int cap) {
+    const orc_geom& g = *gp;
+    Tracer tr(g);
+    std::vector<Seg> segs;
+    Ray R = make_ray(g, cos_t, sin_t, xc, yc, a, j, jj);
+    tr.trace(R, segs);
+    int n = 0;
+    for (const Seg& sg : segs) {
+        if (n >= cap) break;
+        out[5 * n + 0] = sg.xmc; out[5 * n + 1] = sg.ymc; out[5 * n + 2] = sg.zmc;
+        out[5 * n + 3] = sg.rr; out[5 * n + 4] = sg.ph;
+        ++n;
+    }
+    return (int)segs.size();
+}
+
 void orc_const_map(int W, double pitch_iso, double* cmap) {
     /* F/filterSinogram.cpp:65-120 with MAX_PIC_NUM 800, CUTOFF_FREQ 1.6, GAMMA 1, ALPHA 0.5 */
     const double CUT = 1.60, GAM = 1.00, ALP = 0.50;

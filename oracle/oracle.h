@@ -84,6 +84,8 @@ int orc_adjoint(const orc_geom* g, const double* cos_t, const double* sin_t, con
                 const double* yc, const float* vol_f32, const float* atten, const float* pdf, int NE,
                 double xfactor, const double* target, double* grad, double* loss);
 
+int orc_trace_ray(const orc_geom* g, const double* cos_t, const double* sin_t, const double* xc, const double* yc,
+                  int a, int j, int jj, double* out, int cap);
 void orc_const_map(int W, double pitch_iso, double* cmap /* [W][W] */);
 void orc_filter_sinogram(int H, int W, int n_ang, const float* proj, float* out, const double* cmap,
                          const double* xshift, const double* yshift, double pitch_iso);

@@ -60,3 +60,43 @@ extern "C" int ht_line_integrals(int N, double s, int Nz, double t, int H, int W
             }
     return 0;
 }
+
+namespace {
+struct HostRec {
+    double* out;
+    int cap, n;
+    void seg(int xmc, int ymc, int zmc, double rr) {
+        if (n < cap) {
+            out[5 * n + 0] = xmc; out[5 * n + 1] = ymc; out[5 * n + 2] = zmc; out[5 * n + 3] = rr; out[5 * n + 4] = 1;
+        }
+        ++n;
+    }
+};
+}  // namespace
+
+// Charged segments of one ray, with the wave-wide step range of its 64-row wave.
+extern "C" int ht_trace_ray(int N, double s, int Nz, double t, int H, int W, int nAng, double x_reg, double y_reg,
+                            double z_reg, double pitch, double pitch_iso, double sad, double pixnum, int nmat,
+                            const double* tab, int a, int j, int jj, double* out, int cap) {
+    vcbct::ProjGeom g;
+    g.N = N; g.Nz = Nz; g.H = H; g.W = W; g.nAng = nAng; g.nmat = nmat;
+    g.s = s; g.t = t; g.x_reg = x_reg; g.y_reg = y_reg; g.z_reg = z_reg;
+    g.pitch = pitch; g.pitch_iso = pitch_iso; g.sad = sad; g.pixnum = pixnum;
+    const int rg = j / 64;
+    int kmin = INT_MAX, kmax = INT_MIN;
+    vcbct::RayCtx me;
+    for (int l = 0; l < 64; ++l) {
+        int jl = rg * 64 + l;
+        bool act = jl < H;
+        vcbct::RayCtx r;
+        vcbct::ray_setup(g, tab[a], tab[nAng + a], tab[2 * nAng + a], tab[3 * nAng + a], act ? jl : 0, jj, r);
+        int lo, hi;
+        vcbct::z_step_range(g, r, act, lo, hi);
+        kmin = std::min(kmin, lo);
+        kmax = std::max(kmax, hi);
+        if (jl == j) me = r;
+    }
+    HostRec rec{out, cap, 0};
+    vcbct::traverse_lane(g, me, true, kmin, kmax, rec);
+    return rec.n;
+}

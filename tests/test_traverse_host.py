@@ -10,14 +10,17 @@ import pytest
 from oracle import oracle as O
 from virtualcbct_amd import phantoms
 
-LIB = os.path.join(os.path.dirname(__file__), "host_traverse", "libhosttrav.so")
+HERE = os.path.join(os.path.dirname(__file__), "host_traverse")
+LIB = os.path.join(HERE, "libhosttrav.so")
+LIBS = {"plain": os.path.join(HERE, "libhosttrav.so"), "fma": os.path.join(HERE, "libhosttrav_fma.so")}
 D = C.POINTER(C.c_double)
 
 
-def host_line_integrals(g, vol):
-    if not os.path.exists(LIB):
-        pytest.skip("host traversal harness not built")
-    L_ = C.CDLL(LIB)
+def host_line_integrals(g, vol, lib=None):
+    lib = lib or LIB
+    if not os.path.exists(lib):
+        pytest.skip(f"host traversal harness not built: {lib}")
+    L_ = C.CDLL(lib)
     cs, sn, xc, yc = g.tables()
     tab = np.ascontiguousarray(np.concatenate([cs, sn, xc, yc]))
     L = np.zeros((g.n_ang, g.H, g.W, g.nmat))
@@ -40,23 +43,47 @@ def geom(N, Nz, s, t, H, W, angles_deg, xs=0.0, z_reg=0.0):
 ANGLES = [-170.0001, -90.00001, -40.00001, 9.99999, 45.0, 59.99999, 133.3, 179.99999]
 
 
+@pytest.mark.parametrize("build", ["plain", "fma"])
 @pytest.mark.parametrize("case", [
     (32, 8, 0.5, 0.5, 24, 24, 0.3, 0.0123),
     (32, 8, 0.5, 0.5, 24, 24, 0.0, 0.0),       # centre column on the rotation axis: exact y = 0 ties
     (64, 16, 0.25, 0.3, 80, 70, -0.7, 0.3),    # z_reg shift, non-square detector
     (48, 100, 0.3, 0.05, 130, 20, 0.0, 0.0),   # many thin slices: many z crossings per ray
     (41, 9, 0.37, 0.21, 33, 29, 0.11, -0.2),   # odd sizes, box half-width not on the plane grid
+    (16, 4, 0.8, 1.2, 14, 14, 0.0, 0.01),      # planes offset by s/2 from the voxel grid: every midpoint a tie
 ])
-def test_host_traversal_matches_oracle(case):
+def test_host_traversal_matches_oracle(case, build):
     N, Nz, s, t, H, W, zr, xs = case
     rng = np.random.default_rng(N * 1000 + Nz)
     g = geom(N, Nz, s, t, H, W, ANGLES, xs, zr)
     vol = rng.integers(0, 2000, (Nz, 6, N, N)).astype(np.uint16)
     Lo, _, _ = O.line_integrals(g, vol)
-    Lh, _ = host_line_integrals(g, vol)
+    Lh, _ = host_line_integrals(g, vol, LIBS[build])
     den = np.maximum(np.abs(Lo).sum(-1), 1e-9)
     err = np.abs(Lo - Lh).sum(-1) / den
     assert err.max() < 1e-8, np.unravel_index(err.argmax(), err.shape)
+
+
+@pytest.mark.parametrize("build", ["plain", "fma"])
+def test_host_traversal_random_geometries(build):
+    # seeded sweep of small geometries: pitches that do and do not divide 1 cm, offsets, z shifts, odd sizes
+    rng = np.random.default_rng(777)
+    for it in range(60):
+        N = int(rng.integers(8, 40))
+        s = float(rng.choice([0.05, 0.1, 0.2, 0.25, 0.3, 0.4, 0.5, 0.8, 1.0, 0.37]))
+        Nz = int(rng.integers(2, 24))
+        t = float(rng.choice([0.1, 0.2, 0.5, 1.0, 0.33]))
+        H, W = int(rng.integers(8, 40)), int(rng.integers(8, 40))
+        ang = list(rng.uniform(-180, 180, 5)) + [-90.00001, 45.0, 179.99999]
+        xs = float(rng.choice([0.0, 0.013, -0.2, 1.15]))
+        zr = float(rng.choice([0.0, 0.3, -0.45]))
+        g = geom(N, Nz, s, t, H, W, ang, xs, zr)
+        vol = rng.integers(0, 2000, (Nz, 6, N, N)).astype(np.uint16)
+        Lo, _, _ = O.line_integrals(g, vol)
+        Lh, _ = host_line_integrals(g, vol, LIBS[build])
+        den = np.maximum(np.abs(Lo).sum(-1), 1e-9)
+        err = np.abs(Lo - Lh).sum(-1) / den
+        assert err.max() < 1e-8, (it, N, s, Nz, t, H, W, xs, zr, np.unravel_index(err.argmax(), err.shape))
 
 
 def test_host_traversal_c1_geometry_one_angle():
@@ -70,5 +97,21 @@ def test_host_traversal_c1_geometry_one_angle():
                    xc=xc[sel], yc=yc[sel])
     Lo, _, _ = O.line_integrals(g, vol)
     Lh, _ = host_line_integrals(g, vol)
+    den = np.maximum(np.abs(Lo).sum(-1), 1e-9)
+    assert (np.abs(Lo - Lh).sum(-1) / den).max() < 1e-8
+
+
+@pytest.mark.parametrize("build", ["plain", "fma"])
+def test_host_traversal_c2_offset_geometry(build):
+    # the 115 mm half-fan offset workload at two angles (one with lumped-entry ties at the cylinder edge)
+    w = phantoms.WORKLOADS["c2"]
+    vol = phantoms.water_cylinder(w["N"], w["Nz"], w["s"], w["radius"])
+    ang, xs, ys = phantoms.flexmap_angles(360, w["xshift_proj"])
+    cs, sn, xc, yc = O.trig_table_proj(w["W"], w["H"], ang, xs, ys)
+    sel = [45, 200]
+    g = O.Geometry(N=w["N"], s=w["s"], Nz=w["Nz"], t=w["t"], H=w["H"], W=w["W"], cos_t=cs[sel], sin_t=sn[sel],
+                   xc=xc[sel], yc=yc[sel])
+    Lo, _, _ = O.line_integrals(g, vol)
+    Lh, _ = host_line_integrals(g, vol, LIBS[build])
     den = np.maximum(np.abs(Lo).sum(-1), 1e-9)
     assert (np.abs(Lo - Lh).sum(-1) / den).max() < 1e-8

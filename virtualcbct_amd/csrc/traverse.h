@@ -151,6 +151,23 @@ __host__ __device__ __forceinline__ void z_step_range(const ProjGeom& g, const R
     khi = (int)fmin(fmax(b, 0.0), 1.0e8);
 }
 
+// ((a + b) * 0.5) / s + n truncated, unfused, in the reference's operation order.
+__host__ __device__ __forceinline__ int exact_index(double a, double b, double s, int n) {
+#pragma clang fp contract(off)
+    double m = (a + b) * 0.5;
+    double q = m / s;
+    return (int)(q + n);
+}
+
+// Plane k of the major axis, xin + pha * ik * s in the reference's order (P/reprojectionOnGPU.cu:214), unfused:
+// its value feeds the tie-deciding index above.
+__host__ __device__ __forceinline__ double plane_u(const RayCtx& r, int k, double s) {
+#pragma clang fp contract(off)
+    double t = r.pha * (double)k;
+    t = t * s;
+    return r.uin + t;
+}
+
 // Segment visitor interface: V::seg(xmc, ymc, zmc, rr) is called for every charged segment (in range and
 // inside the inscribed circle).  `on` predicates lanes.
 template <class V>
@@ -170,6 +187,19 @@ __host__ __device__ __forceinline__ void charge(const ProjGeom& g, const RayCtx&
     int xmc = (int)(xm * inv_s + nxm);
     int ymc = (int)(-ym * inv_s + nym);
     int zmc = (int)(-zm * inv_t + nzm);
+    // The major-axis index can sit exactly on a voxel boundary for whole families of segments: the lumped
+    // entry segment whenever the ray enters through a major-axis face ((12.8 + 12) / 2 = 248 s), and every
+    // plane-to-plane segment when the plane grid is offset by s/2 from the voxel grid.  Near such a tie the
+    // index is recomputed with the reference's own expression, ((a + b) * 0.5) / s + n, which decides it
+    // (P/reprojectionOnGPU.cu:256-257); the fast value is within a few ulps of it otherwise.
+    {
+        const double fu = r.xmaj ? (xm * inv_s + nxm) : (-ym * inv_s + nym);
+        if (fabs(fu - rint(fu)) < 1e-9) {
+            const int iu = r.xmaj ? exact_index(ua, ub, g.s, nxm) : exact_index(-ua, -ub, g.s, nym);
+            if (r.xmaj) xmc = iu;
+            else ymc = iu;
+        }
+    }
     if ((unsigned)xmc < (unsigned)g.N && (unsigned)ymc < (unsigned)g.N && (unsigned)zmc < (unsigned)g.Nz &&
         xm * xm + ym * ym < rmax * rmax) {
         double rr = fabs(ub - ua) * r.K;
@@ -202,7 +232,7 @@ __host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const 
         kstart = 1;
         uprev = u0;
     } else {
-        uprev = r.uin + r.pha * (double)(kstart - 1) * g.s;
+        uprev = plane_u(r, kstart - 1, g.s);
     }
     double vprev = (uprev - r.su) * r.dv + r.sv;
     double zprev = (uprev - r.su) * r.dz + r.sz;
@@ -213,7 +243,7 @@ __host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const 
     for (int k = kstart; k <= kmax; ++k) {
         // the reference continues while the previous plane lies in the box (inclusive)
         if (!(uprev <= lim_u && uprev >= -lim_u && vprev <= lim_u && vprev >= -lim_u)) break;
-        double uk = r.uin + r.pha * (double)k * g.s;
+        double uk = plane_u(r, k, g.s);
         double vk = (uk - r.su) * r.dv + r.sv;
         double zk = (uk - r.su) * r.dz + r.sz;
         int vma = (int)(vk * inv_s);
