@@ -289,8 +289,8 @@ def main():
                            "plain_rays_bins_per_s": rays3 * NE / (min(plain_ms) * 1e-3),
                            "batched_rays_bins_per_s": rays3 * NE / (min(batch_ms) * 1e-3),
                            "speedup": float(min(plain_ms) / min(batch_ms)), "identical_to_plain": same,
-                           "persistent_blocks": B.blocks, "autotune": autotune_log().strip().split("\n")}
-        log(f"batched: plain {min(plain_ms):.2f} ms, persistent {min(batch_ms):.2f} ms, graph {min(graph_ms):.2f} ms")
+                           "batch_grid": ("flat" if B.blocks == 0 else f"persistent {B.blocks} blocks"), "autotune": autotune_log().strip().split("\n")}
+        log(f"batched: plain {min(plain_ms):.2f} ms, batched {min(batch_ms):.2f} ms, graph {min(graph_ms):.2f} ms")
         B.close()
         for Pi in projs:
             Pi.close()

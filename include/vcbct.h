@@ -182,6 +182,8 @@ int vcbct_batch_add_fdk(vcbct_batch* b, vcbct_fdk_plan* plan, const float* dev_p
 int vcbct_batch_run(vcbct_batch* b, void* stream);
 int vcbct_batch_capture_graph(vcbct_batch* b);
 int vcbct_batch_launch_graph(vcbct_batch* b, void* stream);
+/* Grid the autotuner chose for the batch's forward problems: the persistent grid size in blocks, or 0 for
+ * the flat grid (every problem's blocks laid end to end, no work queue); -1 for NULL. */
 int vcbct_batch_blocks(const vcbct_batch* b);
 /* Text log of the autotuner's decisions (candidates and timings); returns its length. */
 int vcbct_autotune_log(char* buf, int capacity);

@@ -17,8 +17,14 @@ struct HostAcc {
     int N, Nz, nmat;
     double L[VCBCT_MAX_MAT];
     int nseg;
-    void seg(int xmc, int ymc, int zmc, double rr) {
+    void step(const vcbct::Seg& a, const vcbct::Seg& b, const vcbct::Seg& c) {
+        if (a.ok) seg(a.cell / Nz, a.cell % Nz, a.rr);
+        if (b.ok) seg(b.cell / Nz, b.cell % Nz, b.rr);
+        if (c.ok) seg(c.cell / Nz, c.cell % Nz, c.rr);
+    }
+    void seg(int xy, int zmc, double rr) {
         const size_t n2 = (size_t)N * N;
+        const int ymc = xy / N, xmc = xy % N;
         for (int m = 0; m < nmat; ++m) L[m] += (double)vol[((size_t)zmc * nmat + m) * n2 + (size_t)ymc * N + xmc] * rr;
         ++nseg;
     }
@@ -64,8 +70,14 @@ extern "C" int ht_line_integrals(int N, double s, int Nz, double t, int H, int W
 namespace {
 struct HostRec {
     double* out;
-    int cap, n;
-    void seg(int xmc, int ymc, int zmc, double rr) {
+    int cap, n, N, Nz;
+    void step(const vcbct::Seg& a, const vcbct::Seg& b, const vcbct::Seg& c) {
+        if (a.ok) seg(a.cell / Nz, a.cell % Nz, a.rr);
+        if (b.ok) seg(b.cell / Nz, b.cell % Nz, b.rr);
+        if (c.ok) seg(c.cell / Nz, c.cell % Nz, c.rr);
+    }
+    void seg(int xy, int zmc, double rr) {
+        const int ymc = xy / N, xmc = xy % N;
         if (n < cap) {
             out[5 * n + 0] = xmc; out[5 * n + 1] = ymc; out[5 * n + 2] = zmc; out[5 * n + 3] = rr; out[5 * n + 4] = 1;
         }
@@ -96,7 +108,7 @@ extern "C" int ht_trace_ray(int N, double s, int Nz, double t, int H, int W, int
         kmax = std::max(kmax, hi);
         if (jl == j) me = r;
     }
-    HostRec rec{out, cap, 0};
+    HostRec rec{out, cap, 0, N, Nz};
     vcbct::traverse_lane(g, me, true, kmin, kmax, rec);
     return rec.n;
 }

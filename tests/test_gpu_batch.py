@@ -49,7 +49,7 @@ def test_batch_forward_equals_plain(torch):
     torch.cuda.synchronize()
     for o, p in zip(outs, plain):
         assert torch.equal(o, p)
-    assert b.blocks > 0
+    assert b.blocks >= 0  # 0 = flat grid chosen, > 0 = persistent grid size
     print(autotune_log())
     # one of them against the oracle
     P, vol, atten, pdf, g = probs[2]

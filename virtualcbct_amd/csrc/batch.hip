@@ -30,7 +30,8 @@ struct vcbct_batch {
     int* d_total;
     int* d_counters;
     int cap;
-    int nblocks;
+    int nblocks;      // persistent grid size, or 0 for the flat grid
+    int flat_blocks;  // total blocks of the flat grid
     bool dirty;
     struct Fdk {
         vcbct_fdk_plan* plan;
@@ -63,6 +64,7 @@ extern "C" int vcbct_batch_create(vcbct_batch** out, int device) {
     b->d_counters = nullptr;
     b->cap = 0;
     b->nblocks = 0;
+    b->flat_blocks = 0;
     b->dirty = true;
     b->have_graph = false;
     if (hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess) {
@@ -106,6 +108,11 @@ extern "C" int vcbct_batch_add_forward(vcbct_batch* b, vcbct_projector* p, float
     P.out = dev_out;
     P.inv_xf = 1.0 / xfactor;
     P.cols_per_xcd = (p->g.W + 7) / 8;
+    P.quads_per_xcd = ((p->g.W + 3) / 4 + 7) / 8;
+    const long blocks = (long)((p->g.H + 63) / 64) * p->g.nAng * P.quads_per_xcd * 8;
+    if ((long)b->flat_blocks + blocks > 0x7fffffffL) return fail(VCBCT_EINVAL, "batch too large");
+    P.block_start = b->flat_blocks;
+    b->flat_blocks += (int)blocks;
     const long items = (long)((p->g.H + 63) / 64) * p->g.nAng * P.cols_per_xcd;
     for (int q = 0; q < 8; ++q) {
         if ((long)b->totals[q] + items > 0x7fffffffL - 1024) return fail(VCBCT_EINVAL, "batch too large");
@@ -129,10 +136,14 @@ extern "C" int vcbct_batch_add_fdk(vcbct_batch* b, vcbct_fdk_plan* plan, const f
     return 0;
 }
 
+static int batch_forward(vcbct_batch* b, int nblocks, hipStream_t st) {
+    if (b->probs.empty()) return 0;
+    if (nblocks == 0) return launch_forward_flat(b->d_probs, (int)b->probs.size(), b->flat_blocks, st);
+    return launch_forward_batch(b->d_probs, (int)b->probs.size(), b->d_total, b->d_counters, nblocks, st);
+}
+
 static int batch_launch(vcbct_batch* b, int nblocks, hipStream_t st) {
-    int rc = 0;
-    if (!b->probs.empty())
-        rc = launch_forward_batch(b->d_probs, (int)b->probs.size(), b->d_total, b->d_counters, nblocks, st);
+    int rc = batch_forward(b, nblocks, st);
     for (size_t i = 0; !rc && i < b->fdk.size(); ++i) {
         rc = launch_zero(b->fdk[i].rec, b->fdk[i].rec_bytes, st);
         if (!rc) rc = vcbct_fdk_plan_execute(b->fdk[i].plan, b->fdk[i].proj, b->fdk[i].filt, b->fdk[i].rec, st);
@@ -155,36 +166,37 @@ static int batch_finalize(vcbct_batch* b) {
         return fail(VCBCT_ENOMEM, "batch counters");
     if (np) VCBCT_HIP(hipMemcpy(b->d_probs, b->probs.data(), np * sizeof(BatchProblem), hipMemcpyHostToDevice));
     VCBCT_HIP(hipMemcpy(b->d_total, b->totals, 8 * sizeof(int), hipMemcpyHostToDevice));
-    // persistent grid: a multiple of 8 blocks (one queue per XCD), tuned among 2, 4, 6 blocks per CU
+    // candidates: the flat grid (0) and persistent grids of 2, 4, 6 blocks per CU (one queue per XCD)
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
     cus = (cus + 7) / 8 * 8;
-    int chosen = 0;
+    int chosen = -1;
     {
         std::lock_guard<std::mutex> lk(g_btune_mu);
         auto it = g_batch_blocks.find(b->signature);
         if (it != g_batch_blocks.end()) chosen = it->second;
     }
     const char* off = getenv("VCBCT_NO_AUTOTUNE");
-    if (!chosen && np && !(off && *off == '1')) {
+    if (chosen < 0 && np && !(off && *off == '1')) {
         hipEvent_t e0, e1;
         VCBCT_HIP(hipEventCreate(&e0));
         VCBCT_HIP(hipEventCreate(&e1));
         float best = 1e30f;
         std::string rep = "batch " + b->signature + ":";
-        for (int per : {2, 4, 6}) {
+        for (int per : {0, 2, 4, 6}) {
             int nb = cus * per;
-            int rc = launch_forward_batch(b->d_probs, (int)np, b->d_total, b->d_counters, nb, nullptr);
+            int rc = batch_forward(b, nb, nullptr);
             if (rc) return rc;
             (void)hipEventRecord(e0, nullptr);
-            rc = launch_forward_batch(b->d_probs, (int)np, b->d_total, b->d_counters, nb, nullptr);
+            rc = batch_forward(b, nb, nullptr);
             if (rc) return rc;
             (void)hipEventRecord(e1, nullptr);
             (void)hipEventSynchronize(e1);
             float ms = 0;
             (void)hipEventElapsedTime(&ms, e0, e1);
             char s[64];
-            snprintf(s, sizeof s, " %dx%d=%.3fms", per, cus, ms);
+            if (per == 0) snprintf(s, sizeof s, " flat=%.3fms", ms);
+            else snprintf(s, sizeof s, " persistent %dx%d=%.3fms", per, cus, ms);
             rep += s;
             if (ms < best) { best = ms; chosen = nb; }
         }
@@ -192,9 +204,10 @@ static int batch_finalize(vcbct_batch* b) {
         (void)hipEventDestroy(e1);
         std::lock_guard<std::mutex> lk(g_btune_mu);
         g_batch_blocks[b->signature] = chosen;
-        g_btune_log += rep + " -> " + std::to_string(chosen) + " blocks\n";
+        g_btune_log += rep + " -> " + (chosen ? "persistent " + std::to_string(chosen) + " blocks" : std::string("flat")) +
+                       "\n";
     }
-    b->nblocks = chosen ? chosen : cus * 4;
+    b->nblocks = chosen >= 0 ? chosen : 0;
     b->dirty = false;
     return 0;
 }
@@ -245,6 +258,7 @@ extern "C" int vcbct_batch_launch_graph(vcbct_batch* b, void* stream) {
     return 0;
 }
 
+// Persistent grid size chosen for the batch, 0 when the flat grid was chosen, -1 for NULL.
 extern "C" int vcbct_batch_blocks(const vcbct_batch* b) { return b ? b->nblocks : -1; }
 
 extern "C" int vcbct_autotune_log(char* buf, int cap) {

@@ -51,8 +51,9 @@ static int check_geometry(const vcbct_geometry* g) {
     if (!(g->recon_scale > 0) || !(g->recon_step > 0) || !(g->sad_cm > 0) || !(g->det_pitch_cm > 0) ||
         !(g->det_pitch_iso_cm > 0) || !(g->det_pixel_num > 0))
         return fail(VCBCT_EINVAL, "geometry pitches and distances must be positive");
+    // voxel indices are 32-bit in the ray kernels
     double vox = (double)g->recon_size * g->recon_size * g->recon_slices;
-    if (vox * 32.0 > 200e9) return fail(VCBCT_EINVAL, "volume too large for one device");
+    if (vox >= 2147483648.0) return fail(VCBCT_EINVAL, "volume of %.0f voxels exceeds 2^31 - 1", vox);
     return 0;
 }
 

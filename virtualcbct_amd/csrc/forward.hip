@@ -110,37 +110,81 @@ __global__ __launch_bounds__(256) void k_untranspose_f32(const float4* __restric
 // ------------------------------------------------------------------------------------------------
 // Voxel access
 
-__device__ __forceinline__ void load_vox(const uint4* __restrict__ v, size_t idx, double out[VCBCT_MAX_MAT]) {
-    uint4 q = v[idx];
-    out[0] = (double)(q.x & 0xffffu);
-    out[1] = (double)(q.x >> 16);
-    out[2] = (double)(q.y & 0xffffu);
-    out[3] = (double)(q.y >> 16);
-    out[4] = (double)(q.z & 0xffffu);
-    out[5] = (double)(q.z >> 16);
-}
-__device__ __forceinline__ void load_vox(const float4* __restrict__ v, size_t idx, double out[VCBCT_MAX_MAT]) {
-    float4 a = v[2 * idx], b = v[2 * idx + 1];
-    out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w; out[4] = b.x; out[5] = b.y;
-}
-
+// Six material line integrals of one ray, sum_seg w_m * rr, accumulated in FP32 (a ray has a few hundred
+// segments; the relative rounding of the sums stays near 1e-6).  Voxel layout [y][x][z][8]: u16 maps are
+// one 16-byte record per voxel, FP32 maps two.  The gathers of a plane step are all issued before the
+// first of them is consumed, so a wave keeps up to three in flight.
 template <class VoxT>
-struct AccumL {
-    const VoxT* __restrict__ vox;
-    int N, Nz, nmat;
-    double L[VCBCT_MAX_MAT];
-    __device__ __forceinline__ void seg(int xmc, int ymc, int zmc, double rr) {
-        double w[VCBCT_MAX_MAT];
-        load_vox(vox, ((size_t)ymc * N + xmc) * Nz + zmc, w);
-#pragma unroll
-        for (int m = 0; m < VCBCT_MAX_MAT; ++m) L[m] += w[m] * rr;  // padded materials are zero
+struct AccumL;
+
+template <>
+struct AccumL<uint4> {
+    const uint4* __restrict__ vox;
+    int Nz;
+    float L[VCBCT_MAX_MAT];
+    __device__ __forceinline__ void step(const Seg& s0, const Seg& s1, const Seg& s2) {
+#pragma clang fp contract(fast)
+        uint3 q0, q1, q2;  // read only where the matching segment is charged
+        if (s0.ok) q0 = ld(s0);
+        if (s1.ok) q1 = ld(s1);
+        if (s2.ok) q2 = ld(s2);
+        if (s0.ok) add(q0, (float)s0.rr);
+        if (s1.ok) add(q1, (float)s1.rr);
+        if (s2.ok) add(q2, (float)s2.rr);
+    }
+    __device__ __forceinline__ uint3 ld(const Seg& s) const {
+        const uint4 q = vox[s.cell];
+        return make_uint3(q.x, q.y, q.z);
+    }
+    __device__ __forceinline__ void add(const uint3& q, float rr) {
+#pragma clang fp contract(fast)
+        L[0] += (float)(q.x & 0xffffu) * rr;
+        L[1] += (float)(q.x >> 16) * rr;
+        L[2] += (float)(q.y & 0xffffu) * rr;
+        L[3] += (float)(q.y >> 16) * rr;
+        L[4] += (float)(q.z & 0xffffu) * rr;
+        L[5] += (float)(q.z >> 16) * rr;
+    }
+};
+
+template <>
+struct AccumL<float4> {
+    const float4* __restrict__ vox;
+    int Nz;
+    float L[VCBCT_MAX_MAT];
+    struct Q {
+        float4 a;
+        float2 b;
+    };
+    __device__ __forceinline__ void step(const Seg& s0, const Seg& s1, const Seg& s2) {
+        Q q0, q1, q2;  // read only where the matching segment is charged
+        if (s0.ok) q0 = ld(s0);
+        if (s1.ok) q1 = ld(s1);
+        if (s2.ok) q2 = ld(s2);
+        if (s0.ok) add(q0, (float)s0.rr);
+        if (s1.ok) add(q1, (float)s1.rr);
+        if (s2.ok) add(q2, (float)s2.rr);
+    }
+    __device__ __forceinline__ Q ld(const Seg& s) const {
+        const size_t i = 2 * (size_t)s.cell;
+        const float4 b = vox[i + 1];
+        return Q{vox[i], make_float2(b.x, b.y)};
+    }
+    __device__ __forceinline__ void add(const Q& q, float rr) {
+#pragma clang fp contract(fast)
+        L[0] += q.a.x * rr;
+        L[1] += q.a.y * rr;
+        L[2] += q.a.z * rr;
+        L[3] += q.a.w * rr;
+        L[4] += q.b.x * rr;
+        L[5] += q.b.y * rr;
     }
 };
 
 // Counts the charged segments of a ray (the algorithmic work unit of the roofline model).
 struct CountSeg {
     int n = 0;
-    __device__ __forceinline__ void seg(int, int, int, double) { ++n; }
+    __device__ __forceinline__ void step(const Seg& a, const Seg& b, const Seg& c) { n += (int)a.ok + (int)b.ok + (int)c.ok; }
 };
 
 // Work decomposition shared by the ray kernels.  A block = 4 waves = 4 adjacent detector columns of one
@@ -185,11 +229,9 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
     }
     AccumL<VoxT> acc;
     acc.vox = vox;
-    acc.N = g.N;
     acc.Nz = g.Nz;
-    acc.nmat = g.nmat;
 #pragma unroll
-    for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0;
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0f;
     traverse(g, r, active, acc);
     if (!active) return;
 
@@ -287,6 +329,39 @@ __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __res
     }
 }
 
+// Flat batched grid: the blocks of every problem laid end to end (each problem's block count is a multiple
+// of 8, so block b still runs on XCD b % 8 and the per-problem XCD-aware tile mapping carries over).
+__global__ __launch_bounds__(256) void k_forward_flat(const BatchProblem* __restrict__ pr, int np) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {  // last problem whose first block <= b
+        int mid = (lo + hi + 1) >> 1;
+        if (pr[mid].block_start <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchProblem& P = pr[lo];
+    const int local = b - P.block_start;
+    const int xcd = local & 7, l8 = local >> 3;
+    const int q = l8 % P.quads_per_xcd;
+    const int ra = l8 / P.quads_per_xcd;
+    const int a = ra % P.g.nAng, rg = ra / P.g.nAng;
+    const int jj = ((xcd * P.quads_per_xcd + q) << 2) + (threadIdx.x >> 6);
+    if (jj >= P.g.W) return;
+    if (P.vox_is_f32)
+        forward_wave<VCBCT_MODE_INTENSITY, float4>(P.g, P.tab, (const float4*)P.vox, P.mu, P.pdf, P.NE, P.inv_xf, a,
+                                                   rg, jj, P.out, nullptr);
+    else
+        forward_wave<VCBCT_MODE_INTENSITY, uint4>(P.g, P.tab, (const uint4*)P.vox, P.mu, P.pdf, P.NE, P.inv_xf, a, rg,
+                                                  jj, P.out, nullptr);
+}
+
+int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks, hipStream_t st) {
+    if (total_blocks <= 0) return 0;
+    hipLaunchKernelGGL(k_forward_flat, dim3(total_blocks), dim3(256), 0, st, d_problems, np);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+
 int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xcd_total, int* d_counters, int nblocks,
                          hipStream_t st) {
     int rc = launch_zero(d_counters, 8 * sizeof(int), st);
@@ -306,12 +381,18 @@ struct ScatterG {
     float* __restrict__ gvox;
     int N, Nz, nmat;
     float c[VCBCT_MAX_MAT];
-    __device__ __forceinline__ void seg(int xmc, int ymc, int zmc, double rr) {
-        float* p = gvox + (((size_t)ymc * N + xmc) * Nz + zmc) * 8;
-        float rf = (float)rr;
+    __device__ __forceinline__ void seg(const Seg& s) {
+        if (!s.ok) return;
+        float* p = gvox + (size_t)s.cell * 8;
+        float rf = (float)s.rr;
 #pragma unroll
         for (int m = 0; m < VCBCT_MAX_MAT; ++m)
             if (m < nmat && c[m] != 0.f) atomicAdd(p + m, c[m] * rf);
+    }
+    __device__ __forceinline__ void step(const Seg& a, const Seg& b, const Seg& c2) {
+        seg(a);
+        seg(b);
+        seg(c2);
     }
 };
 
@@ -331,11 +412,9 @@ __global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __res
     ray_setup(g, tab[a], tab[g.nAng + a], tab[2 * g.nAng + a], tab[3 * g.nAng + a], active ? j : 0, jj, r);
     AccumL<float4> acc;
     acc.vox = vox;
-    acc.N = g.N;
     acc.Nz = g.Nz;
-    acc.nmat = g.nmat;
 #pragma unroll
-    for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0;
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0f;
     traverse(g, r, active, acc);
 
     ScatterG sc;

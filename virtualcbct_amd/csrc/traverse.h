@@ -168,57 +168,109 @@ __host__ __device__ __forceinline__ double plane_u(const RayCtx& r, int k, doubl
     return r.uin + t;
 }
 
-// Segment visitor interface: V::seg(xmc, ymc, zmc, rr) is called for every charged segment (in range and
-// inside the inscribed circle).  `on` predicates lanes.
-template <class V>
-__host__ __device__ __forceinline__ void charge(const ProjGeom& g, const RayCtx& r, double ua, double ub, bool on, V& vis) {
-    // The per-segment arithmetic may fuse multiply-adds: it already differs from the reference's operation
-    // order (slopes instead of divisions), and the set-up above, whose roundings are restated, stays unfused.
-#pragma clang fp contract(fast)
-    if (!on) return;
-    const int nxm = (int)(g.N * 0.5), nym = nxm, nzm = (int)(g.Nz * 0.5);
-    const double inv_s = 1.0 / g.s, inv_t = 1.0 / g.t;
-    const double rmax = g.N * 0.5 * g.s;
-    double um = (ua + ub) * 0.5;
-    double vm = (um - r.su) * r.dv + r.sv;
-    double zm = (um - r.su) * r.dz + r.sz;
-    double xm = r.xmaj ? um : vm;
-    double ym = r.xmaj ? vm : um;
-    int xmc = (int)(xm * inv_s + nxm);
-    int ymc = (int)(-ym * inv_s + nym);
-    int zmc = (int)(-zm * inv_t + nzm);
-    // The major-axis index can sit exactly on a voxel boundary for whole families of segments: the lumped
-    // entry segment whenever the ray enters through a major-axis face ((12.8 + 12) / 2 = 248 s), and every
-    // plane-to-plane segment when the plane grid is offset by s/2 from the voxel grid.  Near such a tie the
-    // index is recomputed with the reference's own expression, ((a + b) * 0.5) / s + n, which decides it
-    // (P/reprojectionOnGPU.cu:256-257); the fast value is within a few ulps of it otherwise.
-    {
-        const double fu = r.xmaj ? (xm * inv_s + nxm) : (-ym * inv_s + nym);
-        if (fabs(fu - rint(fu)) < 1e-9) {
-            const int iu = r.xmaj ? exact_index(ua, ub, g.s, nxm) : exact_index(-ua, -ub, g.s, nym);
-            if (r.xmaj) xmc = iu;
-            else ymc = iu;
-        }
-    }
-    if ((unsigned)xmc < (unsigned)g.N && (unsigned)ymc < (unsigned)g.N && (unsigned)zmc < (unsigned)g.Nz &&
-        xm * xm + ym * ym < rmax * rmax) {
-        double rr = fabs(ub - ua) * r.K;
-        vis.seg(xmc, ymc, zmc, rr);
-    }
+// Per-ray constants of segment charging.  Voxel indices follow the reference's midpoint truncation,
+// (int)(coord / pitch + n): along the major axis u -> (int)(u * smaj + nmaj), along the minor axis
+// v -> (int)(v * smin + nmaj), in z -> (int)(z * ninv_t + nzm) (x index = x / s + nxm, y index = -y / s + nym,
+// z index = -z / t + nzm; nxm == nym).  A voxel's cell in the [y][x][z] layout is
+// (y * N + x) * Nz + z = imin * mstride + imaj * majmul + z.
+struct ChargeC {
+    double smaj, smin, nmaj, ninv_t, nzm, rmax2;
+    int mstride, majmul;
+};
+
+// imin * mstride with a full-rate 24-bit multiply when both factors fit in 24 bits.
+template <bool MUL24>
+__host__ __device__ __forceinline__ int minor_term(int imin, int mstride) {
+#ifdef __HIP_DEVICE_COMPILE__
+    if (MUL24) return (int)__umul24((unsigned)imin, (unsigned)mstride);
+#endif
+    return (int)((unsigned)imin * (unsigned)mstride);  // wraps harmlessly for uncharged segments
 }
 
-// Walks the compat traversal of one ray (lanes of a wave in lockstep over plane steps).
-// `active`: lane holds a real detector pixel.  Returns nothing; the visitor accumulates.
-// One lane's walk over plane steps [kmin, kmax] (the wave-wide union of the lanes' z ranges).
-template <class V>
-__host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
-                                                       int kmax, V& vis) {
-#pragma clang fp contract(fast)
-    if (kmin > kmax) return;  // no lane of this wave can reach the volume's z band
+__host__ __device__ __forceinline__ ChargeC charge_consts(const ProjGeom& g, const RayCtx& r) {
+    ChargeC c;
+    const double inv_s = 1.0 / g.s;
+    c.smaj = r.xmaj ? inv_s : -inv_s;
+    c.smin = r.xmaj ? -inv_s : inv_s;
+    c.nmaj = (double)(int)(g.N * 0.5);
+    c.ninv_t = -1.0 / g.t;
+    c.nzm = (double)(int)(g.Nz * 0.5);
+    const double rmax = g.N * 0.5 * g.s;
+    c.rmax2 = rmax * rmax;
+    c.mstride = r.xmaj ? g.N * g.Nz : g.Nz;
+    c.majmul = r.xmaj ? g.Nz : g.N * g.Nz;
+    return c;
+}
 
+// Major-axis voxel index of the segment [ua, ub] from its midpoint.  The index can sit exactly on a voxel
+// boundary for whole families of segments: the lumped entry segment whenever the ray enters through a
+// major-axis face ((12.8 + 12) / 2 = 248 s), and every plane-to-plane segment when the plane grid is
+// offset from the voxel grid (s = 0.8 cm: planes at whole cm, boundaries at multiples of 0.8).  Near such a
+// tie the index is recomputed with the reference's own expression, ((a + b) * 0.5) / s + n, which decides
+// it (P/reprojectionOnGPU.cu:256-257); the fast value is within a few ulps of it otherwise.
+__host__ __device__ __forceinline__ int major_index(const ProjGeom& g, const RayCtx& r, const ChargeC& c, double ua,
+                                                    double ub) {
+#pragma clang fp contract(fast)
+    const double um = (ua + ub) * 0.5;
+    const double fu = um * c.smaj + c.nmaj;
+    int iu = (int)fu;
+    if (fabs(fu - rint(fu)) < 1e-9) {
+        const int n = (int)c.nmaj;
+        iu = r.xmaj ? exact_index(ua, ub, g.s, n) : exact_index(-ua, -ub, g.s, n);
+    }
+    return iu;
+}
+
+// One charged segment: voxel cell ((y * N + x) * Nz + z), length; ok = charged by this lane.
+struct Seg {
+    int cell;
+    double rr;
+    bool ok;
+};
+
+// The segment [ua, ub] (P/reprojectionOnGPU.cu:256-269): midpoint voxel by truncation, in-range test,
+// inscribed-circle mask at the midpoint, length |ub - ua| * sqrt(1 + dv^2 + dz^2).  `majterm`/`majok` carry
+// the major-axis part of the cell when the caller knows it for the whole plane step (STEPMAJ); otherwise it
+// is taken from this segment's midpoint.
+template <bool STEPMAJ, bool MUL24>
+__host__ __device__ __forceinline__ Seg make_seg(const ProjGeom& g, const RayCtx& r, const ChargeC& c, double ua,
+                                                 double ub, bool on, int majterm, bool majok) {
+    // The per-segment arithmetic may fuse multiply-adds: it already differs from the reference's operation
+    // order (slopes instead of divisions), and the set-up, whose roundings are restated, stays unfused.
+#pragma clang fp contract(fast)
+    Seg sg;
+    sg.ok = false;
+    if (!on) return sg;  // the other fields are read only where ok is set
+    const double um = (ua + ub) * 0.5;
+    const double d = um - r.su;
+    const double vm = d * r.dv + r.sv;
+    const double zm = d * r.dz + r.sz;
+    const int imin = (int)(vm * c.smin + c.nmaj);
+    const int zmc = (int)(zm * c.ninv_t + c.nzm);
+    if (!STEPMAJ) {
+        const int imaj = major_index(g, r, c, ua, ub);
+        majok = (unsigned)imaj < (unsigned)g.N;
+        majterm = (int)((unsigned)imaj * (unsigned)c.majmul);
+    }
+    sg.ok = majok && (unsigned)imin < (unsigned)g.N && (unsigned)zmc < (unsigned)g.Nz && um * um + vm * vm < c.rmax2;
+    sg.cell = (int)((unsigned)minor_term<MUL24>(imin, c.mstride) + (unsigned)majterm + (unsigned)zmc);
+    sg.rr = fabs(ub - ua) * r.K;
+    return sg;
+}
+
+// Hands the (up to three) segments of one plane step to the visitor.  Visitors implement
+// step(s0, s1, s2) and see each segment with ok set exactly once, in ray order.
+// One lane's walk over plane steps [kmin, kmax] (the wave-wide union of the lanes' z ranges).
+// ALIGNED: the whole-cm plane grid lies on voxel boundaries (1/s is an integer), so every segment of a plane
+// step has the major-axis index of the step's midpoint, computed once per step.
+template <bool ALIGNED, bool MUL24, class V>
+__host__ __device__ __forceinline__ void traverse_steps(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
+                                                        int kmax, V& vis) {
+#pragma clang fp contract(fast)
     const double lim_u = double((int)(g.N * 0.5)) * g.s;  // nxm == nym
     const double inv_s = 1.0 / g.s, inv_t = 1.0 / g.t;
     const bool on = active && r.hit;
+    const ChargeC c = charge_consts(g, r);
 
     // plane 0 and the lumped entry segment (wave-uniform box test)
     double u0 = r.uin;
@@ -228,7 +280,8 @@ __host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const 
     int kstart = kmin;
     double uprev;
     if (kstart <= 1) {  // kmin 0 or 1: the lumped entry segment is needed (and cheap)
-        charge(g, r, r.ui, u0, on, vis);
+        const Seg none{0, 0.0, false};
+        vis.step(make_seg<false, MUL24>(g, r, c, r.ui, u0, on, 0, false), none, none);
         kstart = 1;
         uprev = u0;
     } else {
@@ -238,31 +291,31 @@ __host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const 
     double zprev = (uprev - r.su) * r.dz + r.sz;
     int vmaprev = (int)(vprev * inv_s);
     int zmaprev = (int)(zprev * inv_t);
-    double b0 = uprev;
+    const double ds = r.dirv * g.s, dt = r.dirz * g.t;
 
     for (int k = kstart; k <= kmax; ++k) {
         // the reference continues while the previous plane lies in the box (inclusive)
         if (!(uprev <= lim_u && uprev >= -lim_u && vprev <= lim_u && vprev >= -lim_u)) break;
-        double uk = plane_u(r, k, g.s);
-        double vk = (uk - r.su) * r.dv + r.sv;
-        double zk = (uk - r.su) * r.dz + r.sz;
-        int vma = (int)(vk * inv_s);
-        int zma = (int)(zk * inv_t);
-        bool same_sign = (vk > 0 && vprev > 0) || (vk < 0 && vprev < 0);
-        bool hv = r.vmove && ((vma != vmaprev) || !same_sign);
-        bool hz = (zma != zmaprev);
+        const double uk = plane_u(r, k, g.s);
+        const double duk = uk - r.su;
+        const double vk = duk * r.dv + r.sv;
+        const double zk = duk * r.dz + r.sz;
+        const int vma = (int)(vk * inv_s);
+        const int zma = (int)(zk * inv_t);
+        // sign change of v counts as a crossing (truncation maps (-s, s) to one index)
+        const bool hv = r.vmove && ((vma != vmaprev) || !(vk * vprev > 0.0));
+        const bool hz = zma != zmaprev;
         double uv = uk, uz = uk;
         if (hv) {
-            double vb = ((vk <= 0 && r.dirv < 0) || (vk >= 0 && r.dirv > 0)) ? (double)(vma * g.s)
-                                                                            : (double)(vma * g.s) - r.dirv * g.s;
+            const double vb = (vma * g.s) - ((vk <= 0 && r.dirv < 0) || (vk >= 0 && r.dirv > 0) ? 0.0 : ds);
             uv = (vb - r.sv) * r.duv + r.su;
         }
         if (hz) {
-            double zb = ((zk < 0 && r.dirz < 0) || (zk > 0 && r.dirz > 0)) ? (double)(zma * g.t)
-                                                                          : (double)(zma * g.t) - r.dirz * g.t;
+            const double zb = (zma * g.t) - ((zk < 0 && r.dirz < 0) || (zk > 0 && r.dirz > 0) ? 0.0 : dt);
             uz = (zb - r.sz) * r.duz + r.su;
         }
-        double chv = fabs(uk - uv), chz = fabs(uk - uz);
+        // crossings ordered by distance from the end plane (the farther one comes first)
+        const double chv = fabs(uk - uv), chz = fabs(uk - uz);
         double e1 = uk, e2 = uk;
         if (hv && hz) {
             if (chz > chv) { e1 = uz; e2 = uv; }
@@ -272,14 +325,47 @@ __host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const 
         } else if (hv && chv != 0) {
             e1 = uv;
         }
-        charge(g, r, b0, e1, on, vis);
-        charge(g, r, e1, e2, on && (e2 != e1), vis);
-        charge(g, r, e2, uk, on && (uk != e2), vis);
-        b0 = uk;
+        const double b0 = uprev;
+        if (ALIGNED) {
+            const int imaj = (int)((b0 + uk) * 0.5 * c.smaj + c.nmaj);
+            if ((unsigned)imaj < (unsigned)g.N) {
+                const int mt = imaj * c.majmul;
+                vis.step(make_seg<true, MUL24>(g, r, c, b0, e1, on, mt, true),
+                         make_seg<true, MUL24>(g, r, c, e1, e2, on && (e2 != e1), mt, true),
+                         make_seg<true, MUL24>(g, r, c, e2, uk, on && (uk != e2), mt, true));
+            }
+        } else {
+            vis.step(make_seg<false, MUL24>(g, r, c, b0, e1, on, 0, false),
+                     make_seg<false, MUL24>(g, r, c, e1, e2, on && (e2 != e1), 0, false),
+                     make_seg<false, MUL24>(g, r, c, e2, uk, on && (uk != e2), 0, false));
+        }
         uprev = uk;
         vprev = vk;
         vmaprev = vma;
         zmaprev = zma;
+    }
+}
+
+// Whether the whole-cm plane grid of the stepping loop coincides with the voxel boundaries (1/s integral).
+__host__ __device__ __forceinline__ bool planes_aligned(const ProjGeom& g) {
+    const double q = 1.0 / g.s;
+    return q >= 1.0 && fabs(q - rint(q)) < 1e-9;
+}
+
+// Walks the compat traversal of one ray (lanes of a wave in lockstep over plane steps).
+// `active`: lane holds a real detector pixel.  The visitor accumulates.
+template <class V>
+__host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
+                                                       int kmax, V& vis) {
+    if (kmin > kmax) return;  // no lane of this wave can reach the volume's z band
+    // minor-axis strides up to N * Nz multiply in 24 bits when they fit
+    const bool mul24 = (long long)g.N * g.Nz < (1LL << 24);
+    if (planes_aligned(g)) {
+        if (mul24) traverse_steps<true, true>(g, r, active, kmin, kmax, vis);
+        else traverse_steps<true, false>(g, r, active, kmin, kmax, vis);
+    } else {
+        if (mul24) traverse_steps<false, true>(g, r, active, kmin, kmax, vis);
+        else traverse_steps<false, false>(g, r, active, kmin, kmax, vis);
     }
 }
 

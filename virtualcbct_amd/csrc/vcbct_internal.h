@@ -50,10 +50,13 @@ struct BatchProblem {
     double inv_xf;
     int NE, vox_is_f32;
     int cols_per_xcd;   // ceil(W / 8)
-    int xcd_start[8];   // first work item of this problem in each XCD queue
+    int xcd_start[8];   // first work item of this problem in each XCD queue (persistent form)
+    int quads_per_xcd;  // flat form: ceil(ceil(W / 4) / 8) four-column blocks per XCD per (row group, angle)
+    int block_start;    // flat form: first block of this problem
 };
 int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xcd_total, int* d_counters, int nblocks,
                          hipStream_t st);
+int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks, hipStream_t st);
 
 // Kernels launchers (forward.hip)
 int launch_transpose_u16(const uint16_t* src, uint16_t* dst, const ProjGeom& g, hipStream_t st);
