@@ -38,6 +38,7 @@ extern "C" int ht_line_integrals(int N, double s, int Nz, double t, int H, int W
     g.N = N; g.Nz = Nz; g.H = H; g.W = W; g.nAng = nAng; g.nmat = nmat;
     g.s = s; g.t = t; g.x_reg = x_reg; g.y_reg = y_reg; g.z_reg = z_reg;
     g.pitch = pitch; g.pitch_iso = pitch_iso; g.sad = sad; g.pixnum = pixnum;
+    vcbct::derive_geom(g);
 #pragma omp parallel for collapse(2) schedule(dynamic)
     for (int a = 0; a < nAng; ++a)
         for (int jj = 0; jj < W; ++jj)
@@ -94,6 +95,7 @@ extern "C" int ht_trace_ray(int N, double s, int Nz, double t, int H, int W, int
     g.N = N; g.Nz = Nz; g.H = H; g.W = W; g.nAng = nAng; g.nmat = nmat;
     g.s = s; g.t = t; g.x_reg = x_reg; g.y_reg = y_reg; g.z_reg = z_reg;
     g.pitch = pitch; g.pitch_iso = pitch_iso; g.sad = sad; g.pixnum = pixnum;
+    vcbct::derive_geom(g);
     const int rg = j / 64;
     int kmin = INT_MAX, kmax = INT_MIN;
     vcbct::RayCtx me;

@@ -3,6 +3,7 @@
 // size is chosen by timing candidates on the first run (autotuner, cached per batch signature).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -31,7 +32,9 @@ struct vcbct_batch {
     int* d_counters;
     int cap;
     int nblocks;      // persistent grid size, or 0 for the flat grid
-    int flat_blocks;  // total blocks of the flat grid
+    int flat_blocks;  // running total of flat-grid blocks (range check while problems are added)
+    int n_u16;        // after finalize: problems [0, n_u16) have u16 volumes, the rest FP32 maps
+    int flat_u16, flat_f32;  // flat-grid blocks of each voxel type
     bool dirty;
     struct Fdk {
         vcbct_fdk_plan* plan;
@@ -65,6 +68,8 @@ extern "C" int vcbct_batch_create(vcbct_batch** out, int device) {
     b->cap = 0;
     b->nblocks = 0;
     b->flat_blocks = 0;
+    b->n_u16 = 0;
+    b->flat_u16 = b->flat_f32 = 0;
     b->dirty = true;
     b->have_graph = false;
     if (hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess) {
@@ -109,16 +114,14 @@ extern "C" int vcbct_batch_add_forward(vcbct_batch* b, vcbct_projector* p, float
     P.inv_xf = 1.0 / xfactor;
     P.cols_per_xcd = (p->g.W + 7) / 8;
     P.quads_per_xcd = ((p->g.W + 3) / 4 + 7) / 8;
+    // block and work-item offsets are assigned at finalize (problems grouped by voxel type); here only the
+    // totals are checked against the 32-bit grid and queue ranges
     const long blocks = (long)((p->g.H + 63) / 64) * p->g.nAng * P.quads_per_xcd * 8;
-    if ((long)b->flat_blocks + blocks > 0x7fffffffL) return fail(VCBCT_EINVAL, "batch too large");
-    P.block_start = b->flat_blocks;
-    b->flat_blocks += (int)blocks;
     const long items = (long)((p->g.H + 63) / 64) * p->g.nAng * P.cols_per_xcd;
-    for (int q = 0; q < 8; ++q) {
-        if ((long)b->totals[q] + items > 0x7fffffffL - 1024) return fail(VCBCT_EINVAL, "batch too large");
-        P.xcd_start[q] = b->totals[q];
-        b->totals[q] += (int)items;
-    }
+    if ((long)b->flat_blocks + blocks > 0x7fffffffL || (long)b->totals[0] + items > 0x7fffffffL - 1024)
+        return fail(VCBCT_EINVAL, "batch too large");
+    b->flat_blocks += (int)blocks;
+    b->totals[0] += (int)items;
     b->probs.push_back(P);
     char sig[128];
     snprintf(sig, sizeof sig, "[N%d Z%d A%d H%d W%d E%d]", p->g.N, p->g.Nz, p->g.nAng, p->g.H, p->g.W, p->NE);
@@ -138,7 +141,12 @@ extern "C" int vcbct_batch_add_fdk(vcbct_batch* b, vcbct_fdk_plan* plan, const f
 
 static int batch_forward(vcbct_batch* b, int nblocks, hipStream_t st) {
     if (b->probs.empty()) return 0;
-    if (nblocks == 0) return launch_forward_flat(b->d_probs, (int)b->probs.size(), b->flat_blocks, st);
+    if (nblocks == 0) {  // one flat launch per voxel type present
+        const int np = (int)b->probs.size();
+        int rc = launch_forward_flat(b->d_probs, b->n_u16, b->flat_u16, 0, st);
+        if (!rc) rc = launch_forward_flat(b->d_probs + b->n_u16, np - b->n_u16, b->flat_f32, 1, st);
+        return rc;
+    }
     return launch_forward_batch(b->d_probs, (int)b->probs.size(), b->d_total, b->d_counters, nblocks, st);
 }
 
@@ -154,6 +162,25 @@ static int batch_launch(vcbct_batch* b, int nblocks, hipStream_t st) {
 static int batch_finalize(vcbct_batch* b) {
     if (!b->dirty) return 0;
     const size_t np = b->probs.size();
+    // u16 problems first, then FP32 maps (stable), each type with its own flat grid; the persistent queues
+    // run over all problems in that order
+    std::stable_sort(b->probs.begin(), b->probs.end(),
+                     [](const BatchProblem& x, const BatchProblem& y) { return x.vox_is_f32 < y.vox_is_f32; });
+    b->n_u16 = 0;
+    b->flat_u16 = b->flat_f32 = 0;
+    memset(b->totals, 0, sizeof b->totals);
+    for (BatchProblem& P : b->probs) {
+        const int nrg = (P.g.H + 63) / 64;
+        int& flat = P.vox_is_f32 ? b->flat_f32 : b->flat_u16;
+        P.block_start = flat;
+        flat += nrg * P.g.nAng * P.quads_per_xcd * 8;
+        if (!P.vox_is_f32) ++b->n_u16;
+        const int items = nrg * P.g.nAng * P.cols_per_xcd;
+        for (int q = 0; q < 8; ++q) {
+            P.xcd_start[q] = b->totals[q];
+            b->totals[q] += items;
+        }
+    }
     if (np > (size_t)b->cap) {
         (void)hipFree(b->d_probs);
         b->d_probs = nullptr;

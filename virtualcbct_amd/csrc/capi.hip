@@ -95,6 +95,7 @@ extern "C" int vcbct_projector_create(vcbct_projector** out, const vcbct_geometr
     G.pitch = g->det_pitch_cm;
     G.sad = g->sad_cm;
     G.pixnum = g->det_pixel_num;
+    derive_geom(G);
     std::vector<double> tab(4 * (size_t)G.nAng);
     memcpy(tab.data(), cos_t, G.nAng * sizeof(double));
     memcpy(tab.data() + G.nAng, sin_t, G.nAng * sizeof(double));
@@ -277,8 +278,9 @@ static int run_adjoint(vcbct_projector* p, const float* tg, float* dev_grad, dou
     const size_t nv = vox_count(p->g);
     if (!p->d_grad_vox && dev_malloc((void**)&p->d_grad_vox, nv * 32) != hipSuccess)
         return fail(VCBCT_ENOMEM, "device allocation of the gradient volume failed");
-    // zero with a fill kernel, not hipMemsetAsync, so the call stays safe under stream capture
-    rc = launch_zero(p->d_grad_vox, nv * 32, st);
+    // scatter target [mat][y][x][z] (FP32 planes); zero with a fill kernel, not hipMemsetAsync, so the call
+    // stays safe under stream capture
+    rc = launch_zero(p->d_grad_vox, nv * 4 * p->g.nmat, st);
     if (rc) return rc;
     if (dev_loss && (rc = launch_zero(dev_loss, sizeof(double), st))) return rc;
     rc = launch_adjoint(p->g, p->d_tables, p->d_vox_f32, spec_of(p), xfactor, tg, mode, p->d_grad_vox, dev_loss, st);

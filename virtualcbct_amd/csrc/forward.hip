@@ -81,30 +81,26 @@ __global__ __launch_bounds__(256) void k_transpose_f32(const float* __restrict__
 }
 
 // [y][x][z][8] float -> [mat][z][y][x]
-__global__ __launch_bounds__(256) void k_untranspose_f32(const float4* __restrict__ src, float* __restrict__ dst, int N,
+// Gradient maps from the scatter layout [mat][y][x][z] back to the caller's [mat][z][y][x] (LDS-tiled
+// transpose of each (y, material) plane: z-runs in, x-runs out).
+__global__ __launch_bounds__(256) void k_untranspose_f32(const float* __restrict__ src, float* __restrict__ dst, int N,
                                                          int Nz, int nmat) {
-    __shared__ float tile[VCBCT_MAX_MAT][TT][TT + 1];
-    const int x0 = blockIdx.x * TT, z0 = blockIdx.y * TT, y = blockIdx.z;
+    __shared__ float tile[TT][TT + 1];
+    const int x0 = blockIdx.x * TT, z0 = blockIdx.y * TT;
+    const int y = blockIdx.z % N, m = blockIdx.z / N;
     const size_t n2 = (size_t)N * N, n3 = n2 * Nz;
+    const float* s = src + (size_t)m * n3 + (size_t)y * N * Nz;
     for (int i = threadIdx.x; i < TT * TT; i += 256) {
         int xx = i / TT, zz = i % TT;
         int x = x0 + xx, z = z0 + zz;
-        float w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (x < N && z < Nz) {
-            size_t o = (((size_t)y * N + x) * Nz + z) * 2;
-            float4 a = src[o], b = src[o + 1];
-            w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w; w[4] = b.x; w[5] = b.y;
-        }
-#pragma unroll
-        for (int m = 0; m < VCBCT_MAX_MAT; ++m) tile[m][zz][xx] = w[m];
+        tile[zz][xx] = (x < N && z < Nz) ? s[(size_t)x * Nz + z] : 0.f;
     }
     __syncthreads();
     const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
-    for (int m = 0; m < nmat; ++m)
-        for (int zz = ty; zz < TT; zz += 8) {
-            int z = z0 + zz, x = x0 + tx;
-            if (z < Nz && x < N) dst[(size_t)m * n3 + (size_t)z * n2 + (size_t)y * N + x] = tile[m][zz][tx];
-        }
+    for (int zz = ty; zz < TT; zz += 8) {
+        int z = z0 + zz, x = x0 + tx;
+        if (z < Nz && x < N) dst[(size_t)m * n3 + (size_t)z * n2 + (size_t)y * N + x] = tile[zz][tx];
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -217,13 +213,11 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
     const int j = rg * 64 + lane;
     const bool active = j < g.H;
 
-    RayCtx r;
-    ray_setup(g, tab[a], tab[g.nAng + a], tab[2 * g.nAng + a], tab[3 * g.nAng + a], active ? j : 0, jj, r);
-
+    const double c = tab[a], sn = tab[g.nAng + a], xc = tab[2 * g.nAng + a], yc = tab[3 * g.nAng + a];
     const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - j) * g.W + jj;
     if (MODE == VCBCT_MODE_SEGCOUNT) {
         CountSeg cnt;
-        traverse(g, r, active, cnt);
+        traverse_wave(g, c, sn, xc, yc, j, jj, active, cnt);
         if (active) aux[o] = (float)cnt.n;
         return;
     }
@@ -232,7 +226,7 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
     acc.Nz = g.Nz;
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0f;
-    traverse(g, r, active, acc);
+    const double leng = traverse_wave(g, c, sn, xc, yc, j, jj, active, acc);
     if (!active) return;
 
     if (MODE == VCBCT_MODE_LINEINT) {
@@ -247,14 +241,14 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
         // (P/reprojectionOnGPU.cu:432, 1038-1044)
         float s = 0.0f;
         if (zero) {
-            for (int e = 0; e < NE; ++e) s += (float)(pdf[e] * 1.0 * r.leng);
+            for (int e = 0; e < NE; ++e) s += (float)(pdf[e] * 1.0 * leng);
         } else {
             for (int e = 0; e < NE; ++e) {
                 double tr = 0.0;
 #pragma unroll
                 for (int m = 0; m < VCBCT_MAX_MAT; ++m)
                     if (m < g.nmat) tr += mu[m * NE + e] * acc.L[m];
-                s += (float)(pdf[e] * exp(-tr * inv_xf) * r.leng);
+                s += (float)(pdf[e] * exp(-tr * inv_xf) * leng);
             }
         }
         out[o] = s;
@@ -329,8 +323,10 @@ __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __res
     }
 }
 
-// Flat batched grid: the blocks of every problem laid end to end (each problem's block count is a multiple
-// of 8, so block b still runs on XCD b % 8 and the per-problem XCD-aware tile mapping carries over).
+// Flat batched grid: the blocks of every problem of one voxel type laid end to end (each problem's block
+// count is a multiple of 8, so block b still runs on XCD b % 8 and the per-problem XCD-aware tile mapping
+// carries over).
+template <class VoxT>
 __global__ __launch_bounds__(256) void k_forward_flat(const BatchProblem* __restrict__ pr, int np) {
     const int b = blockIdx.x;
     int lo = 0, hi = np - 1;
@@ -347,17 +343,16 @@ __global__ __launch_bounds__(256) void k_forward_flat(const BatchProblem* __rest
     const int a = ra % P.g.nAng, rg = ra / P.g.nAng;
     const int jj = ((xcd * P.quads_per_xcd + q) << 2) + (threadIdx.x >> 6);
     if (jj >= P.g.W) return;
-    if (P.vox_is_f32)
-        forward_wave<VCBCT_MODE_INTENSITY, float4>(P.g, P.tab, (const float4*)P.vox, P.mu, P.pdf, P.NE, P.inv_xf, a,
-                                                   rg, jj, P.out, nullptr);
-    else
-        forward_wave<VCBCT_MODE_INTENSITY, uint4>(P.g, P.tab, (const uint4*)P.vox, P.mu, P.pdf, P.NE, P.inv_xf, a, rg,
-                                                  jj, P.out, nullptr);
+    forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.NE, P.inv_xf, a, rg, jj,
+                                            P.out, nullptr);
 }
 
-int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks, hipStream_t st) {
-    if (total_blocks <= 0) return 0;
-    hipLaunchKernelGGL(k_forward_flat, dim3(total_blocks), dim3(256), 0, st, d_problems, np);
+int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks, int vox_is_f32, hipStream_t st) {
+    if (np <= 0 || total_blocks <= 0) return 0;
+    if (vox_is_f32)
+        hipLaunchKernelGGL(k_forward_flat<float4>, dim3(total_blocks), dim3(256), 0, st, d_problems, np);
+    else
+        hipLaunchKernelGGL(k_forward_flat<uint4>, dim3(total_blocks), dim3(256), 0, st, d_problems, np);
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
@@ -376,18 +371,21 @@ int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xc
 //   c_m = w_ray * mubar_m / xf,  mubar_m = sum_e pdf_e e_e mu_me / sum_e pdf_e e_e,
 // with w_ray = 2 (p - target) (loss mode) or the upstream gradient dL/dp (upstream mode), then scatter
 // c_m * rr into the gradient volume along the same traversal (second traversal, FP32 atomics into the
-// [y][x][z][8] layout, so the six materials of a voxel share one 32-byte sector).
+// planar [mat][y][x][z] layout, transposed to the caller's [mat][z][y][x] afterwards).
 struct ScatterG {
     float* __restrict__ gvox;
     int N, Nz, nmat;
     float c[VCBCT_MAX_MAT];
+    size_t plane;  // N * N * Nz: one material plane of the [mat][y][x][z] scatter layout
     __device__ __forceinline__ void seg(const Seg& s) {
         if (!s.ok) return;
-        float* p = gvox + (size_t)s.cell * 8;
+        // planar layout: the lanes of a wave (consecutive rows, neighbouring z) hit a contiguous run of one
+        // material plane, so each atomic instruction touches few cache lines
+        float* p = gvox + (size_t)s.cell;
         float rf = (float)s.rr;
 #pragma unroll
         for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-            if (m < nmat && c[m] != 0.f) atomicAdd(p + m, c[m] * rf);
+            if (m < nmat && c[m] != 0.f) atomicAdd(p + m * plane, c[m] * rf);
     }
     __device__ __forceinline__ void step(const Seg& a, const Seg& b, const Seg& c2) {
         seg(a);
@@ -408,20 +406,20 @@ __global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __res
     const int a = t.a;
     if (jj >= g.W) return;
     const bool active = j < g.H;
-    RayCtx r;
-    ray_setup(g, tab[a], tab[g.nAng + a], tab[2 * g.nAng + a], tab[3 * g.nAng + a], active ? j : 0, jj, r);
+    const double c = tab[a], sn = tab[g.nAng + a], xc = tab[2 * g.nAng + a], yc = tab[3 * g.nAng + a];
     AccumL<float4> acc;
     acc.vox = vox;
     acc.Nz = g.Nz;
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0f;
-    traverse(g, r, active, acc);
+    traverse_wave(g, c, sn, xc, yc, j, jj, active, acc);
 
     ScatterG sc;
     sc.gvox = gvox;
     sc.N = g.N;
     sc.Nz = g.Nz;
     sc.nmat = g.nmat;
+    sc.plane = (size_t)g.N * g.N * g.Nz;
     double lossc = 0.0;
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) sc.c[m] = 0.f;
@@ -453,7 +451,7 @@ __global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __res
         for (int m = 0; m < VCBCT_MAX_MAT; ++m)
             if (m < g.nmat) sc.c[m] = (float)(w * (mub[m] / se) * inv_xf);
     }
-    traverse(g, r, active, sc);
+    traverse_wave(g, c, sn, xc, yc, j, jj, active, sc);
     if (MODE == VCBCT_ADJ_LOSS && loss) {
         // wave reduction of the loss, one atomic per wave
         double v = lossc;
@@ -501,8 +499,8 @@ int launch_transpose_f32(const float* src, float* dst, const ProjGeom& g, hipStr
     return 0;
 }
 int launch_untranspose_f32(const float* src, float* dst, const ProjGeom& g, hipStream_t st) {
-    dim3 grid((g.N + TT - 1) / TT, (g.Nz + TT - 1) / TT, g.N);
-    hipLaunchKernelGGL(k_untranspose_f32, grid, dim3(256), 0, st, (const float4*)src, dst, g.N, g.Nz, g.nmat);
+    dim3 grid((g.N + TT - 1) / TT, (g.Nz + TT - 1) / TT, g.N * g.nmat);
+    hipLaunchKernelGGL(k_untranspose_f32, grid, dim3(256), 0, st, src, dst, g.N, g.Nz, g.nmat);
     VCBCT_HIP(hipGetLastError());
     return 0;
 }

@@ -38,6 +38,7 @@ struct RayCtx {
     double K;         // segment length per |du|
     double dirv, dirz;
     bool vmove;       // pv != sv
+    bool swap;        // entry point is the second box hit (the only lane-dependent choice of the xy path)
 };
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -92,8 +93,9 @@ __host__ __device__ __forceinline__ void ray_setup(const ProjGeom& g, double c, 
     r.hit = m >= 2;
     if (!r.hit) { xx[0] = xx[1] = yy[0] = yy[1] = zz[0] = zz[1] = 0.0; }
     double xi, yi, zi, xf, yf, zf;
-    if ((xx[0] - sx) * (xx[0] - sx) + (yy[0] - sy) * (yy[0] - sy) + (zz[0] - sz) * (zz[0] - sz) <
-        (xx[1] - sx) * (xx[1] - sx) + (yy[1] - sy) * (yy[1] - sy) + (zz[1] - sz) * (zz[1] - sz)) {
+    r.swap = !((xx[0] - sx) * (xx[0] - sx) + (yy[0] - sy) * (yy[0] - sy) + (zz[0] - sz) * (zz[0] - sz) <
+               (xx[1] - sx) * (xx[1] - sx) + (yy[1] - sy) * (yy[1] - sy) + (zz[1] - sz) * (zz[1] - sz));
+    if (!r.swap) {
         xi = xx[0]; yi = yy[0]; zi = zz[0]; xf = xx[1]; yf = yy[1]; zf = zz[1];
     } else {
         xi = xx[1]; yi = yy[1]; zi = zz[1]; xf = xx[0]; yf = yy[0]; zf = zz[0];
@@ -189,14 +191,12 @@ __host__ __device__ __forceinline__ int minor_term(int imin, int mstride) {
 
 __host__ __device__ __forceinline__ ChargeC charge_consts(const ProjGeom& g, const RayCtx& r) {
     ChargeC c;
-    const double inv_s = 1.0 / g.s;
-    c.smaj = r.xmaj ? inv_s : -inv_s;
-    c.smin = r.xmaj ? -inv_s : inv_s;
-    c.nmaj = (double)(int)(g.N * 0.5);
-    c.ninv_t = -1.0 / g.t;
-    c.nzm = (double)(int)(g.Nz * 0.5);
-    const double rmax = g.N * 0.5 * g.s;
-    c.rmax2 = rmax * rmax;
+    c.smaj = r.xmaj ? g.inv_s : -g.inv_s;
+    c.smin = r.xmaj ? -g.inv_s : g.inv_s;
+    c.nmaj = g.nmaj;
+    c.ninv_t = g.ninv_t;
+    c.nzm = g.nzm;
+    c.rmax2 = g.rmax2;
     c.mstride = r.xmaj ? g.N * g.Nz : g.Nz;
     c.majmul = r.xmaj ? g.Nz : g.N * g.Nz;
     return c;
@@ -267,8 +267,8 @@ template <bool ALIGNED, bool MUL24, class V>
 __host__ __device__ __forceinline__ void traverse_steps(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
                                                         int kmax, V& vis) {
 #pragma clang fp contract(fast)
-    const double lim_u = double((int)(g.N * 0.5)) * g.s;  // nxm == nym
-    const double inv_s = 1.0 / g.s, inv_t = 1.0 / g.t;
+    const double lim_u = g.lim_u;  // nxm == nym
+    const double inv_s = g.inv_s, inv_t = g.inv_t;
     const bool on = active && r.hit;
     const ChargeC c = charge_consts(g, r);
 
@@ -346,25 +346,17 @@ __host__ __device__ __forceinline__ void traverse_steps(const ProjGeom& g, const
     }
 }
 
-// Whether the whole-cm plane grid of the stepping loop coincides with the voxel boundaries (1/s integral).
-__host__ __device__ __forceinline__ bool planes_aligned(const ProjGeom& g) {
-    const double q = 1.0 / g.s;
-    return q >= 1.0 && fabs(q - rint(q)) < 1e-9;
-}
-
 // Walks the compat traversal of one ray (lanes of a wave in lockstep over plane steps).
 // `active`: lane holds a real detector pixel.  The visitor accumulates.
 template <class V>
 __host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
                                                        int kmax, V& vis) {
     if (kmin > kmax) return;  // no lane of this wave can reach the volume's z band
-    // minor-axis strides up to N * Nz multiply in 24 bits when they fit
-    const bool mul24 = (long long)g.N * g.Nz < (1LL << 24);
-    if (planes_aligned(g)) {
-        if (mul24) traverse_steps<true, true>(g, r, active, kmin, kmax, vis);
+    if (g.aligned) {
+        if (g.mul24) traverse_steps<true, true>(g, r, active, kmin, kmax, vis);
         else traverse_steps<true, false>(g, r, active, kmin, kmax, vis);
     } else {
-        if (mul24) traverse_steps<false, true>(g, r, active, kmin, kmax, vis);
+        if (g.mul24) traverse_steps<false, true>(g, r, active, kmin, kmax, vis);
         else traverse_steps<false, false>(g, r, active, kmin, kmax, vis);
     }
 }
@@ -372,12 +364,41 @@ __host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const 
 }  // namespace vcbct
 
 namespace vcbct {
+// Walks the rays of one wave: lane = detector row j (inactive lanes, j >= H, walk row 0 and charge
+// nothing), all lanes one detector column jj at the angle whose trig row is (c, sn, xc, yc).  The plane
+// steps walked are the union of the lanes' z ranges (wave-wide min/max).  Returns the lane's cos^3 factor.
+__device__ __forceinline__ double first_lane(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The fields of a ray's xy path that do not depend on which box hit is the entry are the same, bit for
+// bit, on every lane of the wave (the lanes differ only in z): the major axis (|xi - xf| > |yi - yf| is
+// symmetric in the two hits), the source position in (u, v), the slopes dv/du and du/dv, and the
+// hit/minor-move flags.  Reading them from the first lane puts them in scalar registers.
+__device__ __forceinline__ void share_xy_path(RayCtx& r) {
+    r.xmaj = __builtin_amdgcn_readfirstlane((int)r.xmaj) != 0;
+    r.hit = __builtin_amdgcn_readfirstlane((int)r.hit) != 0;
+    r.vmove = __builtin_amdgcn_readfirstlane((int)r.vmove) != 0;
+    r.su = first_lane(r.su);
+    r.sv = first_lane(r.sv);
+    r.dv = first_lane(r.dv);
+    r.duv = first_lane(r.duv);
+}
+
 template <class V>
-__device__ __forceinline__ void traverse(const ProjGeom& g, const RayCtx& r, bool active, V& vis) {
+__device__ __forceinline__ double traverse_wave(const ProjGeom& g, double c, double sn, double xc, double yc, int j,
+                                                int jj, bool active, V& vis) {
+    RayCtx r;
+    ray_setup(g, c, sn, xc, yc, active ? j : 0, jj, r);
+    share_xy_path(r);
     int klo, khi;
     z_step_range(g, r, active, klo, khi);
     const int kmin = wave_min(klo);
     const int kmax = wave_max(khi);
     traverse_lane(g, r, active, kmin, kmax, vis);
+    return r.leng;
 }
 }  // namespace vcbct

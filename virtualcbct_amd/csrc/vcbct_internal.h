@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <string>
 
+#include <cmath>
+
 #include "vcbct.h"
 
 namespace vcbct {
@@ -28,7 +30,30 @@ struct ProjGeom {
     double s, t;            // voxel pitch x/y and z [cm]
     double x_reg, y_reg, z_reg;
     double pitch_iso, pitch, sad, pixnum;
+    // derived on the host by derive_geom() (kernel arguments, so they sit in scalar registers)
+    double inv_s, inv_t;    // 1/s, 1/t
+    double ninv_t;          // -1/t
+    double nmaj, nzm;       // (int)(N/2), (int)(Nz/2) as doubles: index offsets of the midpoint rule
+    double lim_u;           // box half-width (int)(N/2) * s
+    double rmax2;           // squared radius of the inscribed-circle mask, (N/2 * s)^2
+    int aligned;            // 1/s integral: the whole-cm plane grid lies on voxel boundaries
+    int mul24;              // N * Nz < 2^24: voxel strides multiply in 24 bits
 };
+
+// Fills the derived fields of a ProjGeom whose primary fields are set.
+inline void derive_geom(ProjGeom& g) {
+    g.inv_s = 1.0 / g.s;
+    g.inv_t = 1.0 / g.t;
+    g.ninv_t = -1.0 / g.t;
+    g.nmaj = (double)(int)(g.N * 0.5);
+    g.nzm = (double)(int)(g.Nz * 0.5);
+    g.lim_u = double((int)(g.N * 0.5)) * g.s;
+    const double rmax = g.N * 0.5 * g.s;
+    g.rmax2 = rmax * rmax;
+    const double q = 1.0 / g.s;
+    g.aligned = (q >= 1.0 && std::fabs(q - std::rint(q)) < 1e-9) ? 1 : 0;
+    g.mul24 = ((long long)g.N * g.Nz < (1LL << 24)) ? 1 : 0;
+}
 
 // Per-call spectrum view (device pointers to FP64 copies of the reference's float tables).
 struct Spectrum {
@@ -52,11 +77,11 @@ struct BatchProblem {
     int cols_per_xcd;   // ceil(W / 8)
     int xcd_start[8];   // first work item of this problem in each XCD queue (persistent form)
     int quads_per_xcd;  // flat form: ceil(ceil(W / 4) / 8) four-column blocks per XCD per (row group, angle)
-    int block_start;    // flat form: first block of this problem
+    int block_start;    // flat form: first block of this problem in its voxel type's grid
 };
 int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xcd_total, int* d_counters, int nblocks,
                          hipStream_t st);
-int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks, hipStream_t st);
+int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks, int vox_is_f32, hipStream_t st);
 
 // Kernels launchers (forward.hip)
 int launch_transpose_u16(const uint16_t* src, uint16_t* dst, const ProjGeom& g, hipStream_t st);
