@@ -91,6 +91,106 @@ def cpu_baseline(inp, seconds_target=15.0):
             "sample": f"1 angle x 160000 rays x 91 bins with the FP64 oracle, {sec:.1f} s"}
 
 
+def bench_adjoint(line, inp, dev, stream, e):
+    import torch
+    from virtualcbct_amd.projector import Projector
+    # adjoint on the differentiable workload (256^2 x 64 FP32 maps, 90 angles, 256^2 detector)
+    from virtualcbct_amd import phantoms, io
+    N4, Z4, n4, D4 = 256, 64, 90, 256
+    ang4 = np.deg2rad(-180.0 + 4.0 * np.arange(n4) + 1e-4)
+    c4, s4, x4, y4 = io.trig_table(D4, D4, ang4, np.zeros(n4), np.zeros(n4))
+    P4 = Projector(N4, 0.1, Z4, 0.1, D4, D4, n4, c4, s4, x4, y4)
+    maps = torch.from_numpy(phantoms.water_cylinder_f32(N4, Z4, 0.1, 10.0)).to(dev).contiguous()
+    P4.set_volume(maps)
+    P4.set_spectrum(inp["atten"], inp["pdf"])
+    tgt = P4.logproj(1.0) * 1.05
+    P4.adjoint(tgt, 1.0)
+    torch.cuda.synchronize()
+    ta = []
+    for _ in range(5):
+        e[0].record(stream)
+        grad, loss = P4.adjoint(tgt, 1.0)
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        ta.append(e[0].elapsed_time(e[1]))
+    segs4 = float(P4.segment_counts().double().sum().item())
+    rays4 = n4 * D4 * D4
+    adj_bytes = 48.0 * segs4 + 8.0 * rays4
+    line["adjoint"] = {"grads_per_s": 1e3 / float(np.median(ta)), "ms": float(np.median(ta)),
+                       "workload": "256^2 x 64 FP32 maps (1 mm), 90 angles, 256^2 detector, 91 bins",
+                       "roofline_frac": adj_bytes / (np.median(ta) * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    log(f"adjoint: {np.median(ta):.3f} ms/gradient")
+    P4.close()
+
+
+def bench_batched(line, inp, dev, stream, e, NE):
+    import torch
+    from virtualcbct_amd.projector import Projector
+    # batched: eight phantoms of uneven length (64 .. 256 slices), 360 angles, 91 bins — one batched launch
+    # (grid form chosen by the autotuner) and its hipGraph replay against the plain per-phantom launches
+    from virtualcbct_amd import phantoms, io
+    from virtualcbct_amd.batch import Batch, autotune_log
+    zs = (64, 96, 128, 160, 192, 224, 240, 256)
+    ang3, xs3, ys3 = phantoms.flexmap_angles(360)
+    c3, s3, x3, y3 = io.trig_table(400, 400, ang3, xs3, ys3)
+    projs, outs3 = [], []
+    for i, Z in enumerate(zs):
+        volt = phantoms.ellipsoid_phantom_torch(512, Z, 0.05, 0.1, 3000 + i, device=dev)
+        Pi = Projector(512, 0.05, Z, 0.1, 400, 400, 360, c3, s3, x3, y3)
+        Pi.set_volume(volt)
+        Pi.set_spectrum(inp["atten"], inp["pdf"])
+        del volt
+        projs.append(Pi)
+        outs3.append(torch.empty(Pi.shape_proj, dtype=torch.float32, device=dev))
+    torch.cuda.synchronize()
+    for Pi, o in zip(projs, outs3):
+        Pi.forward(1000.0, out=o)
+    torch.cuda.synchronize()
+    plain_ms = []
+    for _ in range(2):
+        e[0].record(stream)
+        for Pi, o in zip(projs, outs3):
+            Pi.forward(1000.0, out=o)
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        plain_ms.append(e[0].elapsed_time(e[1]))
+    ref0 = outs3[3].clone()
+    B = Batch()
+    for Pi, o in zip(projs, outs3):
+        B.add_forward(Pi, o, 1000.0)
+    B.run()  # finalises and autotunes the grid
+    torch.cuda.synchronize()
+    batch_ms = []
+    for _ in range(2):
+        e[0].record(stream)
+        B.run()
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        batch_ms.append(e[0].elapsed_time(e[1]))
+    same = bool(torch.equal(outs3[3], ref0))
+    B.capture()
+    graph_ms = []
+    for _ in range(2):
+        e[0].record(stream)
+        B.replay()
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        graph_ms.append(e[0].elapsed_time(e[1]))
+    rays3 = 360 * 400 * 400 * len(zs)
+    line["batched"] = {"workload": "8 ellipsoid phantoms 512^2 x {64..256} slices, 360 angles, 400^2, 91 bins",
+                       "plain_ms": float(min(plain_ms)), "batched_ms": float(min(batch_ms)),
+                       "graph_ms": float(min(graph_ms)),
+                       "plain_rays_bins_per_s": rays3 * NE / (min(plain_ms) * 1e-3),
+                       "batched_rays_bins_per_s": rays3 * NE / (min(batch_ms) * 1e-3),
+                       "speedup": float(min(plain_ms) / min(batch_ms)), "identical_to_plain": same,
+                       "batch_grid": ("flat" if B.blocks == 0 else f"persistent {B.blocks} blocks"), "autotune": autotune_log().strip().split("\n")}
+    log(f"batched: plain {min(plain_ms):.2f} ms, batched {min(batch_ms):.2f} ms, graph {min(graph_ms):.2f} ms")
+    B.close()
+    for Pi in projs:
+        Pi.close()
+    del outs3
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +198,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline forward only (used under the profiler)")
+    ap.add_argument("--only", choices=["adjoint", "batched"], default=None,
+                    help="run one secondary section alone and print its JSON (used under the profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (scale for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -132,6 +234,16 @@ def main():
     out = torch.empty(P.shape_proj, dtype=torch.float32, device=dev)
     NE = len(inp["pdf"])
     stream = torch.cuda.current_stream()
+
+    if args.only:
+        line = {"section": args.only}
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        if args.only == "adjoint":
+            bench_adjoint(line, inp, dev, stream, e)
+        else:
+            bench_batched(line, inp, dev, stream, e, NE)
+        print(json.dumps(line), flush=True)
+        return
 
     log(f"forward warmup x{args.warmup} ...")
     for _ in range(args.warmup):
@@ -203,98 +315,10 @@ def main():
         log(f"fdk: filter {np.median(tf):.3f} ms, backprojection {np.median(tb):.3f} ms, "
             f"{line['fdk']['gups']:.1f} GUPS")
         del filt, rec
-        # adjoint on the differentiable workload (256^2 x 64 FP32 maps, 90 angles, 256^2 detector)
-        from virtualcbct_amd import phantoms, io
-        N4, Z4, n4, D4 = 256, 64, 90, 256
-        ang4 = np.deg2rad(-180.0 + 4.0 * np.arange(n4) + 1e-4)
-        c4, s4, x4, y4 = io.trig_table(D4, D4, ang4, np.zeros(n4), np.zeros(n4))
-        P4 = Projector(N4, 0.1, Z4, 0.1, D4, D4, n4, c4, s4, x4, y4)
-        maps = torch.from_numpy(phantoms.water_cylinder_f32(N4, Z4, 0.1, 10.0)).to(dev).contiguous()
-        P4.set_volume(maps)
-        P4.set_spectrum(inp["atten"], inp["pdf"])
-        tgt = P4.logproj(1.0) * 1.05
-        P4.adjoint(tgt, 1.0)
-        torch.cuda.synchronize()
-        ta = []
-        for _ in range(5):
-            e[0].record(stream)
-            grad, loss = P4.adjoint(tgt, 1.0)
-            e[1].record(stream)
-            torch.cuda.synchronize()
-            ta.append(e[0].elapsed_time(e[1]))
-        segs4 = float(P4.segment_counts().double().sum().item())
-        rays4 = n4 * D4 * D4
-        adj_bytes = 48.0 * segs4 + 8.0 * rays4
-        line["adjoint"] = {"grads_per_s": 1e3 / float(np.median(ta)), "ms": float(np.median(ta)),
-                           "workload": "256^2 x 64 FP32 maps (1 mm), 90 angles, 256^2 detector, 91 bins",
-                           "roofline_frac": adj_bytes / (np.median(ta) * 1e-3) / 1e9 / HBM_PEAK_GBS}
-        log(f"adjoint: {np.median(ta):.3f} ms/gradient")
-        P4.close()
+        bench_adjoint(line, inp, dev, stream, e)
 
     if not args.no_extras:
-        # batched: eight phantoms of uneven length (64 .. 256 slices), 360 angles, 91 bins — one persistent
-        # launch (and its hipGraph replay) against the plain per-phantom launches
-        from virtualcbct_amd import phantoms, io
-        from virtualcbct_amd.batch import Batch, autotune_log
-        zs = (64, 96, 128, 160, 192, 224, 240, 256)
-        ang3, xs3, ys3 = phantoms.flexmap_angles(360)
-        c3, s3, x3, y3 = io.trig_table(400, 400, ang3, xs3, ys3)
-        projs, outs3 = [], []
-        for i, Z in enumerate(zs):
-            volt = phantoms.ellipsoid_phantom_torch(512, Z, 0.05, 0.1, 3000 + i, device=dev)
-            Pi = Projector(512, 0.05, Z, 0.1, 400, 400, 360, c3, s3, x3, y3)
-            Pi.set_volume(volt)
-            Pi.set_spectrum(inp["atten"], inp["pdf"])
-            del volt
-            projs.append(Pi)
-            outs3.append(torch.empty(Pi.shape_proj, dtype=torch.float32, device=dev))
-        torch.cuda.synchronize()
-        for Pi, o in zip(projs, outs3):
-            Pi.forward(1000.0, out=o)
-        torch.cuda.synchronize()
-        plain_ms = []
-        for _ in range(2):
-            e[0].record(stream)
-            for Pi, o in zip(projs, outs3):
-                Pi.forward(1000.0, out=o)
-            e[1].record(stream)
-            torch.cuda.synchronize()
-            plain_ms.append(e[0].elapsed_time(e[1]))
-        ref0 = outs3[3].clone()
-        B = Batch()
-        for Pi, o in zip(projs, outs3):
-            B.add_forward(Pi, o, 1000.0)
-        B.run()  # finalises and autotunes the persistent grid
-        torch.cuda.synchronize()
-        batch_ms = []
-        for _ in range(2):
-            e[0].record(stream)
-            B.run()
-            e[1].record(stream)
-            torch.cuda.synchronize()
-            batch_ms.append(e[0].elapsed_time(e[1]))
-        same = bool(torch.equal(outs3[3], ref0))
-        B.capture()
-        graph_ms = []
-        for _ in range(2):
-            e[0].record(stream)
-            B.replay()
-            e[1].record(stream)
-            torch.cuda.synchronize()
-            graph_ms.append(e[0].elapsed_time(e[1]))
-        rays3 = 360 * 400 * 400 * len(zs)
-        line["batched"] = {"workload": "8 ellipsoid phantoms 512^2 x {64..256} slices, 360 angles, 400^2, 91 bins",
-                           "plain_ms": float(min(plain_ms)), "batched_ms": float(min(batch_ms)),
-                           "graph_ms": float(min(graph_ms)),
-                           "plain_rays_bins_per_s": rays3 * NE / (min(plain_ms) * 1e-3),
-                           "batched_rays_bins_per_s": rays3 * NE / (min(batch_ms) * 1e-3),
-                           "speedup": float(min(plain_ms) / min(batch_ms)), "identical_to_plain": same,
-                           "batch_grid": ("flat" if B.blocks == 0 else f"persistent {B.blocks} blocks"), "autotune": autotune_log().strip().split("\n")}
-        log(f"batched: plain {min(plain_ms):.2f} ms, batched {min(batch_ms):.2f} ms, graph {min(graph_ms):.2f} ms")
-        B.close()
-        for Pi in projs:
-            Pi.close()
-        del outs3
+        bench_batched(line, inp, dev, stream, e, NE)
 
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

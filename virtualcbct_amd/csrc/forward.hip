@@ -129,7 +129,7 @@ struct AccumL<uint4> {
         if (s2.ok) add(q2, (float)s2.rr);
     }
     __device__ __forceinline__ uint3 ld(const Seg& s) const {
-        const uint4 q = vox[s.cell];
+        const uint4 q = ldg(vox, (size_t)s.cell);
         return make_uint3(q.x, q.y, q.z);
     }
     __device__ __forceinline__ void add(const uint3& q, float rr) {
@@ -163,8 +163,8 @@ struct AccumL<float4> {
     }
     __device__ __forceinline__ Q ld(const Seg& s) const {
         const size_t i = 2 * (size_t)s.cell;
-        const float4 b = vox[i + 1];
-        return Q{vox[i], make_float2(b.x, b.y)};
+        const float4 b = ldg(vox, i + 1);
+        return Q{ldg(vox, i), make_float2(b.x, b.y)};
     }
     __device__ __forceinline__ void add(const Q& q, float rr) {
 #pragma clang fp contract(fast)
@@ -213,12 +213,13 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
     const int j = rg * 64 + lane;
     const bool active = j < g.H;
 
-    const double c = tab[a], sn = tab[g.nAng + a], xc = tab[2 * g.nAng + a], yc = tab[3 * g.nAng + a];
+    const double c = ldg(tab, a), sn = ldg(tab, g.nAng + a), xc = ldg(tab, 2 * g.nAng + a),
+                 yc = ldg(tab, 3 * g.nAng + a);
     const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - j) * g.W + jj;
     if (MODE == VCBCT_MODE_SEGCOUNT) {
         CountSeg cnt;
         traverse_wave(g, c, sn, xc, yc, j, jj, active, cnt);
-        if (active) aux[o] = (float)cnt.n;
+        if (active) stg(aux, o, (float)cnt.n);
         return;
     }
     AccumL<VoxT> acc;
@@ -230,7 +231,7 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
     if (!active) return;
 
     if (MODE == VCBCT_MODE_LINEINT) {
-        for (int m = 0; m < g.nmat; ++m) aux[o * g.nmat + m] = (float)acc.L[m];
+        for (int m = 0; m < g.nmat; ++m) stg(aux, o * g.nmat + m, (float)acc.L[m]);
         return;
     }
     bool zero = true;
@@ -241,28 +242,29 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
         // (P/reprojectionOnGPU.cu:432, 1038-1044)
         float s = 0.0f;
         if (zero) {
-            for (int e = 0; e < NE; ++e) s += (float)(pdf[e] * 1.0 * leng);
+            for (int e = 0; e < NE; ++e) s += (float)(ldg(pdf, e) * 1.0 * leng);
         } else {
             for (int e = 0; e < NE; ++e) {
                 double tr = 0.0;
 #pragma unroll
                 for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                    if (m < g.nmat) tr += mu[m * NE + e] * acc.L[m];
-                s += (float)(pdf[e] * exp(-tr * inv_xf) * leng);
+                    if (m < g.nmat) tr += ldg(mu, m * NE + e) * acc.L[m];
+                s += (float)(ldg(pdf, e) * exp(-tr * inv_xf) * leng);
             }
         }
-        out[o] = s;
+        stg(out, o, s);
     } else {  // VCBCT_MODE_LOGPROJ: p = -log(sum_e pdf_e e^{-mu.L/xf} / sum_e pdf_e), FP64
         double sw = 0.0, se = 0.0;
         for (int e = 0; e < NE; ++e) {
             double tr = 0.0;
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                if (m < g.nmat) tr += mu[m * NE + e] * acc.L[m];
-            sw += pdf[e];
-            se += pdf[e] * exp(-tr * inv_xf);
+                if (m < g.nmat) tr += ldg(mu, m * NE + e) * acc.L[m];
+            const double pe = ldg(pdf, e);
+            sw += pe;
+            se += pe * exp(-tr * inv_xf);
         }
-        out[o] = (float)(-log(se / sw));
+        stg(out, o, (float)(-log(se / sw)));
     }
 }
 
@@ -406,7 +408,8 @@ __global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __res
     const int a = t.a;
     if (jj >= g.W) return;
     const bool active = j < g.H;
-    const double c = tab[a], sn = tab[g.nAng + a], xc = tab[2 * g.nAng + a], yc = tab[3 * g.nAng + a];
+    const double c = ldg(tab, a), sn = ldg(tab, g.nAng + a), xc = ldg(tab, 2 * g.nAng + a),
+                 yc = ldg(tab, 3 * g.nAng + a);
     AccumL<float4> acc;
     acc.vox = vox;
     acc.Nz = g.Nz;
@@ -429,13 +432,14 @@ __global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __res
             double tr = 0.0;
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                if (m < g.nmat) tr += mu[m * NE + e] * acc.L[m];
-            double we = pdf[e] * exp(-tr * inv_xf);
-            sw += pdf[e];
+                if (m < g.nmat) tr += ldg(mu, m * NE + e) * acc.L[m];
+            const double pe = ldg(pdf, e);
+            double we = pe * exp(-tr * inv_xf);
+            sw += pe;
             se += we;
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                if (m < g.nmat) mub[m] += we * mu[m * NE + e];
+                if (m < g.nmat) mub[m] += we * ldg(mu, m * NE + e);
         }
         const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - j) * g.W + jj;
         double w;

@@ -55,6 +55,25 @@ inline void derive_geom(ProjGeom& g) {
     g.mul24 = ((long long)g.N * g.Nz < (1LL << 24)) ? 1 : 0;
 }
 
+#ifdef __HIPCC__
+// Loads and stores through pointers known to address global memory.  Pointers read from device-resident
+// descriptors (the batched launch) are otherwise generic, and generic accesses compile to FLAT
+// instructions; through a global pointer the compiler emits global (or, for uniform read-only data,
+// scalar) memory instructions.
+template <class T>
+__device__ __forceinline__ T ldg(const T* p, size_t i) {
+    typedef __attribute__((address_space(1))) const T GT;
+    T r;
+    __builtin_memcpy(&r, (GT*)p + i, sizeof(T));  // (HIP vector types have no copy from a qualified object)
+    return r;
+}
+template <class T>
+__device__ __forceinline__ void stg(T* p, size_t i, T v) {
+    typedef __attribute__((address_space(1))) T GT;
+    ((GT*)p)[i] = v;
+}
+#endif
+
 // Per-call spectrum view (device pointers to FP64 copies of the reference's float tables).
 struct Spectrum {
     const double* mu;   // [nmat][NE]
