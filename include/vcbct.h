@@ -185,6 +185,9 @@ int vcbct_batch_launch_graph(vcbct_batch* b, void* stream);
 /* Grid the autotuner chose for the batch's forward problems: the persistent grid size in blocks, or 0 for
  * the flat grid (every problem's blocks laid end to end, no work queue); -1 for NULL. */
 int vcbct_batch_blocks(const vcbct_batch* b);
+/* Fixes the forward grid instead of autotuning it: 0 = flat grid, n > 0 = persistent grid of n blocks
+ * (rounded up to a multiple of 8), -1 = autotune (default).  No reference counterpart (tuning control). */
+int vcbct_batch_set_grid(vcbct_batch* b, int blocks);
 /* Text log of the autotuner's decisions (candidates and timings); returns its length. */
 int vcbct_autotune_log(char* buf, int capacity);
 

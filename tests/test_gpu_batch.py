@@ -113,3 +113,35 @@ def test_fdk_plan_equals_one_shot(torch):
     assert torch.allclose(a, b, rtol=1e-6, atol=1e-4)
     ref = O.reconstruct_image(40, 9, -2, 0.45, 0.4, O.filter_sinogram(proj.cpu().numpy(), xs, ys), 1000, ang, xs)
     assert np.abs(a.cpu().numpy() - ref).max() <= 1e-4 * np.abs(ref).max() + 1e-3
+
+
+def test_every_grid_form_equals_plain_mixed_voxel_types(torch):
+    """u16 volumes and FP32 maps in one batch; the flat grid and persistent grids (including one far smaller
+    than the work, so that waves drain their own XCD queue and steal from the others) all reproduce the
+    plain per-problem launches bit for bit."""
+    from virtualcbct_amd.batch import Batch
+    probs = problems(torch)
+    # a fourth problem with FP32 maps (the differentiable path's voxel type)
+    P0, vol0, atten0, pdf0, g0 = probs[0]
+    from virtualcbct_amd.projector import Projector
+    Pf = Projector(g0.N, g0.s, g0.Nz, g0.t, g0.H, g0.W, g0.n_ang, g0.cos_t, g0.sin_t, g0.xc, g0.yc)
+    maps = torch.from_numpy(np.ascontiguousarray(vol0.transpose(1, 0, 2, 3)).astype(np.float32) / 1000.0).cuda()
+    Pf.set_volume(maps)
+    Pf.set_spectrum(atten0, pdf0)
+    allp = [p[0] for p in probs[:2]] + [Pf] + [probs[2][0]]  # interleaved types: the batch groups them
+    plain = [P.forward(1000.0).clone() for P in allp]
+    for grid in (0, 24, 512, 1536):
+        b = Batch()
+        b.set_grid(grid)
+        outs = []
+        for P in allp:
+            o = torch.full(P.shape_proj, -1.0, device="cuda")
+            b.add_forward(P, o, 1000.0)
+            outs.append(o)
+        b.run()
+        torch.cuda.synchronize()
+        assert b.blocks == grid
+        for i, (o, p) in enumerate(zip(outs, plain)):
+            assert torch.equal(o, p), (grid, i)
+        b.close()
+    Pf.close()

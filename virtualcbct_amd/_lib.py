@@ -91,6 +91,7 @@ SIGNATURES = {
     "vcbct_batch_capture_graph": (c_int, [c_void_p]),
     "vcbct_batch_launch_graph": (c_int, [c_void_p, c_void_p]),
     "vcbct_batch_blocks": (c_int, [c_void_p]),
+    "vcbct_batch_set_grid": (c_int, [c_void_p, c_int]),
     "vcbct_autotune_log": (c_int, [C.c_char_p, c_int]),
     "vcbct_read_proj_config": (c_int, [c_char_p, C.POINTER(vcbct_proj_config)]),
     "vcbct_read_flexmap": (c_int, [c_char_p, P_D, P_D, P_D, c_int]),

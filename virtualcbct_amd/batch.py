@@ -1,5 +1,6 @@
 """Batched launch: many forward problems (phantoms of uneven slice counts, angle counts and energy bins) in one
-persistent kernel launch, optionally followed by per-problem FDK plans, capturable in a hipGraph."""
+kernel launch per voxel type (a flat grid or a persistent grid with per-XCD work queues, chosen by the
+autotuner), optionally followed by per-problem FDK plans, capturable in a hipGraph."""
 from __future__ import annotations
 
 import ctypes as C
@@ -42,7 +43,12 @@ class Batch:
 
     @property
     def blocks(self):
+        """Grid in use: 0 = flat, n > 0 = persistent with n blocks (after the first run)."""
         return lib().vcbct_batch_blocks(self.h)
+
+    def set_grid(self, blocks):
+        """Fix the grid instead of autotuning: 0 = flat, n > 0 = persistent with n blocks, -1 = autotune."""
+        check(lib().vcbct_batch_set_grid(self.h, int(blocks)))
 
     def close(self):
         if getattr(self, "h", None):

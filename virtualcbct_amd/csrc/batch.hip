@@ -1,6 +1,7 @@
 // Batched launch: many forward problems (phantoms of uneven slice counts, angle counts and spectra) in one
-// persistent launch, optionally followed by per-problem FDK, captured in a hipGraph.  The persistent grid
-// size is chosen by timing candidates on the first run (autotuner, cached per batch signature).
+// launch per voxel type, optionally followed by per-problem FDK, captured in a hipGraph.  The grid (flat, or
+// persistent with per-XCD work queues and 2/4/6 blocks per CU) is chosen by timing the candidates on the
+// first run (autotuner, cached per batch signature) unless the caller fixes it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,13 +27,15 @@ using namespace vcbct;
 struct vcbct_batch {
     int device;
     std::vector<BatchProblem> probs;
-    int totals[8];
+    int totals[16];   // persistent queues: items per XCD queue, u16 problems [0, 8), FP32 maps [8, 16)
     BatchProblem* d_probs;
     int* d_total;
     int* d_counters;
     int cap;
     int nblocks;      // persistent grid size, or 0 for the flat grid
+    int forced;       // grid fixed by the caller (-1: autotune)
     int flat_blocks;  // running total of flat-grid blocks (range check while problems are added)
+    long items_total;  // running total of persistent work items (same check)
     int n_u16;        // after finalize: problems [0, n_u16) have u16 volumes, the rest FP32 maps
     int flat_u16, flat_f32;  // flat-grid blocks of each voxel type
     bool dirty;
@@ -67,7 +70,9 @@ extern "C" int vcbct_batch_create(vcbct_batch** out, int device) {
     b->d_counters = nullptr;
     b->cap = 0;
     b->nblocks = 0;
+    b->forced = -1;
     b->flat_blocks = 0;
+    b->items_total = 0;
     b->n_u16 = 0;
     b->flat_u16 = b->flat_f32 = 0;
     b->dirty = true;
@@ -118,10 +123,10 @@ extern "C" int vcbct_batch_add_forward(vcbct_batch* b, vcbct_projector* p, float
     // totals are checked against the 32-bit grid and queue ranges
     const long blocks = (long)((p->g.H + 63) / 64) * p->g.nAng * P.quads_per_xcd * 8;
     const long items = (long)((p->g.H + 63) / 64) * p->g.nAng * P.cols_per_xcd;
-    if ((long)b->flat_blocks + blocks > 0x7fffffffL || (long)b->totals[0] + items > 0x7fffffffL - 1024)
+    if ((long)b->flat_blocks + blocks > 0x7fffffffL || b->items_total + items > 0x7fffffffL - 1024)
         return fail(VCBCT_EINVAL, "batch too large");
     b->flat_blocks += (int)blocks;
-    b->totals[0] += (int)items;
+    b->items_total += items;
     b->probs.push_back(P);
     char sig[128];
     snprintf(sig, sizeof sig, "[N%d Z%d A%d H%d W%d E%d]", p->g.N, p->g.Nz, p->g.nAng, p->g.H, p->g.W, p->NE);
@@ -147,7 +152,12 @@ static int batch_forward(vcbct_batch* b, int nblocks, hipStream_t st) {
         if (!rc) rc = launch_forward_flat(b->d_probs + b->n_u16, np - b->n_u16, b->flat_f32, 1, st);
         return rc;
     }
-    return launch_forward_batch(b->d_probs, (int)b->probs.size(), b->d_total, b->d_counters, nblocks, st);
+    const int np = (int)b->probs.size();
+    int rc = launch_forward_batch(b->d_probs, b->n_u16, b->d_total, b->d_counters, nblocks, 0, st);
+    if (!rc)
+        rc = launch_forward_batch(b->d_probs + b->n_u16, np - b->n_u16, b->d_total + 8, b->d_counters + 8, nblocks,
+                                  1, st);
+    return rc;
 }
 
 static int batch_launch(vcbct_batch* b, int nblocks, hipStream_t st) {
@@ -176,9 +186,10 @@ static int batch_finalize(vcbct_batch* b) {
         flat += nrg * P.g.nAng * P.quads_per_xcd * 8;
         if (!P.vox_is_f32) ++b->n_u16;
         const int items = nrg * P.g.nAng * P.cols_per_xcd;
+        int* tot = b->totals + (P.vox_is_f32 ? 8 : 0);
         for (int q = 0; q < 8; ++q) {
-            P.xcd_start[q] = b->totals[q];
-            b->totals[q] += items;
+            P.xcd_start[q] = tot[q];
+            tot[q] += items;
         }
     }
     if (np > (size_t)b->cap) {
@@ -188,17 +199,17 @@ static int batch_finalize(vcbct_batch* b) {
             return fail(VCBCT_ENOMEM, "batch descriptors");
         b->cap = (int)np;
     }
-    if (!b->d_total && (dev_malloc((void**)&b->d_total, 8 * sizeof(int)) != hipSuccess ||
-                        dev_malloc((void**)&b->d_counters, 8 * sizeof(int)) != hipSuccess))
+    if (!b->d_total && (dev_malloc((void**)&b->d_total, 16 * sizeof(int)) != hipSuccess ||
+                        dev_malloc((void**)&b->d_counters, 16 * sizeof(int)) != hipSuccess))
         return fail(VCBCT_ENOMEM, "batch counters");
     if (np) VCBCT_HIP(hipMemcpy(b->d_probs, b->probs.data(), np * sizeof(BatchProblem), hipMemcpyHostToDevice));
-    VCBCT_HIP(hipMemcpy(b->d_total, b->totals, 8 * sizeof(int), hipMemcpyHostToDevice));
+    VCBCT_HIP(hipMemcpy(b->d_total, b->totals, 16 * sizeof(int), hipMemcpyHostToDevice));
     // candidates: the flat grid (0) and persistent grids of 2, 4, 6 blocks per CU (one queue per XCD)
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
     cus = (cus + 7) / 8 * 8;
-    int chosen = -1;
-    {
+    int chosen = b->forced;
+    if (chosen < 0) {
         std::lock_guard<std::mutex> lk(g_btune_mu);
         auto it = g_batch_blocks.find(b->signature);
         if (it != g_batch_blocks.end()) chosen = it->second;
@@ -282,6 +293,15 @@ extern "C" int vcbct_batch_launch_graph(vcbct_batch* b, void* stream) {
     if (!b->have_graph) return fail(VCBCT_ESTATE, "no captured graph (vcbct_batch_capture_graph)");
     VCBCT_HIP(hipSetDevice(b->device));
     VCBCT_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return 0;
+}
+
+extern "C" int vcbct_batch_set_grid(vcbct_batch* b, int blocks) {
+    if (!b) return fail(VCBCT_EINVAL, "batch is NULL");
+    if (blocks < -1 || blocks > (1 << 20)) return fail(VCBCT_EINVAL, "batch grid %d out of range", blocks);
+    if (b->have_graph) return fail(VCBCT_ESTATE, "batch already captured; create a new batch");
+    b->forced = blocks > 0 ? (blocks + 7) / 8 * 8 : blocks;
+    b->dirty = true;
     return 0;
 }
 

@@ -295,6 +295,7 @@ __device__ __forceinline__ int find_problem(const BatchProblem* __restrict__ pr,
     return lo;
 }
 
+template <class VoxT>
 __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __restrict__ pr, int np,
                                                        const int* __restrict__ xcd_total, int* __restrict__ counters) {
     const int home = blockIdx.x & 7;
@@ -305,7 +306,9 @@ __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __res
         for (;;) {
             int item = 0;
             if (lane == 0) item = atomicAdd(&counters[q], 1);
-            item = __shfl(item, 0, 64);
+            // broadcast from lane 0 into a scalar register: the problem and its parameters are then
+            // wave-uniform (scalar loads), as in the plain kernel
+            item = __builtin_amdgcn_readfirstlane(item);
             if (item >= total) break;
             const int p = find_problem(pr, np, q, item);
             const BatchProblem& P = pr[p];
@@ -315,12 +318,8 @@ __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __res
             const int a = ra % P.g.nAng, rg = ra / P.g.nAng;
             const int jj = q * P.cols_per_xcd + cl;
             if (jj >= P.g.W) continue;
-            if (P.vox_is_f32)
-                forward_wave<VCBCT_MODE_INTENSITY, float4>(P.g, P.tab, (const float4*)P.vox, P.mu, P.pdf, P.NE,
-                                                           P.inv_xf, a, rg, jj, P.out, nullptr);
-            else
-                forward_wave<VCBCT_MODE_INTENSITY, uint4>(P.g, P.tab, (const uint4*)P.vox, P.mu, P.pdf, P.NE,
-                                                          P.inv_xf, a, rg, jj, P.out, nullptr);
+            forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.NE, P.inv_xf, a,
+                                                    rg, jj, P.out, nullptr);
         }
     }
 }
@@ -360,10 +359,16 @@ int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks
 }
 
 int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xcd_total, int* d_counters, int nblocks,
-                         hipStream_t st) {
+                         int vox_is_f32, hipStream_t st) {
+    if (np <= 0) return 0;
     int rc = launch_zero(d_counters, 8 * sizeof(int), st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_forward_batch, dim3(nblocks), dim3(256), 0, st, d_problems, np, d_xcd_total, d_counters);
+    if (vox_is_f32)
+        hipLaunchKernelGGL(k_forward_batch<float4>, dim3(nblocks), dim3(256), 0, st, d_problems, np, d_xcd_total,
+                           d_counters);
+    else
+        hipLaunchKernelGGL(k_forward_batch<uint4>, dim3(nblocks), dim3(256), 0, st, d_problems, np, d_xcd_total,
+                           d_counters);
     VCBCT_HIP(hipGetLastError());
     return 0;
 }

@@ -99,7 +99,7 @@ struct BatchProblem {
     int block_start;    // flat form: first block of this problem in its voxel type's grid
 };
 int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xcd_total, int* d_counters, int nblocks,
-                         hipStream_t st);
+                         int vox_is_f32, hipStream_t st);
 int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks, int vox_is_f32, hipStream_t st);
 
 // Kernels launchers (forward.hip)
