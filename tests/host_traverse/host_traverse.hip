@@ -31,6 +31,10 @@ struct HostAcc {
 };
 }  // namespace
 
+// 1: waves whose lanes share the box-entry choice walk with the 64-step table (as on the GPU); 0: per lane.
+static int g_use_table = 1;
+extern "C" void ht_set_table(int on) { g_use_table = on; }
+
 extern "C" int ht_line_integrals(int N, double s, int Nz, double t, int H, int W, int nAng, double x_reg, double y_reg,
                                  double z_reg, double pitch, double pitch_iso, double sad, double pixnum, int nmat,
                                  const double* tab, const uint16_t* vol, double* L, int32_t* nseg) {
@@ -55,11 +59,14 @@ extern "C" int ht_line_integrals(int N, double s, int Nz, double t, int H, int W
                     kmin = std::min(kmin, lo);
                     kmax = std::max(kmax, hi);
                 }
+                bool same = true;
+                for (int l = 1; l < 64; ++l) same = same && r[l].swap == r[0].swap;
+                vcbct::StepRec tab[64];
                 for (int l = 0; l < 64; ++l) {
                     if (!act[l]) continue;
                     int j = rg * 64 + l;
                     HostAcc acc{vol, N, Nz, nmat, {0, 0, 0, 0, 0, 0}, 0};
-                    vcbct::traverse_lane(g, r[l], act[l], kmin, kmax, acc);
+                    vcbct::traverse_lane(g, r[l], act[l], kmin, kmax, acc, (same && g_use_table) ? tab : nullptr);
                     size_t o = (size_t)a * H * W + (size_t)(H - 1 - j) * W + jj;
                     for (int m = 0; m < nmat; ++m) L[o * nmat + m] = acc.L[m];
                     if (nseg) nseg[o] = acc.nseg;

@@ -16,11 +16,12 @@ LIBS = {"plain": os.path.join(HERE, "libhosttrav.so"), "fma": os.path.join(HERE,
 D = C.POINTER(C.c_double)
 
 
-def host_line_integrals(g, vol, lib=None):
+def host_line_integrals(g, vol, lib=None, table=True):
     lib = lib or LIB
     if not os.path.exists(lib):
         pytest.skip(f"host traversal harness not built: {lib}")
     L_ = C.CDLL(lib)
+    L_.ht_set_table(1 if table else 0)
     cs, sn, xc, yc = g.tables()
     tab = np.ascontiguousarray(np.concatenate([cs, sn, xc, yc]))
     L = np.zeros((g.n_ang, g.H, g.W, g.nmat))
@@ -43,6 +44,7 @@ def geom(N, Nz, s, t, H, W, angles_deg, xs=0.0, z_reg=0.0):
 ANGLES = [-170.0001, -90.00001, -40.00001, 9.99999, 45.0, 59.99999, 133.3, 179.99999]
 
 
+@pytest.mark.parametrize("table", [True, False], ids=["step-table", "per-lane"])
 @pytest.mark.parametrize("build", ["plain", "fma"])
 @pytest.mark.parametrize("case", [
     (32, 8, 0.5, 0.5, 24, 24, 0.3, 0.0123),
@@ -52,20 +54,21 @@ ANGLES = [-170.0001, -90.00001, -40.00001, 9.99999, 45.0, 59.99999, 133.3, 179.9
     (41, 9, 0.37, 0.21, 33, 29, 0.11, -0.2),   # odd sizes, box half-width not on the plane grid
     (16, 4, 0.8, 1.2, 14, 14, 0.0, 0.01),      # planes offset by s/2 from the voxel grid: every midpoint a tie
 ])
-def test_host_traversal_matches_oracle(case, build):
+def test_host_traversal_matches_oracle(case, build, table):
     N, Nz, s, t, H, W, zr, xs = case
     rng = np.random.default_rng(N * 1000 + Nz)
     g = geom(N, Nz, s, t, H, W, ANGLES, xs, zr)
     vol = rng.integers(0, 2000, (Nz, 6, N, N)).astype(np.uint16)
     Lo, _, _ = O.line_integrals(g, vol)
-    Lh, _ = host_line_integrals(g, vol, LIBS[build])
+    Lh, _ = host_line_integrals(g, vol, LIBS[build], table)
     den = np.maximum(np.abs(Lo).sum(-1), 1e-9)
     err = np.abs(Lo - Lh).sum(-1) / den
     assert err.max() < 1e-8, np.unravel_index(err.argmax(), err.shape)
 
 
+@pytest.mark.parametrize("table", [True, False], ids=["step-table", "per-lane"])
 @pytest.mark.parametrize("build", ["plain", "fma"])
-def test_host_traversal_random_geometries(build):
+def test_host_traversal_random_geometries(build, table):
     # seeded sweep of small geometries: pitches that do and do not divide 1 cm, offsets, z shifts, odd sizes
     rng = np.random.default_rng(777)
     for it in range(60):
@@ -80,7 +83,7 @@ def test_host_traversal_random_geometries(build):
         g = geom(N, Nz, s, t, H, W, ang, xs, zr)
         vol = rng.integers(0, 2000, (Nz, 6, N, N)).astype(np.uint16)
         Lo, _, _ = O.line_integrals(g, vol)
-        Lh, _ = host_line_integrals(g, vol, LIBS[build])
+        Lh, _ = host_line_integrals(g, vol, LIBS[build], table)
         den = np.maximum(np.abs(Lo).sum(-1), 1e-9)
         err = np.abs(Lo - Lh).sum(-1) / den
         assert err.max() < 1e-8, (it, N, s, Nz, t, H, W, xs, zr, np.unravel_index(err.argmax(), err.shape))

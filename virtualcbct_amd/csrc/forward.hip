@@ -208,7 +208,8 @@ template <int MODE, class VoxT>
 __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __restrict__ tab,
                                              const VoxT* __restrict__ vox, const double* __restrict__ mu,
                                              const double* __restrict__ pdf, int NE, double inv_xf, int a, int rg,
-                                             int jj, float* __restrict__ out, float* __restrict__ aux) {
+                                             int jj, float* __restrict__ out, float* __restrict__ aux,
+                                             StepRec* steps) {
     const int lane = threadIdx.x & 63;
     const int j = rg * 64 + lane;
     const bool active = j < g.H;
@@ -218,7 +219,7 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
     const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - j) * g.W + jj;
     if (MODE == VCBCT_MODE_SEGCOUNT) {
         CountSeg cnt;
-        traverse_wave(g, c, sn, xc, yc, j, jj, active, cnt);
+        traverse_wave(g, c, sn, xc, yc, j, jj, active, cnt, steps);
         if (active) stg(aux, o, (float)cnt.n);
         return;
     }
@@ -227,7 +228,7 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
     acc.Nz = g.Nz;
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0f;
-    const double leng = traverse_wave(g, c, sn, xc, yc, j, jj, active, acc);
+    const double leng = traverse_wave(g, c, sn, xc, yc, j, jj, active, acc, steps);
     if (!active) return;
 
     if (MODE == VCBCT_MODE_LINEINT) {
@@ -273,9 +274,11 @@ __global__ __launch_bounds__(256) void k_forward(ProjGeom g, const double* __res
                                                  const double* __restrict__ mu, const double* __restrict__ pdf, int NE,
                                                  double inv_xf, int a0, int nA, int quads_per_xcd,
                                                  float* __restrict__ out, float* __restrict__ aux) {
+    __shared__ StepRec steps[4][64];
     const Tile t = tile_of(nA, g.W, quads_per_xcd);
     if (t.col >= g.W) return;  // whole wave (column) out of range: uniform exit
-    forward_wave<MODE, VoxT>(g, tab, vox, mu, pdf, NE, inv_xf, a0 + t.a, t.rg, t.col, out, aux);
+    forward_wave<MODE, VoxT>(g, tab, vox, mu, pdf, NE, inv_xf, a0 + t.a, t.rg, t.col, out, aux,
+                             steps[threadIdx.x >> 6]);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -298,6 +301,7 @@ __device__ __forceinline__ int find_problem(const BatchProblem* __restrict__ pr,
 template <class VoxT>
 __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __restrict__ pr, int np,
                                                        const int* __restrict__ xcd_total, int* __restrict__ counters) {
+    __shared__ StepRec steps[4][64];
     const int home = blockIdx.x & 7;
     const int lane = threadIdx.x & 63;
     for (int qi = 0; qi < 8; ++qi) {
@@ -319,7 +323,7 @@ __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __res
             const int jj = q * P.cols_per_xcd + cl;
             if (jj >= P.g.W) continue;
             forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.NE, P.inv_xf, a,
-                                                    rg, jj, P.out, nullptr);
+                                                    rg, jj, P.out, nullptr, steps[threadIdx.x >> 6]);
         }
     }
 }
@@ -329,6 +333,7 @@ __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __res
 // carries over).
 template <class VoxT>
 __global__ __launch_bounds__(256) void k_forward_flat(const BatchProblem* __restrict__ pr, int np) {
+    __shared__ StepRec steps[4][64];
     const int b = blockIdx.x;
     int lo = 0, hi = np - 1;
     while (lo < hi) {  // last problem whose first block <= b
@@ -345,7 +350,7 @@ __global__ __launch_bounds__(256) void k_forward_flat(const BatchProblem* __rest
     const int jj = ((xcd * P.quads_per_xcd + q) << 2) + (threadIdx.x >> 6);
     if (jj >= P.g.W) return;
     forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.NE, P.inv_xf, a, rg, jj,
-                                            P.out, nullptr);
+                                            P.out, nullptr, steps[threadIdx.x >> 6]);
 }
 
 int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks, int vox_is_f32, hipStream_t st) {
@@ -406,6 +411,8 @@ __global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __res
                                                  const double* __restrict__ mu, const double* __restrict__ pdf, int NE,
                                                  double inv_xf, int quads_per_xcd, const float* __restrict__ tg,
                                                  float* __restrict__ gvox, double* __restrict__ loss) {
+    __shared__ StepRec step_tabs[4][64];
+    StepRec* steps = step_tabs[threadIdx.x >> 6];
     const Tile t = tile_of(g.nAng, g.W, quads_per_xcd);
     const int lane = threadIdx.x & 63;
     const int j = t.rg * 64 + lane;
@@ -420,7 +427,7 @@ __global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __res
     acc.Nz = g.Nz;
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0f;
-    traverse_wave(g, c, sn, xc, yc, j, jj, active, acc);
+    traverse_wave(g, c, sn, xc, yc, j, jj, active, acc, steps);
 
     ScatterG sc;
     sc.gvox = gvox;
@@ -460,7 +467,7 @@ __global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __res
         for (int m = 0; m < VCBCT_MAX_MAT; ++m)
             if (m < g.nmat) sc.c[m] = (float)(w * (mub[m] / se) * inv_xf);
     }
-    traverse_wave(g, c, sn, xc, yc, j, jj, active, sc);
+    traverse_wave(g, c, sn, xc, yc, j, jj, active, sc, steps);
     if (MODE == VCBCT_ADJ_LOSS && loss) {
         // wave reduction of the loss, one atomic per wave
         double v = lossc;

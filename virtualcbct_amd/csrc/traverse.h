@@ -68,27 +68,42 @@ __host__ __device__ __forceinline__ void ray_setup(const ProjGeom& g, double c, 
 
     const int nxm = (int)(g.N * 0.5), nym = nxm;
     const double sx = r.sx, sy = r.sy, sz = r.sz, px = r.px, py = r.py, pz = r.pz;
-    double xx[4], yy[4], zz[4];
-    int m = 0;
+    // candidate box hits in the reference's order: y = +nym s, y = -nym s, x = +nxm s, x = -nxm s; the
+    // reference keeps them in order (at most four) and uses the first two
+    const double lim = double(nym) * g.s, limx = double(nxm) * g.s;
+    double cx[4], cy[4], cz[4];
+    bool cv[4];
     {
-        double y0 = double(nym) * g.s;
+        double y0 = lim;
         double x0 = (y0 - sy) / (py - sy) * (px - sx) + sx;
-        if (fabs(x0) < double(nxm) * g.s) { xx[m] = x0; yy[m] = y0; zz[m] = (y0 - sy) / (py - sy) * (pz - sz) + sz; m++; }
+        cx[0] = x0; cy[0] = y0; cz[0] = (y0 - sy) / (py - sy) * (pz - sz) + sz; cv[0] = fabs(x0) < limx;
     }
     {
-        double y0 = -double(nym) * g.s;
+        double y0 = -lim;
         double x0 = (y0 - sy) / (py - sy) * (px - sx) + sx;
-        if (fabs(x0) < double(nxm) * g.s) { xx[m] = x0; yy[m] = y0; zz[m] = (y0 - sy) / (py - sy) * (pz - sz) + sz; m++; }
+        cx[1] = x0; cy[1] = y0; cz[1] = (y0 - sy) / (py - sy) * (pz - sz) + sz; cv[1] = fabs(x0) < limx;
     }
     {
-        double x0 = double(nxm) * g.s;
+        double x0 = limx;
         double y0 = (x0 - sx) / (px - sx) * (py - sy) + sy;
-        if (m < 4 && fabs(y0) < double(nym) * g.s) { xx[m] = x0; yy[m] = y0; zz[m] = (x0 - sx) / (px - sx) * (pz - sz) + sz; m++; }
+        cx[2] = x0; cy[2] = y0; cz[2] = (x0 - sx) / (px - sx) * (pz - sz) + sz; cv[2] = fabs(y0) < lim;
     }
     {
-        double x0 = -double(nxm) * g.s;
+        double x0 = -limx;
         double y0 = (x0 - sx) / (px - sx) * (py - sy) + sy;
-        if (m < 4 && fabs(y0) < double(nym) * g.s) { xx[m] = x0; yy[m] = y0; zz[m] = (x0 - sx) / (px - sx) * (pz - sz) + sz; m++; }
+        cx[3] = x0; cy[3] = y0; cz[3] = (x0 - sx) / (px - sx) * (pz - sz) + sz; cv[3] = fabs(y0) < lim;
+    }
+    const int m = (int)cv[0] + (int)cv[1] + (int)cv[2] + (int)cv[3];
+    // first and second accepted hits, chosen with selects (no indexed arrays: they would live in scratch)
+    const int f0 = cv[0] ? 0 : cv[1] ? 1 : cv[2] ? 2 : 3;
+    const int f1 = (f0 < 1 && cv[1]) ? 1 : (f0 < 2 && cv[2]) ? 2 : 3;
+    double xx[2], yy[2], zz[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int f = q ? f1 : f0;
+        xx[q] = f == 0 ? cx[0] : f == 1 ? cx[1] : f == 2 ? cx[2] : cx[3];
+        yy[q] = f == 0 ? cy[0] : f == 1 ? cy[1] : f == 2 ? cy[2] : cy[3];
+        zz[q] = f == 0 ? cz[0] : f == 1 ? cz[1] : f == 2 ? cz[2] : cz[3];
     }
     r.hit = m >= 2;
     if (!r.hit) { xx[0] = xx[1] = yy[0] = yy[1] = zz[0] = zz[1] = 0.0; }
@@ -346,13 +361,139 @@ __host__ __device__ __forceinline__ void traverse_steps(const ProjGeom& g, const
     }
 }
 
+// The wave-uniform part of plane step k (the xy path is shared by the lanes of a wave): plane position,
+// minor-axis crossing, the step's major-axis cell term and whether the reference's loop still runs (the
+// previous plane inside the box).  Same expressions, in the same order, as the per-step loop above.
+struct StepRec {
+    double uk, duk, uv;  // plane k, plane k - su, minor crossing (== uk when none)
+    int majterm;         // imaj * majmul, or -1 when the step's major index is outside the volume
+    int hv;              // minor-axis crossing in this step
+};
+
+__host__ __device__ __forceinline__ bool make_step(const ProjGeom& g, const RayCtx& r, const ChargeC& c, int k,
+                                                   StepRec& t) {
+#pragma clang fp contract(fast)
+    const double up = plane_u(r, k - 1, g.s);
+    const double uk = plane_u(r, k, g.s);
+    const double vp = (up - r.su) * r.dv + r.sv;
+    const double duk = uk - r.su;
+    const double vk = duk * r.dv + r.sv;
+    const bool inprev = up <= g.lim_u && up >= -g.lim_u && vp <= g.lim_u && vp >= -g.lim_u;
+    const int vma = (int)(vk * g.inv_s);
+    const int vmap = (int)(vp * g.inv_s);
+    const bool hv = r.vmove && ((vma != vmap) || !(vk * vp > 0.0));
+    double uv = uk;
+    if (hv) {
+        const double vb = (vma * g.s) - ((vk <= 0 && r.dirv < 0) || (vk >= 0 && r.dirv > 0) ? 0.0 : r.dirv * g.s);
+        uv = (vb - r.sv) * r.duv + r.su;
+    }
+    const int imaj = (int)((up + uk) * 0.5 * c.smaj + c.nmaj);
+    t.uk = uk;
+    t.duk = duk;
+    t.uv = uv;
+    t.majterm = (unsigned)imaj < (unsigned)g.N ? imaj * c.majmul : -1;
+    t.hv = hv ? 1 : 0;
+    return inprev;
+}
+
+// The aligned-grid walk with the wave-uniform step data computed 64 steps at a time, one step per lane, into
+// a per-wave table (`tab`, 64 records; LDS on the GPU), and read back by every lane in step order.  Only
+// the per-lane z part (z crossing, crossing order, segments) stays in the sequential loop.  Requires the
+// lanes to share their xy path bit for bit (caller checks).
+template <bool MUL24, class V>
+__host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
+                                                        int kmax, V& vis, StepRec* tab) {
+#pragma clang fp contract(fast)
+    const double lim_u = g.lim_u;
+    const double inv_t = g.inv_t;
+    const bool on = active && r.hit;
+    const ChargeC c = charge_consts(g, r);
+
+    double u0 = r.uin;
+    double v0 = (u0 - r.su) * r.dv + r.sv;
+    bool inbox0 = (u0 <= lim_u && u0 >= -lim_u && v0 <= lim_u && v0 >= -lim_u);
+    if (!inbox0) return;
+    int k = kmin;
+    double b0;
+    if (k <= 1) {
+        const Seg none{0, 0.0, false};
+        vis.step(make_seg<false, MUL24>(g, r, c, r.ui, u0, on, 0, false), none, none);
+        k = 1;
+        b0 = u0;
+    } else {
+        b0 = plane_u(r, k - 1, g.s);
+    }
+    int zmaprev = (int)(((b0 - r.su) * r.dz + r.sz) * inv_t);
+    const double dt = r.dirz * g.t;
+
+    for (;;) {
+        // steps k .. k+63: record and validity (k' <= kmax and the reference's loop still running)
+        int nsteps;
+#ifdef __HIP_DEVICE_COMPILE__
+        {
+            const int lane = threadIdx.x & 63;
+            StepRec t;
+            const bool ok = make_step(g, r, c, k + lane, t) && k + lane <= kmax;
+            tab[lane] = t;
+            const unsigned long long bad = __ballot(!ok);
+            nsteps = bad ? __ffsll((long long)bad) - 1 : 64;
+            __builtin_amdgcn_wave_barrier();
+        }
+#else
+        nsteps = 64;
+        for (int l = 0; l < 64; ++l) {
+            const bool ok = make_step(g, r, c, k + l, tab[l]) && k + l <= kmax;
+            if (!ok && nsteps == 64) nsteps = l;
+        }
+#endif
+        for (int i = 0; i < nsteps; ++i) {
+            const StepRec t = tab[i];
+            const double uk = t.uk;
+            const double zk = t.duk * r.dz + r.sz;
+            const int zma = (int)(zk * inv_t);
+            const bool hz = zma != zmaprev;
+            const bool hv = t.hv != 0;
+            const double uv = t.uv;
+            double uz = uk;
+            if (hz) {
+                const double zb = (zma * g.t) - ((zk < 0 && r.dirz < 0) || (zk > 0 && r.dirz > 0) ? 0.0 : dt);
+                uz = (zb - r.sz) * r.duz + r.su;
+            }
+            const double chv = fabs(uk - uv), chz = fabs(uk - uz);
+            double e1 = uk, e2 = uk;
+            if (hv && hz) {
+                if (chz > chv) { e1 = uz; e2 = uv; }
+                else if (chz < chv) { e1 = uv; e2 = uz; }
+            } else if (hz && chz != 0) {
+                e1 = uz;
+            } else if (hv && chv != 0) {
+                e1 = uv;
+            }
+            if (t.majterm >= 0)
+                vis.step(make_seg<true, MUL24>(g, r, c, b0, e1, on, t.majterm, true),
+                         make_seg<true, MUL24>(g, r, c, e1, e2, on && (e2 != e1), t.majterm, true),
+                         make_seg<true, MUL24>(g, r, c, e2, uk, on && (uk != e2), t.majterm, true));
+            b0 = uk;
+            zmaprev = zma;
+        }
+#ifdef __HIP_DEVICE_COMPILE__
+        __builtin_amdgcn_wave_barrier();  // all reads of this table before the next fill
+#endif
+        if (nsteps < 64) break;
+        k += 64;
+    }
+}
+
 // Walks the compat traversal of one ray (lanes of a wave in lockstep over plane steps).
 // `active`: lane holds a real detector pixel.  The visitor accumulates.
 template <class V>
 __host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
-                                                       int kmax, V& vis) {
+                                                       int kmax, V& vis, StepRec* tab = nullptr) {
     if (kmin > kmax) return;  // no lane of this wave can reach the volume's z band
-    if (g.aligned) {
+    if (g.aligned && tab) {  // the caller vouches that the lanes share their xy path
+        if (g.mul24) traverse_table<true>(g, r, active, kmin, kmax, vis, tab);
+        else traverse_table<false>(g, r, active, kmin, kmax, vis, tab);
+    } else if (g.aligned) {
         if (g.mul24) traverse_steps<true, true>(g, r, active, kmin, kmax, vis);
         else traverse_steps<true, false>(g, r, active, kmin, kmax, vis);
     } else {
@@ -388,9 +529,12 @@ __device__ __forceinline__ void share_xy_path(RayCtx& r) {
     r.duv = first_lane(r.duv);
 }
 
+// `tab`: this wave's 64-record step table in LDS.  Waves whose lanes all chose the same box entry (every
+// wave but those of rays through a box corner) share their whole xy path and walk with the step table;
+// the others walk per lane.
 template <class V>
 __device__ __forceinline__ double traverse_wave(const ProjGeom& g, double c, double sn, double xc, double yc, int j,
-                                                int jj, bool active, V& vis) {
+                                                int jj, bool active, V& vis, StepRec* tab) {
     RayCtx r;
     ray_setup(g, c, sn, xc, yc, active ? j : 0, jj, r);
     share_xy_path(r);
@@ -398,7 +542,8 @@ __device__ __forceinline__ double traverse_wave(const ProjGeom& g, double c, dou
     z_step_range(g, r, active, klo, khi);
     const int kmin = wave_min(klo);
     const int kmax = wave_max(khi);
-    traverse_lane(g, r, active, kmin, kmax, vis);
+    const bool same = __all((int)r.swap == __builtin_amdgcn_readfirstlane((int)r.swap));
+    traverse_lane(g, r, active, kmin, kmax, vis, same ? tab : nullptr);
     return r.leng;
 }
 }  // namespace vcbct
