@@ -366,9 +366,10 @@ __host__ __device__ __forceinline__ void traverse_steps(const ProjGeom& g, const
 // previous plane inside the box).  Same expressions, in the same order, as the per-step loop above.
 struct StepRec {
     double uk, duk, uv;  // plane k, plane k - su, minor crossing (== uk when none)
-    int majterm;         // imaj * majmul, or -1 when the step's major index is outside the volume
-    int hv;              // minor-axis crossing in this step
+    int majterm;         // imaj * majmul, or -1 when no segment of the step can be charged
+    int flags;           // STEP_HV: minor-axis crossing in this step
 };
+enum { STEP_HV = 1 };
 
 __host__ __device__ __forceinline__ bool make_step(const ProjGeom& g, const RayCtx& r, const ChargeC& c, int k,
                                                    StepRec& t) {
@@ -388,11 +389,18 @@ __host__ __device__ __forceinline__ bool make_step(const ProjGeom& g, const RayC
         uv = (vb - r.sv) * r.duv + r.su;
     }
     const int imaj = (int)((up + uk) * 0.5 * c.smaj + c.nmaj);
+    // The step's segments have their midpoints on the chord from (up, vp) to (uk, vk), inside the chord's
+    // bounding box (one voxel wide).  When that box lies wholly outside the inscribed circle (radius^2
+    // rmax2; margin 1e-9 cm^2, while the rounding of a midpoint is ~1e-13 cm^2) the mask rejects every
+    // segment of the step and it is skipped.
+    const double au = (up > 0.0) == (uk > 0.0) ? fmin(fabs(up), fabs(uk)) : 0.0;
+    const double av = (vp > 0.0) == (vk > 0.0) ? fmin(fabs(vp), fabs(vk)) : 0.0;
+    const bool outside = au * au + av * av > c.rmax2 + 1e-9;
     t.uk = uk;
     t.duk = duk;
     t.uv = uv;
-    t.majterm = (unsigned)imaj < (unsigned)g.N ? imaj * c.majmul : -1;
-    t.hv = hv ? 1 : 0;
+    t.majterm = ((unsigned)imaj < (unsigned)g.N && !outside) ? imaj * c.majmul : -1;
+    t.flags = hv ? STEP_HV : 0;
     return inprev;
 }
 
@@ -448,11 +456,19 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
 #endif
         for (int i = 0; i < nsteps; ++i) {
             const StepRec t = tab[i];
+#ifdef __HIP_DEVICE_COMPILE__
+            // the record is the same for every lane: its integer fields in scalar registers make the
+            // branches on them scalar
+            const int majterm = __builtin_amdgcn_readfirstlane(t.majterm);
+            const int flags = __builtin_amdgcn_readfirstlane(t.flags);
+#else
+            const int majterm = t.majterm, flags = t.flags;
+#endif
             const double uk = t.uk;
             const double zk = t.duk * r.dz + r.sz;
             const int zma = (int)(zk * inv_t);
             const bool hz = zma != zmaprev;
-            const bool hv = t.hv != 0;
+            const bool hv = (flags & STEP_HV) != 0;
             const double uv = t.uv;
             double uz = uk;
             if (hz) {
@@ -469,10 +485,10 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
             } else if (hv && chv != 0) {
                 e1 = uv;
             }
-            if (t.majterm >= 0)
-                vis.step(make_seg<true, MUL24>(g, r, c, b0, e1, on, t.majterm, true),
-                         make_seg<true, MUL24>(g, r, c, e1, e2, on && (e2 != e1), t.majterm, true),
-                         make_seg<true, MUL24>(g, r, c, e2, uk, on && (uk != e2), t.majterm, true));
+            if (majterm >= 0)
+                vis.step(make_seg<true, MUL24>(g, r, c, b0, e1, on, majterm, true),
+                         make_seg<true, MUL24>(g, r, c, e1, e2, on && (e2 != e1), majterm, true),
+                         make_seg<true, MUL24>(g, r, c, e2, uk, on && (uk != e2), majterm, true));
             b0 = uk;
             zmaprev = zma;
         }
