@@ -467,6 +467,11 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
             const double uk = t.uk;
             const double zk = t.duk * r.dz + r.sz;
             const int zma = (int)(zk * inv_t);
+            if (majterm < 0) {  // nothing charged in this step: only the z state moves on
+                b0 = uk;
+                zmaprev = zma;
+                continue;
+            }
             const bool hz = zma != zmaprev;
             const bool hv = (flags & STEP_HV) != 0;
             const double uv = t.uv;
@@ -485,10 +490,9 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
             } else if (hv && chv != 0) {
                 e1 = uv;
             }
-            if (majterm >= 0)
-                vis.step(make_seg<true, MUL24>(g, r, c, b0, e1, on, majterm, true),
-                         make_seg<true, MUL24>(g, r, c, e1, e2, on && (e2 != e1), majterm, true),
-                         make_seg<true, MUL24>(g, r, c, e2, uk, on && (uk != e2), majterm, true));
+            vis.step(make_seg<true, MUL24>(g, r, c, b0, e1, on, majterm, true),
+                     make_seg<true, MUL24>(g, r, c, e1, e2, on && (e2 != e1), majterm, true),
+                     make_seg<true, MUL24>(g, r, c, e2, uk, on && (uk != e2), majterm, true));
             b0 = uk;
             zmaprev = zma;
         }
