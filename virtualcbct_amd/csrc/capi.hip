@@ -121,6 +121,7 @@ extern "C" void vcbct_projector_destroy(vcbct_projector* p) {
     (void)hipFree(p->d_vox_u16);
     (void)hipFree(p->d_vox_f32);
     (void)hipFree(p->d_grad_vox);
+    (void)hipFree(p->d_coef);
     (void)hipFree(p->d_mu);
     (void)hipFree(p->d_pdf);
     free(p);
@@ -278,12 +279,16 @@ static int run_adjoint(vcbct_projector* p, const float* tg, float* dev_grad, dou
     const size_t nv = vox_count(p->g);
     if (!p->d_grad_vox && dev_malloc((void**)&p->d_grad_vox, nv * 32) != hipSuccess)
         return fail(VCBCT_ENOMEM, "device allocation of the gradient volume failed");
+    const size_t nrays = (size_t)p->g.nAng * p->g.H * p->g.W;
+    if (!p->d_coef && dev_malloc((void**)&p->d_coef, nrays * p->g.nmat * sizeof(float)) != hipSuccess)
+        return fail(VCBCT_ENOMEM, "device allocation of the adjoint coefficients failed");
     // scatter target [mat][y][x][z] (FP32 planes); zero with a fill kernel, not hipMemsetAsync, so the call
     // stays safe under stream capture
     rc = launch_zero(p->d_grad_vox, nv * 4 * p->g.nmat, st);
     if (rc) return rc;
     if (dev_loss && (rc = launch_zero(dev_loss, sizeof(double), st))) return rc;
-    rc = launch_adjoint(p->g, p->d_tables, p->d_vox_f32, spec_of(p), xfactor, tg, mode, p->d_grad_vox, dev_loss, st);
+    rc = launch_adjoint(p->g, p->d_tables, p->d_vox_f32, spec_of(p), xfactor, tg, mode, p->d_coef, p->d_grad_vox,
+                        dev_loss, st);
     if (rc) return rc;
     return launch_untranspose_f32(p->d_grad_vox, dev_grad, p->g, st);
 }

@@ -7,6 +7,10 @@
 // the material line integrals L_m, and the spectral sum runs in the same kernel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+
 #include <climits>
 #include <cmath>
 
@@ -379,25 +383,58 @@ int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xc
 }
 
 // ------------------------------------------------------------------------------------------------
-// Adjoint: recompute L_m for the ray (first traversal), form the per-material coefficient
+// Adjoint, in two kernels.
+//
+// k_adjoint_coef: recompute L_m for the ray (traversal over the FP32 maps), then the per-ray, per-material
+// coefficient
 //   c_m = w_ray * mubar_m / xf,  mubar_m = sum_e pdf_e e_e mu_me / sum_e pdf_e e_e,
-// with w_ray = 2 (p - target) (loss mode) or the upstream gradient dL/dp (upstream mode), then scatter
-// c_m * rr into the gradient volume along the same traversal (second traversal, FP32 atomics into the
-// planar [mat][y][x][z] layout, transposed to the caller's [mat][z][y][x] afterwards).
+// with w_ray = 2 (p - target) (loss mode, the loss summed in FP64) or the upstream gradient dL/dp (upstream
+// mode); c is stored [mat][ray].
+// k_adjoint_scatter: walk the ray again and add c_m * rr into the gradient for each charged segment (FP32
+// atomics into the planar [mat][y][x][z] layout, transposed to the caller's [mat][z][y][x] afterwards).  It
+// runs in z-slab passes: a pass walks only the plane steps that can charge the slab's slices and charges
+// only those, so the gradient lines the atomics touch in a pass stay in the last-level cache.
+// Adds c_m * rr of each charged segment to the gradient.  A float atomic instruction whose lanes name the same
+// address more than once is processed lane by lane (measured on MI355X: two lanes per address cost 6x, four
+// per address as much as 64 scattered lanes), and neighbouring detector rows often charge the same voxel
+// (rows closer than slices).  So the lanes of a slot that name the same voxel are first combined: cells
+// form runs over consecutive lanes (the lanes of a wave share the xy cell of a slot and their z is monotone
+// in the row); the run head sums its run with shuffles and alone issues the atomics.
 struct ScatterG {
     float* __restrict__ gvox;
-    int N, Nz, nmat;
     float c[VCBCT_MAX_MAT];
     size_t plane;  // N * N * Nz: one material plane of the [mat][y][x][z] scatter layout
+    int nmat, s0, ns;  // slab [s0, s0 + ns)
     __device__ __forceinline__ void seg(const Seg& s) {
-        if (!s.ok) return;
-        // planar layout: the lanes of a wave (consecutive rows, neighbouring z) hit a contiguous run of one
-        // material plane, so each atomic instruction touches few cache lines
+        const int lane = threadIdx.x & 63;
+        const bool ok = s.ok && (unsigned)(s.z - s0) < (unsigned)ns;
+        if (!__any(ok)) return;
+        const int key = ok ? s.cell : -1 - lane;  // lanes that charge nothing never join a run
+        const int prev = __shfl_up(key, 1, 64);
+        const bool start = lane == 0 || prev != key;
+        // lanes outside the current exec mask (a caller's divergent branch) end every run
+        const unsigned long long starts = __ballot(start) | ~__ballot(true);
+        // run [lane, next start): its length, for run heads
+        const unsigned long long after = lane == 63 ? 0ull : (starts >> (lane + 1));
+        const int len = after ? __ffsll((long long)after) : 64 - lane;
+        const int maxlen = wave_max(ok && start ? len : 1);
+        const float rf = (float)s.rr;
+        float v[VCBCT_MAX_MAT];
+#pragma unroll
+        for (int m = 0; m < VCBCT_MAX_MAT; ++m) v[m] = ok ? c[m] * rf : 0.f;
+        for (int i = 1; i < maxlen; ++i) {
+#pragma unroll
+            for (int m = 0; m < VCBCT_MAX_MAT; ++m) {
+                const float w = __shfl_down(v[m], i, 64);
+                if (i < len) v[m] += w;
+            }
+        }
+        if (!(ok && start)) return;
+        // planar layout: the heads (neighbouring z of one (x, y)) hit a contiguous run of one material plane
         float* p = gvox + (size_t)s.cell;
-        float rf = (float)s.rr;
 #pragma unroll
         for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-            if (m < nmat && c[m] != 0.f) atomicAdd(p + m * plane, c[m] * rf);
+            if (m < nmat && v[m] != 0.f) atomicAdd(p + m * plane, v[m]);
     }
     __device__ __forceinline__ void step(const Seg& a, const Seg& b, const Seg& c2) {
         seg(a);
@@ -407,10 +444,11 @@ struct ScatterG {
 };
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __restrict__ tab, const float4* __restrict__ vox,
-                                                 const double* __restrict__ mu, const double* __restrict__ pdf, int NE,
-                                                 double inv_xf, int quads_per_xcd, const float* __restrict__ tg,
-                                                 float* __restrict__ gvox, double* __restrict__ loss) {
+__global__ __launch_bounds__(256) void k_adjoint_coef(ProjGeom g, const double* __restrict__ tab,
+                                                      const float4* __restrict__ vox, const double* __restrict__ mu,
+                                                      const double* __restrict__ pdf, int NE, double inv_xf,
+                                                      int quads_per_xcd, const float* __restrict__ tg,
+                                                      float* __restrict__ coef, double* __restrict__ loss) {
     __shared__ StepRec step_tabs[4][64];
     StepRec* steps = step_tabs[threadIdx.x >> 6];
     const Tile t = tile_of(g.nAng, g.W, quads_per_xcd);
@@ -429,15 +467,7 @@ __global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __res
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0f;
     traverse_wave(g, c, sn, xc, yc, j, jj, active, acc, steps);
 
-    ScatterG sc;
-    sc.gvox = gvox;
-    sc.N = g.N;
-    sc.Nz = g.Nz;
-    sc.nmat = g.nmat;
-    sc.plane = (size_t)g.N * g.N * g.Nz;
     double lossc = 0.0;
-#pragma unroll
-    for (int m = 0; m < VCBCT_MAX_MAT; ++m) sc.c[m] = 0.f;
     if (active) {
         double sw = 0.0, se = 0.0, mub[VCBCT_MAX_MAT] = {0, 0, 0, 0, 0, 0};
         for (int e = 0; e < NE; ++e) {
@@ -453,21 +483,19 @@ __global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __res
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
                 if (m < g.nmat) mub[m] += we * ldg(mu, m * NE + e);
         }
+        const size_t nrays = (size_t)g.nAng * g.H * g.W;
         const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - j) * g.W + jj;
         double w;
         if (MODE == VCBCT_ADJ_LOSS) {
             double p = -log(se / sw);
-            double res = p - (double)tg[o];
+            double res = p - (double)ldg(tg, o);
             lossc = res * res;
             w = 2.0 * res;
         } else {
-            w = (double)tg[o];
+            w = (double)ldg(tg, o);
         }
-#pragma unroll
-        for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-            if (m < g.nmat) sc.c[m] = (float)(w * (mub[m] / se) * inv_xf);
+        for (int m = 0; m < g.nmat; ++m) stg(coef, (size_t)m * nrays + o, (float)(w * (mub[m] / se) * inv_xf));
     }
-    traverse_wave(g, c, sn, xc, yc, j, jj, active, sc, steps);
     if (MODE == VCBCT_ADJ_LOSS && loss) {
         // wave reduction of the loss, one atomic per wave
         double v = lossc;
@@ -475,6 +503,38 @@ __global__ __launch_bounds__(256) void k_adjoint(ProjGeom g, const double* __res
         for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
         if (lane == 0) atomicAdd(loss, v);
     }
+}
+
+__global__ __launch_bounds__(256) void k_adjoint_scatter(ProjGeom g, const double* __restrict__ tab, int quads_per_xcd,
+                                                         const float* __restrict__ coef, float* __restrict__ gvox,
+                                                         int s0, int ns) {
+    __shared__ StepRec step_tabs[4][64];
+    StepRec* steps = step_tabs[threadIdx.x >> 6];
+    const Tile t = tile_of(g.nAng, g.W, quads_per_xcd);
+    const int lane = threadIdx.x & 63;
+    const int j = t.rg * 64 + lane;
+    const int jj = t.col;
+    const int a = t.a;
+    if (jj >= g.W) return;
+    const bool active = j < g.H;
+    ScatterG sc;
+    sc.gvox = gvox;
+    sc.nmat = g.nmat;
+    sc.plane = (size_t)g.N * g.N * g.Nz;
+    sc.s0 = s0;
+    sc.ns = ns;
+    bool any = false;
+    const size_t nrays = (size_t)g.nAng * g.H * g.W;
+    const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - (active ? j : 0)) * g.W + jj;
+#pragma unroll
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m) {
+        sc.c[m] = (active && m < g.nmat) ? ldg(coef, (size_t)m * nrays + o) : 0.f;
+        any = any || sc.c[m] != 0.f;
+    }
+    if (!__any(any)) return;  // nothing to scatter for this wave (e.g. zero upstream gradient)
+    const double c = ldg(tab, a), sn = ldg(tab, g.nAng + a), xc = ldg(tab, 2 * g.nAng + a),
+                 yc = ldg(tab, 3 * g.nAng + a);
+    traverse_wave(g, c, sn, xc, yc, j, jj, active, sc, steps, s0, s0 + ns);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -548,18 +608,32 @@ int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int
 }
 
 int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double xfactor,
-                   const float* target_or_gp, int mode, float* grad_vox, double* loss, hipStream_t st) {
+                   const float* target_or_gp, int mode, float* coef, float* grad_vox, double* loss, hipStream_t st) {
     int qpx;
     int nb = n_blocks(g, g.nAng, qpx);
     if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
     const double inv_xf = 1.0 / xfactor;
     if (mode == VCBCT_ADJ_LOSS)
-        hipLaunchKernelGGL(k_adjoint<VCBCT_ADJ_LOSS>, dim3(nb), dim3(256), 0, st, g, tables, (const float4*)vox, sp.mu,
-                           sp.pdf, sp.NE, inv_xf, qpx, target_or_gp, grad_vox, loss);
+        hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_LOSS>, dim3(nb), dim3(256), 0, st, g, tables, (const float4*)vox,
+                           sp.mu, sp.pdf, sp.NE, inv_xf, qpx, target_or_gp, coef, loss);
     else
-        hipLaunchKernelGGL(k_adjoint<VCBCT_ADJ_UPSTREAM>, dim3(nb), dim3(256), 0, st, g, tables, (const float4*)vox,
-                           sp.mu, sp.pdf, sp.NE, inv_xf, qpx, target_or_gp, grad_vox, loss);
+        hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_UPSTREAM>, dim3(nb), dim3(256), 0, st, g, tables,
+                           (const float4*)vox, sp.mu, sp.pdf, sp.NE, inv_xf, qpx, target_or_gp, coef, loss);
     VCBCT_HIP(hipGetLastError());
+    // slab passes: the gradient lines of one pass (ns slices of every material plane) within ~128 MB, half
+    // of the last-level cache (VCBCT_ADJ_SLAB_MB overrides, for tuning)
+    double budget = 128e6;
+    if (const char* e = getenv("VCBCT_ADJ_SLAB_MB")) {
+        const double v = atof(e);
+        if (v > 0) budget = v * 1e6;
+    }
+    const double per_slice = (double)g.N * g.N * g.nmat * 4.0;
+    int ns = (int)std::min<double>((double)g.Nz, std::max(1.0, std::floor(budget / per_slice)));
+    for (int s0 = 0; s0 < g.Nz; s0 += ns) {
+        const int n = std::min(ns, g.Nz - s0);
+        hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, grad_vox, s0, n);
+        VCBCT_HIP(hipGetLastError());
+    }
     return 0;
 }
 

@@ -109,7 +109,7 @@ int launch_untranspose_f32(const float* src, float* dst, const ProjGeom& g, hipS
 int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int vox_is_f32, const Spectrum& sp,
                    double xfactor, int a0, int a1, float* out, int mode, float* aux, hipStream_t st);
 int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double xfactor,
-                   const float* target_or_gp, int mode, float* grad_vox, double* loss, hipStream_t st);
+                   const float* target_or_gp, int mode, float* coef, float* grad_vox, double* loss, hipStream_t st);
 
 // Zero-fill kernel (bytes must be a multiple of 4).
 int launch_zero(void* p, size_t bytes, hipStream_t st);
@@ -134,10 +134,10 @@ struct vcbct_projector {
     double* d_tables;      // [4][nAng]: cos, sin, Xc, Yc
     uint16_t* d_vox_u16;   // [N][N][Nz][8] (y, x, z, mat)
     float* d_vox_f32;      // [N][N][Nz][8]
-    float* d_grad_vox;     // adjoint scatter target [N][N][Nz][8]
+    float* d_grad_vox;     // adjoint scatter target, planar [mat][y][x][z]
+    float* d_coef;         // adjoint per-ray coefficients [mat][angle][row][col]
     double* d_mu;          // [nmat][NE]
     double* d_pdf;         // [NE]
     int NE;
     int have_u16, have_f32;
-    uint16_t* d_stage_u16; // upload staging in the reference layout
 };

@@ -24,13 +24,13 @@ def decompose(problems, iters=200, lr=0.01, init=None, nonneg=True, callback=Non
     hist = []
     for it in range(iters):
         opt.zero_grad(set_to_none=True)
-        total = 0.0
+        total = None
         for m, pr in zip(maps, problems):
             for P, t in zip(pr["projectors"], pr["targets"]):
                 p = poly_logproj(m, P, pr.get("xfactor", 1.0))
-                loss = ((p - t) ** 2).sum()
+                loss = ((p.double() - t.double()) ** 2).sum()
                 loss.backward()
-                total += float(loss.detach())
+                total = loss.detach() if total is None else total + loss.detach()
         opt.step()
         if nonneg:
             with torch.no_grad():
@@ -39,4 +39,5 @@ def decompose(problems, iters=200, lr=0.01, init=None, nonneg=True, callback=Non
         hist.append(total)
         if callback:
             callback(it, total)
+    hist = [float(h) for h in hist]  # one synchronisation at the end
     return [m.detach() for m in maps], hist

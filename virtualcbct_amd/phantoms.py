@@ -175,6 +175,19 @@ def write_xraydb(dirpath, seed=1234):
     return a_z
 
 
+def synthetic_spectrum(dirpath, kvp=100, seed=1234):
+    """Spectrum and mu/rho tables of the synthetic workloads: writes the xraydb-format CSVs and a kVp spectrum
+    CSV (1 keV bins from 10 keV) under dirpath and reads them back through the reference's matching rule
+    (attenuation_coefficient).  Returns (Attenuation [6][NE] float32, pdf [NE] float32)."""
+    from . import io
+    db = os.path.join(dirpath, "db")
+    if not os.path.exists(os.path.join(db, "rayl_5keVto150keV.csv")):
+        write_xraydb(db, seed)
+    spec = os.path.join(dirpath, f"spec{kvp}.csv")
+    write_spectrum_csv(spec, kvp=kvp)
+    return io.attenuation_coefficient(spec, db)
+
+
 # --------------------------------------------------------------------------------------------- flexmaps
 def flexmap_rows(n_angles, xshift=0.0, yshift=0.0):
     """Angle rows of the shipped flexmaps: 36 -> P/36_little_slide.flexmap (-170.0001 .. 179.99999),
