@@ -391,9 +391,9 @@ int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xc
 // with w_ray = 2 (p - target) (loss mode, the loss summed in FP64) or the upstream gradient dL/dp (upstream
 // mode); c is stored [mat][ray].
 // k_adjoint_scatter: walk the ray again and add c_m * rr into the gradient for each charged segment (FP32
-// atomics into the planar [mat][y][x][z] layout, transposed to the caller's [mat][z][y][x] afterwards).  It
-// runs in z-slab passes: a pass walks only the plane steps that can charge the slab's slices and charges
-// only those, so the gradient lines the atomics touch in a pass stay in the last-level cache.
+// atomics into the planar [mat][y][x][z] layout, transposed to the caller's [mat][z][y][x] afterwards).
+// (Splitting the scatter into z-slab passes, to keep each pass's gradient lines in the last-level cache, was
+// measured slower: the atomics are executed beyond L2 either way, and the passes repeat traversal work.)
 // Adds c_m * rr of each charged segment to the gradient.  A float atomic instruction whose lanes name the same
 // address more than once is processed lane by lane (measured on MI355X: two lanes per address cost 6x, four
 // per address as much as 64 scattered lanes), and neighbouring detector rows often charge the same voxel
@@ -404,10 +404,10 @@ struct ScatterG {
     float* __restrict__ gvox;
     float c[VCBCT_MAX_MAT];
     size_t plane;  // N * N * Nz: one material plane of the [mat][y][x][z] scatter layout
-    int nmat, s0, ns;  // slab [s0, s0 + ns)
+    int nmat;
     __device__ __forceinline__ void seg(const Seg& s) {
         const int lane = threadIdx.x & 63;
-        const bool ok = s.ok && (unsigned)(s.z - s0) < (unsigned)ns;
+        const bool ok = s.ok;
         if (!__any(ok)) return;
         const int key = ok ? s.cell : -1 - lane;  // lanes that charge nothing never join a run
         const int prev = __shfl_up(key, 1, 64);
@@ -506,8 +506,7 @@ __global__ __launch_bounds__(256) void k_adjoint_coef(ProjGeom g, const double* 
 }
 
 __global__ __launch_bounds__(256) void k_adjoint_scatter(ProjGeom g, const double* __restrict__ tab, int quads_per_xcd,
-                                                         const float* __restrict__ coef, float* __restrict__ gvox,
-                                                         int s0, int ns) {
+                                                         const float* __restrict__ coef, float* __restrict__ gvox) {
     __shared__ StepRec step_tabs[4][64];
     StepRec* steps = step_tabs[threadIdx.x >> 6];
     const Tile t = tile_of(g.nAng, g.W, quads_per_xcd);
@@ -521,8 +520,6 @@ __global__ __launch_bounds__(256) void k_adjoint_scatter(ProjGeom g, const doubl
     sc.gvox = gvox;
     sc.nmat = g.nmat;
     sc.plane = (size_t)g.N * g.N * g.Nz;
-    sc.s0 = s0;
-    sc.ns = ns;
     bool any = false;
     const size_t nrays = (size_t)g.nAng * g.H * g.W;
     const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - (active ? j : 0)) * g.W + jj;
@@ -534,7 +531,7 @@ __global__ __launch_bounds__(256) void k_adjoint_scatter(ProjGeom g, const doubl
     if (!__any(any)) return;  // nothing to scatter for this wave (e.g. zero upstream gradient)
     const double c = ldg(tab, a), sn = ldg(tab, g.nAng + a), xc = ldg(tab, 2 * g.nAng + a),
                  yc = ldg(tab, 3 * g.nAng + a);
-    traverse_wave(g, c, sn, xc, yc, j, jj, active, sc, steps, s0, s0 + ns);
+    traverse_wave(g, c, sn, xc, yc, j, jj, active, sc, steps);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -620,20 +617,8 @@ int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, co
         hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_UPSTREAM>, dim3(nb), dim3(256), 0, st, g, tables,
                            (const float4*)vox, sp.mu, sp.pdf, sp.NE, inv_xf, qpx, target_or_gp, coef, loss);
     VCBCT_HIP(hipGetLastError());
-    // slab passes: the gradient lines of one pass (ns slices of every material plane) within ~128 MB, half
-    // of the last-level cache (VCBCT_ADJ_SLAB_MB overrides, for tuning)
-    double budget = 128e6;
-    if (const char* e = getenv("VCBCT_ADJ_SLAB_MB")) {
-        const double v = atof(e);
-        if (v > 0) budget = v * 1e6;
-    }
-    const double per_slice = (double)g.N * g.N * g.nmat * 4.0;
-    int ns = (int)std::min<double>((double)g.Nz, std::max(1.0, std::floor(budget / per_slice)));
-    for (int s0 = 0; s0 < g.Nz; s0 += ns) {
-        const int n = std::min(ns, g.Nz - s0);
-        hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, grad_vox, s0, n);
-        VCBCT_HIP(hipGetLastError());
-    }
+    hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, grad_vox);
+    VCBCT_HIP(hipGetLastError());
     return 0;
 }
 

@@ -142,17 +142,15 @@ __host__ __device__ __forceinline__ void ray_setup(const ProjGeom& g, double c, 
     if (!r.hit) r.pha = 1.0;
 }
 
-// Range of plane steps k (k >= 1) whose segments can have a midpoint slice inside [s0, s1) (default: the
-// whole volume, [0, Nz)).  Slice i holds the midpoints with (int)(-z / t + nzm) == i, i.e.
-// z in ((nzm - i - 1) t, (nzm - i) t], and slice 0 also the truncated (nzm t, (nzm + 1) t].
-__host__ __device__ __forceinline__ void z_step_range(const ProjGeom& g, const RayCtx& r, bool active, int& klo, int& khi,
-                                                      int s0 = 0, int s1 = -1) {
+// Range of plane steps k (k >= 1) whose segments can have a midpoint slice inside [0, Nz-1].  Slice i holds
+// the midpoints with (int)(-z / t + nzm) == i, i.e. z in ((nzm - i - 1) t, (nzm - i) t], and slice 0 also
+// the truncated (nzm t, (nzm + 1) t].
+__host__ __device__ __forceinline__ void z_step_range(const ProjGeom& g, const RayCtx& r, bool active, int& klo, int& khi) {
     klo = INT_MAX;
     khi = INT_MIN;
     if (!active || !r.hit) return;
-    if (s1 < 0) s1 = g.Nz;
     const int nzm = (int)(g.Nz * 0.5);
-    const double zlo = (double)(nzm - s1) * g.t, zhi = (double)(nzm - s0 + (s0 == 0 ? 1 : 0)) * g.t;
+    const double zlo = (double)(nzm - g.Nz) * g.t, zhi = (double)(nzm + 1) * g.t;
     const double z0 = (r.uin - r.su) * r.dz + r.sz;   // z at plane 0
     const double dk = r.pha * g.s * r.dz;             // z change per plane step
     if (fabs(dk) < 1e-300) {
@@ -243,7 +241,6 @@ __host__ __device__ __forceinline__ int major_index(const ProjGeom& g, const Ray
 // One charged segment: voxel cell ((y * N + x) * Nz + z), length; ok = charged by this lane.
 struct Seg {
     int cell;
-    int z;  // z index of the cell (slab-restricted visitors)
     double rr;
     bool ok;
 };
@@ -274,7 +271,6 @@ __host__ __device__ __forceinline__ Seg make_seg(const ProjGeom& g, const RayCtx
     }
     sg.ok = majok && (unsigned)imin < (unsigned)g.N && (unsigned)zmc < (unsigned)g.Nz && um * um + vm * vm < c.rmax2;
     sg.cell = (int)((unsigned)minor_term<MUL24>(imin, c.mstride) + (unsigned)majterm + (unsigned)zmc);
-    sg.z = zmc;
     sg.rr = fabs(ub - ua) * r.K;
     return sg;
 }
@@ -301,7 +297,7 @@ __host__ __device__ __forceinline__ void traverse_steps(const ProjGeom& g, const
     int kstart = kmin;
     double uprev;
     if (kstart <= 1) {  // kmin 0 or 1: the lumped entry segment is needed (and cheap)
-        const Seg none{0, 0, 0.0, false};
+        const Seg none{0, 0.0, false};
         vis.step(make_seg<false, MUL24>(g, r, c, r.ui, u0, on, 0, false), none, none);
         kstart = 1;
         uprev = u0;
@@ -430,7 +426,7 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
     int k = kmin;
     double b0;
     if (k <= 1) {
-        const Seg none{0, 0, 0.0, false};
+        const Seg none{0, 0.0, false};
         vis.step(make_seg<false, MUL24>(g, r, c, r.ui, u0, on, 0, false), none, none);
         k = 1;
         b0 = u0;
@@ -558,16 +554,14 @@ __device__ __forceinline__ void share_xy_path(RayCtx& r) {
 // `tab`: this wave's 64-record step table in LDS.  Waves whose lanes all chose the same box entry (every
 // wave but those of rays through a box corner) share their whole xy path and walk with the step table;
 // the others walk per lane.
-// [s0, s1): slices whose segments the caller needs (the steps walked are culled to them; the visitor
-// drops segments of other slices itself); default the whole volume.
 template <class V>
 __device__ __forceinline__ double traverse_wave(const ProjGeom& g, double c, double sn, double xc, double yc, int j,
-                                                int jj, bool active, V& vis, StepRec* tab, int s0 = 0, int s1 = -1) {
+                                                int jj, bool active, V& vis, StepRec* tab) {
     RayCtx r;
     ray_setup(g, c, sn, xc, yc, active ? j : 0, jj, r);
     share_xy_path(r);
     int klo, khi;
-    z_step_range(g, r, active, klo, khi, s0, s1);
+    z_step_range(g, r, active, klo, khi);
     const int kmin = wave_min(klo);
     const int kmax = wave_max(khi);
     const bool same = __all((int)r.swap == __builtin_amdgcn_readfirstlane((int)r.swap));
