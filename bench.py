@@ -342,13 +342,14 @@ def main():
             "roofline": roofline}
 
     if not args.no_extras:
-        # FDK on the projections just computed
+        # FDK on the projections just computed, through a plan (tables built once, backprojector autotuned)
         proj = out
         filt = torch.empty_like(proj)
         rec = torch.zeros((w["Z"], w["R"], w["R"]), dtype=torch.float32, device=dev)
-        fdk.filter_sinogram_device(proj, inp["xs_fdk"], inp["ys_fdk"], out=filt)
-        fdk.reconstruct_image_device(rec, filt, w["zstart"], w["zstep"], w["rscale"], 1000, inp["a_fdk"],
-                                     inp["xs_fdk"])
+        plan = fdk.FdkPlan(w["H"], w["W"], w["n_ang"], w["R"], w["Z"], w["zstart"], w["zstep"], w["rscale"], 1000,
+                           inp["a_fdk"], inp["xs_fdk"], inp["ys_fdk"])
+        plan.filter(proj, filt)
+        plan.backproject(filt, rec)
         torch.cuda.synchronize()
         nrep = 5
         e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
@@ -356,22 +357,24 @@ def main():
         for _ in range(nrep):
             rec.zero_()
             e[0].record(stream)
-            fdk.filter_sinogram_device(proj, inp["xs_fdk"], inp["ys_fdk"], out=filt)
+            plan.filter(proj, filt)
             e[1].record(stream)
-            fdk.reconstruct_image_device(rec, filt, w["zstart"], w["zstep"], w["rscale"], 1000, inp["a_fdk"],
-                                         inp["xs_fdk"])
+            plan.backproject(filt, rec)
             e[2].record(stream)
             torch.cuda.synchronize()
             tf.append(e[0].elapsed_time(e[1]))
             tb.append(e[1].elapsed_time(e[2]))
+        from virtualcbct_amd.batch import autotune_log
         updates = w["R"] * w["R"] * w["Z"] * w["n_ang"]
         line["fdk"] = {"gups": updates / (np.median(tb) * 1e-3) / 1e9, "bp_ms": float(np.median(tb)),
                        "filter_ms": float(np.median(tf)),
                        "filter_rows_per_s": w["n_ang"] * w["H"] / (np.median(tf) * 1e-3),
                        "updates": updates, "shape": [w["Z"], w["R"], w["R"]],
-                       "bp_roofline_frac": (16.0 * updates / (np.median(tb) * 1e-3) / 1e9) / HBM_PEAK_GBS}
+                       "bp_roofline_frac": (16.0 * updates / (np.median(tb) * 1e-3) / 1e9) / HBM_PEAK_GBS,
+                       "autotune": [l for l in autotune_log().strip().split("\n") if l.startswith("bp ")]}
         log(f"fdk: filter {np.median(tf):.3f} ms, backprojection {np.median(tb):.3f} ms, "
             f"{line['fdk']['gups']:.1f} GUPS")
+        plan.close()
         del filt, rec
         bench_adjoint(line, inp, dev, stream, e)
 

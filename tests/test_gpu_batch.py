@@ -145,3 +145,27 @@ def test_every_grid_form_equals_plain_mixed_voxel_types(torch):
             assert torch.equal(o, p), (grid, i)
         b.close()
     Pf.close()
+
+
+def test_fdk_plan_untuned_equals_one_shot(torch, monkeypatch):
+    """With autotuning off the plan backprojects with 4 slices per thread, as the one-shot call does: the
+    results are bit-identical, and within the FDK bar of the FP64 oracle."""
+    from virtualcbct_amd import fdk
+    monkeypatch.setenv("VCBCT_NO_AUTOTUNE", "1")
+    rng = np.random.default_rng(5)
+    n, H, W = 20, 44, 60  # a geometry no other test tunes (the tuner caches per geometry)
+    proj = torch.from_numpy(rng.uniform(0.1, 1.0, (n, H, W)).astype(np.float32)).cuda()
+    ang = np.deg2rad(np.arange(n) * 18.0 - 170.0001)
+    xs = np.full(n, -0.3)
+    ys = np.full(n, 0.1)
+    plan = fdk.FdkPlan(H, W, n, 36, 7, -1, 0.45, 0.4, 1000, ang, xs, ys)
+    assert plan.slices_per_thread == 4
+    a = plan.execute(proj)
+    filt = fdk.filter_sinogram_device(proj, xs, ys)
+    b = torch.zeros((7, 36, 36), device="cuda")
+    fdk.reconstruct_image_device(b, filt, -1, 0.45, 0.4, 1000, ang, xs)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    ref = O.reconstruct_image(36, 7, -1, 0.45, 0.4, O.filter_sinogram(proj.cpu().numpy(), xs, ys), 1000, ang, xs)
+    assert np.abs(a.cpu().numpy() - ref).max() <= 1e-4 * np.abs(ref).max() + 1e-3
+    plan.close()

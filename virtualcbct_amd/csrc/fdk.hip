@@ -187,10 +187,11 @@ __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, f
 // ------------------------------------------------------------------------------------------------
 // Backprojection.  Thread = one (x, y) voxel column, ZB consecutive slices (ZB picked by the autotuner
 // among 2, 4, 8); block = 16 x 16 columns.
-// Per angle the FP64 geometry (U, its interpolation weight and base column, 1/L2) is computed once
-// per column and reused for the ZB slices.  The interpolation follows F/reconstructImage.cpp:76-175:
-// the horizontal weight is frac(U) but the base column is floor(W/2 + U + xshift_px - 0.5), and the
-// three edge cases at the last column / row sample one tap pair or one tap.
+// Per angle the geometry (U, its interpolation weight and base column, 1/L2) is computed once per column
+// and reused for the ZB slices: U in FP64 in the reference's order, the vertical coordinate in FP32.  The
+// interpolation follows F/reconstructImage.cpp:76-175: the horizontal weight is frac(U) but the base column
+// is floor(W/2 + U + xshift_px - 0.5), and the three edge cases at the last column / row sample one tap pair
+// or one tap.  (A precomputed per-(column, angle) table was measured slower: each z-block re-reads it.)
 template <int ZB>
 __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, const float* __restrict__ sino, int R,
                                                      int Z, double zstart, double step, double scale, int H, int W,
@@ -205,16 +206,19 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
     const double centerPos = (double)R / 2.0 - 0.5;
     const double xx = (x - centerPos) * scale;
     const double yy = -(y - centerPos) * scale;
-    double zz[ZB];
+    float zz[ZB];
     float acc[ZB];
 #pragma unroll
     for (int i = 0; i < ZB; ++i) {
-        zz[i] = zstart + (double)(z0 + i) * step;
+        zz[i] = (float)(zstart + (double)(z0 + i) * step);
         acc[i] = 0.f;
     }
-    const double hW = (double)(W / 2), hH = (double)(H / 2);
+    const double hW = (double)(W / 2);
+    const float hH = (float)(H / 2);
     for (int th = 0; th < n; ++th) {
         const double c = atab[th], s = atab[n + th], dbeta = atab[2 * n + th], xs = atab[3 * n + th];
+        // horizontal: FP64 in the reference's operation order (its base column floor(W/2 + U + xs - 0.5)
+        // and weight frac(U) jump at different points, so U must round as the reference's does)
         double X = xx * c + yy * s;
         double Y = -xx * s + yy * c;
         double L2 = 1.0 - Y / 100.0;
@@ -222,18 +226,20 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
         double fU = floor(U);
         double w = U - fU;
         int UU = (int)floor(hW + U + xs - 0.5);
-        double iL = 1.0 / L2 / pitchW;
-        float wgt = (float)(1.0 / L2 / L2 * dbeta * xfactor);
         const bool ucol = (UU >= 0 && UU < W - 1), uedge = (UU == W - 1);
         if (!ucol && !uedge) continue;
+        // vertical: an ordinary bilinear tap, continuous in the row coordinate, so FP32 suffices
+        const double rL2 = 1.0 / L2;
+        const float iL = (float)(rL2 / pitchW);
+        const float wgt = (float)(rL2 * rL2 * dbeta * xfactor);
         const float* P = sino + (size_t)th * H * W;
         const float wf = (float)w;
 #pragma unroll
         for (int i = 0; i < ZB; ++i) {
-            double ZETA = hH + zz[i] * iL;
-            double fz = floor(ZETA);
-            int NZ = (int)fz;
-            float wz = (float)(ZETA - fz);
+            const float ZETA = hH + zz[i] * iL;
+            const float fz = floorf(ZETA);
+            const int NZ = (int)fz;
+            const float wz = ZETA - fz;
             float sum;
             if (NZ >= 0 && NZ < H - 1) {
                 const float* r0 = P + (size_t)NZ * W + UU;

@@ -112,7 +112,8 @@ def to_short(recon):
 
 class FdkPlan:
     """Per-geometry FDK plan (tables and filter response built once; execution allocates nothing and can be
-    captured in a hipGraph).  The backprojector's slices-per-thread is autotuned at creation."""
+    captured in a hipGraph).  The backprojector variant (per-column geometry table or direct, slices per
+    thread) is autotuned at creation."""
 
     def __init__(self, projHeight, projWidth, nProj, reconSize, reconSlices, zstart, reconStep, reconScale, xfactor,
                  anglesRad, xshift, yshift, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, device=0):
@@ -132,6 +133,21 @@ class FdkPlan:
     @property
     def slices_per_thread(self):
         return lib().vcbct_fdk_plan_slices_per_thread(self.h)
+
+    def filter(self, proj, filtered=None, stream=None):
+        """filterSinogram on the device: -log, ramp convolution, cosine weight; returns the filtered set."""
+        torch = _torch()
+        if filtered is None:
+            filtered = torch.empty(self.shape_proj, dtype=torch.float32, device=proj.device)
+        check(lib().vcbct_fdk_plan_filter(self.h, C.c_void_p(proj.contiguous().data_ptr()),
+                                          C.c_void_p(filtered.data_ptr()), _stream(stream)))
+        return filtered
+
+    def backproject(self, filtered, recon, stream=None):
+        """reconstructImage on the device: accumulates (+=) into recon (FP32 [Z][R][R])."""
+        check(lib().vcbct_fdk_plan_backproject(self.h, C.c_void_p(filtered.contiguous().data_ptr()),
+                                              C.c_void_p(recon.data_ptr()), _stream(stream)))
+        return recon
 
     def execute(self, proj, filtered=None, recon=None, stream=None):
         """filter + backprojection; recon (FP32 [Z][R][R]) is accumulated into (zeroed if created here)."""
