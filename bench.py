@@ -89,6 +89,46 @@ def cpu_baseline(inp, seconds_target=15.0):
             "sample": f"1 angle x 160000 rays x 91 bins with the FP64 oracle, {sec:.1f} s"}
 
 
+def bench_fdk(line, inp, dev, stream, out):
+    import torch
+    from virtualcbct_amd import fdk
+    w = inp["w"]
+    # FDK on the projections just computed, through a plan (tables built once, backprojector autotuned)
+    proj = out
+    filt = torch.empty_like(proj)
+    rec = torch.zeros((w["Z"], w["R"], w["R"]), dtype=torch.float32, device=dev)
+    plan = fdk.FdkPlan(w["H"], w["W"], w["n_ang"], w["R"], w["Z"], w["zstart"], w["zstep"], w["rscale"], 1000,
+                       inp["a_fdk"], inp["xs_fdk"], inp["ys_fdk"])
+    plan.filter(proj, filt)
+    plan.backproject(filt, rec)
+    torch.cuda.synchronize()
+    nrep = 5
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    tf, tb = [], []
+    for _ in range(nrep):
+        rec.zero_()
+        e[0].record(stream)
+        plan.filter(proj, filt)
+        e[1].record(stream)
+        plan.backproject(filt, rec)
+        e[2].record(stream)
+        torch.cuda.synchronize()
+        tf.append(e[0].elapsed_time(e[1]))
+        tb.append(e[1].elapsed_time(e[2]))
+    from virtualcbct_amd.batch import autotune_log
+    updates = w["R"] * w["R"] * w["Z"] * w["n_ang"]
+    line["fdk"] = {"gups": updates / (np.median(tb) * 1e-3) / 1e9, "bp_ms": float(np.median(tb)),
+                   "filter_ms": float(np.median(tf)),
+                   "filter_rows_per_s": w["n_ang"] * w["H"] / (np.median(tf) * 1e-3),
+                   "updates": updates, "shape": [w["Z"], w["R"], w["R"]],
+                   "bp_roofline_frac": (16.0 * updates / (np.median(tb) * 1e-3) / 1e9) / HBM_PEAK_GBS,
+                   "autotune": [l for l in autotune_log().strip().split("\n") if l.startswith("bp ")]}
+    log(f"fdk: filter {np.median(tf):.3f} ms, backprojection {np.median(tb):.3f} ms, "
+        f"{line['fdk']['gups']:.1f} GUPS")
+    plan.close()
+    del filt, rec
+
+
 def bench_adjoint(line, inp, dev, stream, e):
     import torch
     from virtualcbct_amd.projector import Projector
@@ -254,7 +294,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline forward only (used under the profiler)")
-    ap.add_argument("--only", choices=["adjoint", "batched", "decompose"], default=None,
+    ap.add_argument("--only", choices=["adjoint", "batched", "decompose", "fdk"], default=None,
                     help="run one secondary section alone and print its JSON (used under the profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (scale for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -291,6 +331,13 @@ def main():
     NE = len(inp["pdf"])
     stream = torch.cuda.current_stream()
 
+    if args.only == "fdk":  # projections of the headline workload, then the FDK plan (filter + backprojection)
+        out = torch.empty(P.shape_proj, dtype=torch.float32, device=dev)
+        P.forward(1000.0, out=out)
+        line = {"section": "fdk"}
+        bench_fdk(line, inp, dev, stream, out)
+        print(json.dumps(line), flush=True)
+        return
     if args.only:
         line = {"section": args.only}
         e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
@@ -342,40 +389,8 @@ def main():
             "roofline": roofline}
 
     if not args.no_extras:
-        # FDK on the projections just computed, through a plan (tables built once, backprojector autotuned)
-        proj = out
-        filt = torch.empty_like(proj)
-        rec = torch.zeros((w["Z"], w["R"], w["R"]), dtype=torch.float32, device=dev)
-        plan = fdk.FdkPlan(w["H"], w["W"], w["n_ang"], w["R"], w["Z"], w["zstart"], w["zstep"], w["rscale"], 1000,
-                           inp["a_fdk"], inp["xs_fdk"], inp["ys_fdk"])
-        plan.filter(proj, filt)
-        plan.backproject(filt, rec)
-        torch.cuda.synchronize()
-        nrep = 5
+        bench_fdk(line, inp, dev, stream, out)
         e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-        tf, tb = [], []
-        for _ in range(nrep):
-            rec.zero_()
-            e[0].record(stream)
-            plan.filter(proj, filt)
-            e[1].record(stream)
-            plan.backproject(filt, rec)
-            e[2].record(stream)
-            torch.cuda.synchronize()
-            tf.append(e[0].elapsed_time(e[1]))
-            tb.append(e[1].elapsed_time(e[2]))
-        from virtualcbct_amd.batch import autotune_log
-        updates = w["R"] * w["R"] * w["Z"] * w["n_ang"]
-        line["fdk"] = {"gups": updates / (np.median(tb) * 1e-3) / 1e9, "bp_ms": float(np.median(tb)),
-                       "filter_ms": float(np.median(tf)),
-                       "filter_rows_per_s": w["n_ang"] * w["H"] / (np.median(tf) * 1e-3),
-                       "updates": updates, "shape": [w["Z"], w["R"], w["R"]],
-                       "bp_roofline_frac": (16.0 * updates / (np.median(tb) * 1e-3) / 1e9) / HBM_PEAK_GBS,
-                       "autotune": [l for l in autotune_log().strip().split("\n") if l.startswith("bp ")]}
-        log(f"fdk: filter {np.median(tf):.3f} ms, backprojection {np.median(tb):.3f} ms, "
-            f"{line['fdk']['gups']:.1f} GUPS")
-        plan.close()
-        del filt, rec
         bench_adjoint(line, inp, dev, stream, e)
 
     if not args.no_extras:

@@ -215,6 +215,7 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
     }
     const double hW = (double)(W / 2);
     const float hH = (float)(H / 2);
+    const float ipW = (float)(1.0 / pitchW);
     for (int th = 0; th < n; ++th) {
         const double c = atab[th], s = atab[n + th], dbeta = atab[2 * n + th], xs = atab[3 * n + th];
         // horizontal: FP64 in the reference's operation order (its base column floor(W/2 + U + xs - 0.5)
@@ -228,10 +229,10 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
         int UU = (int)floor(hW + U + xs - 0.5);
         const bool ucol = (UU >= 0 && UU < W - 1), uedge = (UU == W - 1);
         if (!ucol && !uedge) continue;
-        // vertical: an ordinary bilinear tap, continuous in the row coordinate, so FP32 suffices
-        const double rL2 = 1.0 / L2;
-        const float iL = (float)(rL2 / pitchW);
-        const float wgt = (float)(rL2 * rL2 * dbeta * xfactor);
+        // vertical: an ordinary bilinear tap, continuous in the row coordinate, and the angle weight: FP32
+        const float rL2 = 1.0f / (float)L2;
+        const float iL = rL2 * ipW;
+        const float wgt = rL2 * rL2 * (float)(dbeta * xfactor);
         const float* P = sino + (size_t)th * H * W;
         const float wf = (float)w;
 #pragma unroll
