@@ -287,6 +287,60 @@ def bench_decompose(line, td, dev, stream, e):
             P.close()
 
 
+def bench_phantom(line, td, dev, stream, e):
+    """Organ-label phantom -> elemental maps: a 543 x 271 x 100 label volume (the ICRP adult-male slice size
+    and the reference converter's slice count) to (a) the reference's per-slice layout [100][6][271][543] u16 and
+    (b) a 512 x 512 x 100 projector volume (centred crop / pad fused in).  CPU beside it: the oracle's
+    restatement of the reference's voxel loop (every organ row tested per voxel) on the same labels."""
+    import torch
+    from oracle import oracle as O
+    from virtualcbct_amd import io, material_maps, phantoms
+    from virtualcbct_amd.projector import Projector
+    media, organs = phantoms.write_material_tables(os.path.join(td, "tables"), seed=3)
+    lab_h = phantoms.synthetic_labels(543, 271, 100, seed=1)
+    lab = torch.from_numpy(lab_h).to(dev)
+    lut, _ = material_maps.material_table(media, organs)
+    nvox = lab_h.size
+    maps = material_maps.labels_to_maps(lab, lut)
+    ang, xs, ys = phantoms.flexmap_angles(1)
+    c, s, x, y = io.trig_table(16, 16, ang, xs, ys)
+    P = Projector(512, 0.05, 100, 0.2, 16, 16, 1, c, s, x, y)
+    P.set_volume_labels(lab, lut)
+    torch.cuda.synchronize()
+    nrep = 20
+    t = {}
+    for name, fn in (("maps", lambda: material_maps.labels_to_maps(lab, lut, out=maps)),
+                     ("projector_volume", lambda: P.set_volume_labels(lab, lut))):
+        e[0].record(stream)
+        for _ in range(nrep):
+            fn()
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        t[name] = e[0].elapsed_time(e[1]) / nrep
+    c0, r0 = material_maps.centred_placement(543, 271, 512)
+    read_b = 100 * min(271, 512) * min(543, 512)  # labels inside the placed window
+    pv = 512 * 512 * 100
+    res = {"workload": "labels 543x271x100 u8 -> maps [100][6][271][543] u16 | projector volume 512x512x100 "
+                       f"(16-byte voxels, placement col0={c0} row0={r0})",
+           "maps_ms": t["maps"], "maps_gb_s": 13.0 * nvox / (t["maps"] * 1e-3) / 1e9,
+           "projector_volume_ms": t["projector_volume"],
+           "projector_volume_gb_s": (read_b + 16.0 * pv) / (t["projector_volume"] * 1e-3) / 1e9,
+           "maps_voxels_per_s": nvox / (t["maps"] * 1e-3)}
+    res["maps_roofline_frac"] = res["maps_gb_s"] / HBM_PEAK_GBS
+    res["projector_volume_roofline_frac"] = res["projector_volume_gb_s"] / HBM_PEAK_GBS
+    w = O.material_weights(media, True, 1)
+    org = O.organ_table(organs)
+    t0 = time.perf_counter()
+    O.labels_to_maps(lab_h, w, org)
+    sec = time.perf_counter() - t0
+    res["cpu_port"] = {"voxels_per_s": nvox / sec, "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+                       "sample": f"the whole 543x271x100 volume, {len(org[0])} organ rows, {sec:.2f} s"}
+    line["phantom"] = res
+    log(f"phantom: maps {t['maps']:.3f} ms ({res['maps_gb_s']:.0f} GB/s), projector volume "
+        f"{t['projector_volume']:.3f} ms ({res['projector_volume_gb_s']:.0f} GB/s), CPU {sec:.2f} s")
+    P.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,7 +348,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline forward only (used under the profiler)")
-    ap.add_argument("--only", choices=["adjoint", "batched", "decompose", "fdk"], default=None,
+    ap.add_argument("--only", choices=["adjoint", "batched", "decompose", "fdk", "phantom"], default=None,
                     help="run one secondary section alone and print its JSON (used under the profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (scale for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -345,6 +399,8 @@ def main():
             bench_adjoint(line, inp, dev, stream, e)
         elif args.only == "decompose":
             bench_decompose(line, td, dev, stream, e)
+        elif args.only == "phantom":
+            bench_phantom(line, td, dev, stream, e)
         else:
             bench_batched(line, inp, dev, stream, e, NE)
         print(json.dumps(line), flush=True)
@@ -396,6 +452,7 @@ def main():
     if not args.no_extras:
         bench_batched(line, inp, dev, stream, e, NE)
         bench_decompose(line, td, dev, stream, e)
+        bench_phantom(line, td, dev, stream, e)
 
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

@@ -102,6 +102,44 @@ int vcbct_reprojection(vcbct_projector* p, const uint16_t* host_vol, const float
                        const float* host_pdf, int NE, double xfactor, float* host_out);
 
 /* ------------------------------------------------------------------------------------------------
+ * Organ-label phantom -> six elemental maps, the step before the projector
+ * (ICRPphantoms_2mmslice/To_material_distribution_for512_random.cpp; R/ below).
+ *
+ * Replaces the table part of R/ :104-201 and the per-voxel lookup rule of :212-226.  media_csv rows are
+ * "ID,H,C,N,O,Na,Mg,P,S,Cl,K,Ca,Fe,I" (weight %, R/AM_media.csv), organs_csv rows "organ,media,density"
+ * (g/cc, R/AM_organs.csv).  host_lut [256][8] receives, for every 8-bit organ label, the six map values
+ * H, C, N, O, P, Ca = (short)(weight% x density x 10) in FP32 as the reference computes them (the
+ * organ's media is row media-1; the last organ row with that label wins); labels with no organ row map
+ * to 0; entries 6 and 7 are 0.  With randomize != 0 the H, C, N weights, and P and Ca where above 0.09 %,
+ * of every medium except ID 53 are redrawn from N(w, 0.1 w) and clamped at 0 (:117-150), by glibc's
+ * rand() sequence after srand(seed) (seed 1 is the reference's unseeded rand(); a private state, the
+ * process's rand() is untouched).  O then takes 100 - H - C - N - P - Ca, renormalised when negative
+ * (:151-162).  The reference's density draw (:178-198) is not made: its result is never used.
+ * host_media_w (optional, [media_cap][6]) receives the H, C, N, O, P, Ca weights used.
+ * Returns the number of media rows, or a negative error. */
+int vcbct_material_table(const char* media_csv, const char* organs_csv, int randomize, unsigned int seed,
+                         uint16_t* host_lut, float* host_media_w, int media_cap);
+
+/* Replaces the voxel loop of R/ :212-226 and the slice writer's layout (:242-274): labels u8
+ * [nz][nrow][ncol] -> maps u16 [nz][6][nrow][ncol] (slice iz of dev_maps is the reference's per-slice
+ * material file MD_<ncol>_<nrow>_<iz>.raw).  dev_lut [256][8] as vcbct_material_table writes it. */
+int vcbct_labels_to_maps(const uint8_t* dev_labels, int ncol, int nrow, int nz, const uint16_t* dev_lut,
+                         uint16_t* dev_maps, void* stream);
+
+/* The same lookup written straight into the projector's device volume (no reference counterpart: the
+ * reference leaves the step from the converter's ncol x nrow slices to the projector's N x N slices to
+ * the user).  Projector voxel (x, y, z) takes the label at column x + col0, row y + row0, slice
+ * z + slice0 of dev_labels [nz][nrow][ncol]; outside that grid it takes label 0.  Materials beyond the
+ * projector's num_material are dropped.  Equivalent to vcbct_labels_to_maps + cropping or padding to
+ * N x N + vcbct_projector_set_volume_u16, without the intermediate maps. */
+int vcbct_projector_set_volume_labels(vcbct_projector* p, const uint8_t* dev_labels, int ncol, int nrow, int nz,
+                                      int col0, int row0, int slice0, const uint16_t* dev_lut, void* stream);
+/* The projector's u16 device volume back in the reference layout [z][mat][y][x] (no reference
+ * counterpart): the N x N slices a label phantom produced, as the projector's slice loader would read them
+ * (P/projectionData.cpp:190-215), for writing them out or checking them. */
+int vcbct_projector_get_volume_u16(vcbct_projector* p, uint16_t* dev_vol, void* stream);
+
+/* ------------------------------------------------------------------------------------------------
  * Differentiable form (no reference counterpart; the reference's scatter U-Net, Unet_scatter.py, is the
  * intended consumer).  Log projection p = -log( sum_e pdf_e exp(-sum_m mu_me L_m/xf) / sum_e pdf_e ).
  */

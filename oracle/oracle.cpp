@@ -767,3 +767,130 @@ int orc_fdk_angles(const char* flexmap, const char* tgt_class, int total, double
 }
 
 }  // extern "C"
+
+/* ------------------------------------------------------------------------------------------------
+ * Organ-label phantom -> elemental maps (ICRPphantoms_2mmslice/To_material_distribution_for512_random.cpp,
+ * R/ below).
+ *
+ * rand() of glibc, restated from its published description (the additive feedback generator behind
+ * random() with the default 128-byte state): h[0] = seed (0 is taken as 1), h[i] = 16807 h[i-1] mod
+ * (2^31 - 1) for 0 < i < 31, h[i] = h[i-31] for 31 <= i < 34, h[i] = h[i-31] + h[i-3] mod 2^32 from
+ * i = 34 on; the k-th rand() after srand(seed) is h[344 + k] >> 1. */
+struct OrcRand {
+    std::vector<uint32_t> h;
+    explicit OrcRand(unsigned seed) : h(344) {
+        h[0] = seed ? seed : 1u;
+        for (int i = 1; i < 31; i++) {
+            long long v = (16807LL * (int32_t)h[i - 1]) % 2147483647LL;
+            if (v < 0) v += 2147483647LL;
+            h[i] = (uint32_t)v;
+        }
+        for (int i = 31; i < 34; i++) h[i] = h[i - 31];
+        for (int i = 34; i < 344; i++) h[i] = h[i - 31] + h[i - 3];
+    }
+    int next() {
+        size_t n = h.size();
+        h.push_back(h[n - 31] + h[n - 3]);
+        return (int)(h.back() >> 1);
+    }
+    /* Uniform0, R/ :14-17 */
+    double uniform() { return ((double)next() + 1.0) / ((double)2147483647 + 2.0); }
+    /* rand_gauss0, R/ :19-24; the log factor's draw is taken before the sine's */
+    double gauss(double mu, double sigma) {
+        double PI = 3.14159265;
+        double a = sqrt(-2.0 * log(uniform()));
+        double z = a * sin(2.0 * PI * uniform());
+        return mu + sigma * z;
+    }
+};
+
+extern "C" {
+
+int orc_rand_sequence(unsigned seed, int n, int* out) {
+    OrcRand r(seed);
+    for (int i = 0; i < n; i++) out[i] = r.next();
+    return n;
+}
+
+int orc_material_weights(const char* media_csv, int randomize, unsigned seed, float* wout, int cap) {
+    /* R/ :104-166: rows in file order; H, C, N always and P, Ca above 0.09 % redrawn from N(w, 0.1 w)
+     * except medium 53; O = 100 minus the five, renormalised when negative.  wout [cap][6]: H C N O P Ca */
+    FILE* fp = fopen(media_csv, "r");
+    if (!fp) return -1;
+    float Hw, Cw, Nw, Ow, Naw, Maw, Pw, Sw, Clw, Kw, Caw, Few, Iw;
+    int ID, i = 0;
+    OrcRand rng(seed);
+    double sigma, mu;
+    while (fscanf(fp, "%d,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f,%f", &ID, &Hw, &Cw, &Nw, &Ow, &Naw, &Maw, &Pw, &Sw,
+                  &Clw, &Kw, &Caw, &Few, &Iw) == 14) {
+        if (i >= cap) { fclose(fp); return -2; }
+        if (randomize && ID != 53) {
+            sigma = double(Hw * 0.1); mu = double(Hw);
+            Hw = (float)rng.gauss(mu, sigma); if (Hw < 0.0) Hw = 0.0;
+            sigma = double(Cw * 0.1); mu = double(Cw);
+            Cw = (float)rng.gauss(mu, sigma); if (Cw < 0.0) Cw = 0.0;
+            sigma = double(Nw * 0.1); mu = double(Nw);
+            Nw = (float)rng.gauss(mu, sigma); if (Nw < 0.0) Nw = 0.0;
+            if (Pw > 0.09) {
+                sigma = double(Pw * 0.1); mu = double(Pw);
+                Pw = (float)rng.gauss(mu, sigma); if (Pw < 0.0) Pw = 0.0;
+            }
+            if (Caw > 0.09) {
+                sigma = double(Caw * 0.1); mu = double(Caw);
+                Caw = (float)rng.gauss(mu, sigma); if (Caw < 0.0) Caw = 0.0;
+            }
+        }
+        Ow = 100 - Hw - Cw - Nw - Pw - Caw;
+        if (Ow < 0.0) {
+            double s = (Hw + Cw + Nw + Pw + Caw) * 0.01;
+            Ow = 0.0;
+            Hw = Hw / s; Cw = Cw / s; Nw = Nw / s; Pw = Pw / s; Caw = Caw / s;
+        }
+        float* w = wout + (size_t)i * 6;
+        w[0] = Hw; w[1] = Cw; w[2] = Nw; w[3] = Ow; w[4] = Pw; w[5] = Caw;
+        i++;
+    }
+    fclose(fp);
+    return i;
+}
+
+int orc_organ_table(const char* organs_csv, int* organ, int* medium, float* dens, int cap) {
+    /* R/ :168-201 (the density draw there is never used and is not made) */
+    FILE* fp = fopen(organs_csv, "r");
+    if (!fp) return -1;
+    int i = 0, a, b;
+    float d;
+    while (fscanf(fp, "%d,%d,%f", &a, &b, &d) == 3) {
+        if (i >= cap) { fclose(fp); return -2; }
+        organ[i] = a; medium[i] = b; dens[i] = d;
+        i++;
+    }
+    fclose(fp);
+    return i;
+}
+
+int orc_labels_to_maps(const uint8_t* labels, int ncol, int nrow, int nz, const float* w, int nmedia,
+                       const int* organ, const int* medium, const float* dens, int norg, int16_t* maps) {
+    /* R/ :212-226 per voxel (every organ row tested, the last match wins; a label without a row leaves the
+     * voxel at the zero of a fresh allocation), laid out as the per-slice files of R/ :257-270:
+     * maps [nz][6][nrow][ncol] */
+    for (int j = 0; j < norg; j++)
+        if (medium[j] < 1 || medium[j] > nmedia) return -3;
+    const size_t n2 = (size_t)ncol * nrow;
+#pragma omp parallel for schedule(static)
+    for (long long zz = 0; zz < (long long)nz; zz++) {
+        for (size_t n = 0; n < n2; n++) {
+            int16_t v[6] = {0, 0, 0, 0, 0, 0};
+            const int lab = labels[(size_t)zz * n2 + n];
+            for (int j = 0; j < norg; j++)
+                if (organ[j] == lab) {
+                    const float* m = w + (size_t)(medium[j] - 1) * 6;
+                    for (int k = 0; k < 6; k++) v[k] = (short)(m[k] * dens[j] * 10);
+                }
+            for (int k = 0; k < 6; k++) maps[((size_t)zz * 6 + k) * n2 + n] = v[k];
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -22,6 +22,11 @@
  *   orc_mask                 F/misc.cpp:9-37            (prepareMaskData)
  *   orc_reconstruct_image    F/reconstructImage.cpp:10-184 (reconstructImage)
  *   orc_fdk_angles           F/projectionDataMVkV.cpp:140-281 (flexmap read with CW/CCW reversal)
+ *   orc_rand_sequence        glibc rand() after srand(seed), the generator R/ :14-24 draws from
+ *   orc_material_weights     R/ :104-166 (media table with the weight draws; R/ = ICRPphantoms_2mmslice/
+ *                            To_material_distribution_for512_random.cpp)
+ *   orc_organ_table          R/ :168-201
+ *   orc_labels_to_maps       R/ :212-226 (voxel loop) in the per-slice layout of R/ :257-270
  */
 #ifndef VCBCT_ORACLE_H
 #define VCBCT_ORACLE_H
@@ -100,6 +105,16 @@ void orc_reconstruct_image(int R, int Z, int zstart, double step, double scale, 
  * used projections written into angles_rad/xshift/yshift (capacity cap), or a negative error. */
 int orc_fdk_angles(const char* flexmap, const char* tgt_class, int total, double* angles_rad,
                    double* xshift, double* yshift, int cap);
+
+/* Organ-label phantom path.  The first n values of glibc's rand() after srand(seed). */
+int orc_rand_sequence(unsigned seed, int n, int* out);
+/* Returns the number of media rows (<= cap) or a negative error; w [cap][6] = H C N O P Ca weight %. */
+int orc_material_weights(const char* media_csv, int randomize, unsigned seed, float* w, int cap);
+/* Returns the number of organ rows (<= cap) or a negative error. */
+int orc_organ_table(const char* organs_csv, int* organ, int* medium, float* dens, int cap);
+/* labels u8 [nz][nrow][ncol] -> maps int16 [nz][6][nrow][ncol]. */
+int orc_labels_to_maps(const uint8_t* labels, int ncol, int nrow, int nz, const float* w, int nmedia,
+                       const int* organ, const int* medium, const float* dens, int norg, int16_t* maps);
 
 #ifdef __cplusplus
 }

@@ -369,3 +369,87 @@ def ref_proj_load(txt, workdir, cap=4096, vol_cap=0):
     raw = strs.raw
     s = [raw[i * 256:(i + 1) * 256].split(b"\0")[0].decode() for i in range(4)]
     return dict(ints=ints, dbls=dbls, strs=s, angles=a[:n], xshift=x[:n], yshift=y[:n], vol=vol)
+
+
+# --------------------------------------------------------------------------------------------------
+# Organ-label phantom path (ICRPphantoms_2mmslice/To_material_distribution_for512_random.cpp).
+
+def rand_sequence(seed, n):
+    out = np.zeros(n, np.int32)
+    lib().orc_rand_sequence(C.c_uint(seed), n, _p(out, _I32))
+    return out
+
+
+def material_weights(media_csv, randomize=True, seed=1, cap=256):
+    """Per-medium weight % [n][6] (H C N O P Ca) after the reference's draws."""
+    w = np.zeros((cap, 6), np.float32)
+    n = lib().orc_material_weights(os.fspath(media_csv).encode(), int(bool(randomize)), C.c_uint(seed), _p(w, _F), cap)
+    if n < 0:
+        raise RuntimeError(f"orc_material_weights failed ({n})")
+    return w[:n].copy()
+
+
+def organ_table(organs_csv, cap=1024):
+    organ = np.zeros(cap, np.int32)
+    medium = np.zeros(cap, np.int32)
+    dens = np.zeros(cap, np.float32)
+    n = lib().orc_organ_table(os.fspath(organs_csv).encode(), _p(organ, _I32), _p(medium, _I32), _p(dens, _F), cap)
+    if n < 0:
+        raise RuntimeError(f"orc_organ_table failed ({n})")
+    return organ[:n].copy(), medium[:n].copy(), dens[:n].copy()
+
+
+def labels_to_maps(labels, weights, organs):
+    """labels uint8 [nz][nrow][ncol] -> int16 [nz][6][nrow][ncol] (the reference's voxel loop)."""
+    lab = np.ascontiguousarray(labels, np.uint8)
+    nz, nrow, ncol = lab.shape
+    w = np.ascontiguousarray(weights, np.float32)
+    organ, medium, dens = (np.ascontiguousarray(a) for a in organs)
+    out = np.empty((nz, 6, nrow, ncol), np.int16)
+    rc = lib().orc_labels_to_maps(lab.ctypes.data_as(C.POINTER(C.c_uint8)), ncol, nrow, nz, _p(w, _F), len(w),
+                                  _p(organ, _I32), _p(medium, _I32), _p(dens, _F), len(organ),
+                                  out.ctypes.data_as(C.POINTER(C.c_int16)))
+    if rc:
+        raise RuntimeError(f"orc_labels_to_maps failed ({rc})")
+    return out
+
+
+def material_lut(media_csv, organs_csv, randomize=True, seed=1):
+    """[256][6] int16: the voxel loop applied to the labels 0..255."""
+    w = material_weights(media_csv, randomize, seed)
+    lab = np.arange(256, dtype=np.uint8).reshape(1, 1, 256)
+    return labels_to_maps(lab, w, organ_table(organs_csv))[0, :, 0, :].T.copy()
+
+
+REF_TO_MATERIAL = os.path.join(HERE, "_ref", "to_material")
+REF_TO_MATERIAL_SHAPE = (100, 271, 543)  # slices, rows, columns: fixed in the reference program
+
+
+def ref_to_material_available():
+    return os.path.exists(REF_TO_MATERIAL)
+
+
+def ref_to_material_run(labels, media_csv, organs_csv, workdir):
+    """Runs the reference converter (built from its source into oracle/_ref) on a label volume of its fixed
+    shape in `workdir`.  Returns (test.raw as int16 [6][100][271][543], the per-slice maps as int16
+    [100][6][271][543], stdout text)."""
+    import shutil
+    import subprocess
+    lab = np.ascontiguousarray(labels, np.uint8)
+    if lab.shape != REF_TO_MATERIAL_SHAPE:
+        raise ValueError(f"the reference program reads exactly {REF_TO_MATERIAL_SHAPE} labels")
+    os.makedirs(os.path.join(workdir, "AM_MD"), exist_ok=True)
+    os.makedirs(os.path.join(workdir, "AM_ED"), exist_ok=True)
+    lab.tofile(os.path.join(workdir, "AM_skin_543x271_700_800.raw"))
+    shutil.copyfile(media_csv, os.path.join(workdir, "AM_media.csv"))
+    shutil.copyfile(organs_csv, os.path.join(workdir, "AM_organs.csv"))
+    r = subprocess.run([REF_TO_MATERIAL], cwd=workdir, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference converter failed: {r.stderr[-500:]}")
+    nz, nrow, ncol = REF_TO_MATERIAL_SHAPE
+    whole = np.fromfile(os.path.join(workdir, "test.raw"), np.int16).reshape(6, nz, nrow, ncol)
+    md = np.empty((nz, 6, nrow, ncol), np.int16)
+    for iz in range(nz):
+        md[iz] = np.fromfile(os.path.join(workdir, "AM_MD", f"AM_MD_{ncol}_{nrow}_{iz:03d}.raw"),
+                             np.int16).reshape(6, nrow, ncol)
+    return whole, md, r.stdout

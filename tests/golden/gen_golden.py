@@ -6,6 +6,7 @@ oracle/Makefile into oracle/_ref/ (run in the build container, where the referen
 Every fixture holds seeded inputs (or the recipe to regenerate them with virtualcbct_amd.phantoms) and the
 reference's outputs for them.
 """
+import hashlib
 import os
 import sys
 import tempfile
@@ -22,6 +23,7 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 REF = "/root/reference"
 P = f"{REF}/Release_CBCT_Projection_CM"
 F = f"{REF}/Release_recon_from_forward_pro_float_CM"
+R = f"{REF}/ICRPphantoms_2mmslice"
 
 
 def small_proj_case():
@@ -40,8 +42,17 @@ def small_proj_case():
                 atten=atten, pdf=pdf, xfactor=1000.0)
 
 
+def material_maps_case(seed=4242):
+    """Label volume for the converter fixture: synthetic_labels(543, 271, 100, seed) with row 0 of slice 0
+    set to 0..255 (then 0)."""
+    lab = phantoms.synthetic_labels(543, 271, 100, seed=seed)
+    lab[0, 0, :] = 0
+    lab[0, 0, :256] = np.arange(256, dtype=np.uint8)
+    return dict(seed=seed, labels=lab)
+
+
 def main():
-    if not O.ref_available():
+    if not O.ref_available() or not O.ref_to_material_available():
         sys.exit("oracle/_ref not built (make -C oracle with the reference tree present)")
 
     # 1. attenuation_coefficient on the shipped spectrum and xraydb tables
@@ -111,6 +122,20 @@ def main():
         out[f"proj{n}_ints"] = d["ints"]
         out[f"proj{n}_dbls"] = d["dbls"]
     np.savez_compressed(f"{OUT}/loaders.npz", **out)
+
+    # 6. the phantom converter (R = ICRPphantoms_2mmslice) on a seeded synthetic label volume of its fixed
+    #    shape; row 0 of slice 0 holds every label 0..255, so the reference's output there is its whole
+    #    label -> map table.  Kept: that table, the printed media weights, and digests of the outputs.
+    md_fixture = material_maps_case()
+    with tempfile.TemporaryDirectory() as td:
+        whole, md, stdout = O.ref_to_material_run(md_fixture["labels"], f"{R}/AM_media.csv", f"{R}/AM_organs.csv", td)
+    lines = stdout.splitlines()
+    i0 = lines.index("AM_media.csv ") + 1
+    media = np.array([[float(v) for v in ln.split()] for ln in lines[i0:i0 + 53]])
+    np.savez_compressed(f"{OUT}/material_maps.npz", seed=md_fixture["seed"], lut=md[0, :, 0, :256].T.copy(),
+                        media_printed=media, md_sha256=np.bytes_(hashlib.sha256(md.tobytes()).hexdigest()),
+                        whole_sha256=np.bytes_(hashlib.sha256(whole.tobytes()).hexdigest()),
+                        md_sums=md.astype(np.int64).sum(axis=(2, 3)))
     print("golden fixtures written to", OUT)
 
 

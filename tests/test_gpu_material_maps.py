@@ -1,0 +1,142 @@
+"""GPU parity of the organ-label -> elemental-map kernels (ICRPphantoms_2mmslice/
+To_material_distribution_for512_random.cpp), bit-exact: labels_to_maps against the oracle and the digest of
+the reference program's own output (fixture), and the fused projector upload (set_volume_labels) against the
+oracle's maps placed on the projector grid, read back through get_volume and through forward projections."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from virtualcbct_amd import material_maps as MM
+from virtualcbct_amd import phantoms
+from virtualcbct_amd._lib import VcbctError
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+ICRP = os.path.join(HERE, "golden", "icrp")
+FIX = os.path.join(HERE, "golden", "material_maps.npz")
+AM = (os.path.join(ICRP, "AM_media.csv"), os.path.join(ICRP, "AM_organs.csv"))
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as T
+    if not T.cuda.is_available():
+        pytest.skip("no GPU")
+    return T
+
+
+@pytest.fixture(scope="module")
+def case():
+    """The fixture's label volume (543 x 271 x 100, every label 0..255 in row 0 of slice 0) and its oracle maps."""
+    fx = np.load(FIX)
+    lab = phantoms.synthetic_labels(543, 271, 100, seed=int(fx["seed"]))
+    lab[0, 0, :] = 0
+    lab[0, 0, :256] = np.arange(256, dtype=np.uint8)
+    lut, w = MM.material_table(*AM)
+    maps = O.labels_to_maps(lab, O.material_weights(AM[0], True, 1), O.organ_table(AM[1]))
+    return dict(fx=fx, labels=lab, lut=lut, maps=maps)
+
+
+def projector(N, Nz, nmat=6, H=16, W=16, n_ang=4):
+    from virtualcbct_amd.projector import Projector
+    ang = np.deg2rad(np.linspace(-170.0001, 150.0, n_ang))
+    cs, sn, xc, yc = O.trig_table_proj(W, H, ang, np.full(n_ang, 0.013), np.zeros(n_ang))
+    return Projector(N, 0.1, Nz, 0.2, H, W, n_ang, cs, sn, xc, yc, num_material=nmat)
+
+
+def place(maps, lut0, N, Nz, col0, row0, slice0, nmat=6):
+    """The oracle maps [nz][6][nrow][ncol] on the projector grid [Nz][nmat][N][N]: voxel (x, y, z) takes label
+    position (x + col0, y + row0, z + slice0); outside the label grid the value of label 0."""
+    nz, _, nrow, ncol = maps.shape
+    out = np.empty((Nz, nmat, N, N), np.int16)
+    out[:] = lut0[:nmat].view(np.int16)[None, :, None, None]
+    xs, ys, zs = np.arange(N) + col0, np.arange(N) + row0, np.arange(Nz) + slice0
+    kx, ky, kz = np.flatnonzero((xs >= 0) & (xs < ncol)), np.flatnonzero((ys >= 0) & (ys < nrow)), \
+        np.flatnonzero((zs >= 0) & (zs < nz))
+    if len(kx) and len(ky) and len(kz):
+        sub = maps[zs[kz]][:, :nmat][:, :, ys[ky]][:, :, :, xs[kx]]
+        out[kz[0]:kz[-1] + 1, :, ky[0]:ky[-1] + 1, kx[0]:kx[-1] + 1] = sub
+    return out
+
+
+def test_labels_to_maps_matches_oracle_and_reference(torch, case):
+    lab = torch.from_numpy(case["labels"]).cuda()
+    out = MM.labels_to_maps(lab, case["lut"])
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert np.array_equal(got, case["maps"])
+    # the digest of the reference program's per-slice files for the same labels
+    assert hashlib.sha256(got.tobytes()).hexdigest() == case["fx"]["md_sha256"].item().decode()
+    # a second, smaller call into the same output shape family (odd sizes, one slice)
+    small = np.ascontiguousarray(case["labels"][3:4, 5:42, 7:100])
+    got2 = MM.labels_to_maps(torch.from_numpy(small).cuda(), case["lut"]).cpu().numpy()
+    assert np.array_equal(got2, case["maps"][3:4, :, 5:42, 7:100])
+
+
+def test_set_volume_labels_centred_512(torch, case):
+    P = projector(512, 100)
+    P.set_volume_labels(torch.from_numpy(case["labels"]).cuda(), case["lut"])
+    got = P.get_volume().cpu().numpy()
+    c0, r0 = MM.centred_placement(543, 271, 512)
+    assert (c0, r0) == (15, -121)
+    assert np.array_equal(got, place(case["maps"], case["lut"][0], 512, 100, c0, r0, 0))
+    P.close()
+
+
+@pytest.mark.parametrize("N,Nz,col0,row0,slice0,nmat", [
+    (300, 120, -40, 30, 7, 6),     # pad left, crop top, slices past the phantom's end
+    (97, 33, 400, -5, -3, 6),      # odd sizes, window at the right edge, slices before the start
+    (128, 40, 1000, 0, 0, 6),      # entirely outside: every voxel takes label 0
+    (256, 64, 100, 10, 20, 4),     # four materials: the last two are dropped
+])
+def test_set_volume_labels_placement(torch, case, N, Nz, col0, row0, slice0, nmat):
+    lut = case["lut"].copy()
+    lut[0, :6] = [11, 22, 33, 44, 55, 66]  # outside the phantom must read label 0's entry
+    P = projector(N, Nz, nmat)
+    P.set_volume_labels(torch.from_numpy(case["labels"]).cuda(), lut, col0, row0, slice0)
+    got = P.get_volume().cpu().numpy()
+    maps = case["maps"].copy()  # label 0 matches no organ row: zero there before the custom entry
+    maps += (case["labels"] == 0)[:, None] * lut[0, :6].astype(np.int16)[None, :, None, None]
+    assert np.array_equal(got, place(maps, lut[0], N, Nz, col0, row0, slice0, nmat))
+    P.close()
+
+
+def test_set_volume_labels_forward_equals_u16_upload(torch, case):
+    """The fused upload and the reference-layout upload give the same device volume, so the same projections."""
+    d = np.load(os.path.join(HERE, "golden", "reproj_small.npz"))
+    N, Nz = 256, 64
+    vol = place(case["maps"], case["lut"][0], N, Nz, 140, -10, 18).view(np.uint16)
+    outs = []
+    for how in ("labels", "u16"):
+        P = projector(N, Nz, H=48, W=64, n_ang=6)
+        if how == "labels":
+            P.set_volume_labels(torch.from_numpy(case["labels"]).cuda(), case["lut"], 140, -10, 18)
+        else:
+            P.set_volume(vol)
+        P.set_spectrum(d["atten"], d["pdf"])
+        outs.append(P.forward(1000.0).cpu().numpy())
+        P.close()
+    assert np.array_equal(outs[0], outs[1]) and outs[0].min() < outs[0].max()
+
+
+def test_get_volume_round_trip_and_state(torch):
+    rng = np.random.default_rng(3)
+    vol = rng.integers(0, 65535, (9, 5, 70, 70)).astype(np.uint16)
+    P = projector(70, 9, nmat=5)
+    with pytest.raises(VcbctError, match="no u16 volume"):
+        P.get_volume()
+    P.set_volume(vol)
+    assert np.array_equal(P.get_volume().cpu().numpy().view(np.uint16), vol)
+    P.close()
+
+
+def test_labels_argument_checks(torch, case):
+    with pytest.raises(ValueError):
+        MM.labels_to_maps(torch.zeros((2, 3, 4), dtype=torch.int16, device="cuda"), case["lut"])
+    with pytest.raises(ValueError):
+        MM.labels_to_maps(torch.zeros((2, 3, 4), dtype=torch.uint8), case["lut"])
+    with pytest.raises(ValueError):
+        MM.device_lut(np.zeros(100, np.uint16))

@@ -118,6 +118,53 @@ def ellipsoid_phantom_torch(N, Nz, scale_cm, step_cm, seed, n_ell=40, n_labels=8
     return vol.permute(0, 3, 1, 2).contiguous().to(torch.int16)
 
 
+def synthetic_labels(ncol=543, nrow=271, nz=100, seed=0, labels=None, n_organs=60):
+    """Organ-label volume uint8 [nz][nrow][ncol] in the layout of the ICRP label slices the reference's
+    converter reads (ICRPphantoms_2mmslice/To_material_distribution_for512_random.cpp:38, 58-61; the
+    voxel files themselves are not distributed).  Seeded: label 0 outside an elliptic body cylinder, the
+    body filled with labels[0], then `n_organs` random ellipsoids (clipped to the body) with labels drawn
+    from `labels` (default 1..141, the organ numbers of the ICRP tables)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    labels = np.arange(1, 142) if labels is None else np.asarray(labels)
+    out = np.zeros((nz, nrow, ncol), np.uint8)
+    yy, xx = np.meshgrid(np.arange(nrow) + 0.5, np.arange(ncol) + 0.5, indexing="ij")
+    ax, ay = 0.46 * ncol, 0.46 * nrow
+    body = ((xx - ncol / 2) / ax) ** 2 + ((yy - nrow / 2) / ay) ** 2 <= 1.0
+    out[:, body] = labels[0]
+    for _ in range(n_organs):
+        c = rng.uniform([0.2 * ncol, 0.2 * nrow, 0.0], [0.8 * ncol, 0.8 * nrow, nz])
+        r = rng.uniform([0.03 * ncol, 0.03 * nrow, 2.0], [0.15 * ncol, 0.2 * nrow, 0.4 * nz])
+        lab = labels[rng.integers(0, len(labels))]
+        z0, z1 = max(0, int(c[2] - r[2])), min(nz, int(c[2] + r[2]) + 1)
+        for z in range(z0, z1):
+            w = 1.0 - ((z + 0.5 - c[2]) / r[2]) ** 2
+            if w <= 0:
+                continue
+            m = (((xx - c[0]) / r[0]) ** 2 + ((yy - c[1]) / r[1]) ** 2 <= w) & body
+            out[z][m] = lab
+    return out
+
+
+def write_material_tables(dirpath, seed=0, n_media=53, n_organs=140):
+    """Seeded media / organ tables in the ICRP CSV formats the converter reads (media rows
+    "ID,H,C,N,O,Na,Mg,P,S,Cl,K,Ca,Fe,I" weight %, organ rows "organ,media,density"); medium n_media is air
+    (N 70 / O 30, never redrawn).  Returns (media_csv, organs_csv)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    os.makedirs(dirpath, exist_ok=True)
+    media, organs = os.path.join(dirpath, "media.csv"), os.path.join(dirpath, "organs.csv")
+    with open(media, "w") as f:
+        for i in range(1, n_media):
+            w = rng.dirichlet(np.ones(13) * 0.5) * 100.0
+            f.write(",".join([str(i)] + [f"{v:.1f}" for v in w]) + "\n")
+        f.write(f"{n_media},0,0,70,30,0,0,0,0,0,0,0,0,0\n")
+    with open(organs, "w") as f:
+        for o in range(1, n_organs + 1):
+            med = n_media if o == n_organs else int(rng.integers(1, n_media))
+            dens = 0.001 if med == n_media else float(rng.uniform(0.9, 2.0))
+            f.write(f"{o},{med},{dens:.2f}\n")
+    return media, organs
+
+
 def write_slices(vol, prefix):
     """Writes <prefix>_%03d.raw, one [mat][y][x] uint16 slice per file (P/projectionData.cpp:197)."""
     os.makedirs(os.path.dirname(prefix) or ".", exist_ok=True)

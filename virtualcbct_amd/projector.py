@@ -100,6 +100,31 @@ class Projector:
             raise ValueError("device volume size does not match the geometry")
         check(lib().vcbct_projector_set_volume_u16(self.h, C.c_void_p(vol.data_ptr()), 1, _stream(stream)))
 
+    def set_volume_labels(self, labels, lut, col0=None, row0=None, slice0=0, stream=None):
+        """Organ-label phantom straight into the device volume: labels uint8 [nz][nrow][ncol] (device), lut
+        [256][8] as material_maps.material_table returns it.  Voxel (x, y, z) takes the label at column
+        x + col0, row y + row0, slice z + slice0 (label 0 outside the phantom); col0/row0 default to the
+        centring offsets."""
+        from . import material_maps
+        nz, nrow, ncol = material_maps._check_labels(labels)
+        c0, r0 = material_maps.centred_placement(ncol, nrow, self.g.recon_size)
+        col0 = c0 if col0 is None else int(col0)
+        row0 = r0 if row0 is None else int(row0)
+        dl = material_maps.device_lut(lut, self.device)
+        check(lib().vcbct_projector_set_volume_labels(self.h, C.c_void_p(labels.data_ptr()), ncol, nrow, nz, col0,
+                                                      row0, int(slice0), C.c_void_p(dl.data_ptr()),
+                                                      _stream(stream)))
+
+    def get_volume(self, stream=None):
+        """The u16 device volume in the reference layout [z][mat][y][x] (int16 device tensor of the uint16
+        values): the projector-ready slices, e.g. of a label phantom."""
+        torch = _torch()
+        n = self.g
+        out = torch.empty((n.recon_slices, n.num_material, n.recon_size, n.recon_size), dtype=torch.int16,
+                          device=f"cuda:{self.device}")
+        check(lib().vcbct_projector_get_volume_u16(self.h, C.c_void_p(out.data_ptr()), _stream(stream)))
+        return out
+
     def set_spectrum(self, Attenuation, pdf_pe):
         """Attenuation [mat][NE] (or flat mat*NE) cm^2/g and pdf_pe [NE], as attenuation_coefficient writes."""
         pdf = np.ascontiguousarray(pdf_pe, np.float32).ravel()
