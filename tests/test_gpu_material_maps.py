@@ -91,6 +91,7 @@ def test_set_volume_labels_centred_512(torch, case):
     (97, 33, 400, -5, -3, 6),      # odd sizes, window at the right edge, slices before the start
     (128, 40, 1000, 0, 0, 6),      # entirely outside: every voxel takes label 0
     (256, 64, 100, 10, 20, 4),     # four materials: the last two are dropped
+    (40, 1100, 5, -3, -500, 6),    # more slices than one LDS chunk (1024); the phantom in the middle
 ])
 def test_set_volume_labels_placement(torch, case, N, Nz, col0, row0, slice0, nmat):
     lut = case["lut"].copy()

@@ -175,7 +175,9 @@ namespace vcbct {
 // ------------------------------------------------------------------------------------------------
 // Device: lookups.
 
-// labels [nz][n2] -> maps [nz][6][n2]; one voxel per thread, the table in LDS.
+// labels [nz][n2] -> maps [nz][6][n2]; one voxel per thread per iteration, the table in LDS.  Grid (bx, nz):
+// a few thousand blocks in all, each striding over its slice, so that the table is loaded once per
+// several thousand voxels.
 __global__ __launch_bounds__(256) void k_labels_to_maps(const uint8_t* __restrict__ labels,
                                                         const uint4* __restrict__ lut, uint16_t* __restrict__ maps,
                                                         int n2) {
@@ -197,15 +199,26 @@ __global__ __launch_bounds__(256) void k_labels_to_maps(const uint8_t* __restric
 }
 
 // labels [nz][nrow][ncol] -> projector voxels [N][N][Nz] (y, x, z; 16 bytes each).  A block covers one y
-// row, 32 x and 32 z: the labels of the tile are read along x (one slice row at a time) into LDS, then
-// written out as 32 z-runs of 16-byte voxels.  Grid: (ceil(N/32), ceil(Nz/32), N).
+// row and LX = 32 x: the labels of up to LZC slices are read along x (one slice row at a time) into LDS,
+// then the block writes its output, which for fixed y and consecutive x is one contiguous range of
+// memory, linearly (whole cache lines, 4 KB per block-wide store).  Grid: (ceil(N/LX), N).
 constexpr int LT = 32;
+constexpr int LX = 32;
+constexpr int LZC = 1024;
+
+// LDS row length of one x for z chunks of up to zmax slices: a multiple of 4 bytes whose word count is
+// odd, so the 32 x of one z fall in distinct banks.
+static inline int label_pitch(int zmax) {
+    int p = (zmax + 3) / 4 * 4;
+    if ((p / 4) % 2 == 0) p += 4;
+    return p;
+}
 
 __global__ __launch_bounds__(256) void k_labels_to_vox(const uint8_t* __restrict__ labels, int ncol, int nrow, int nz,
                                                        int col0, int row0, int slice0, const uint4* __restrict__ lut,
-                                                       uint4 keep, uint4* __restrict__ vox, int N, int Nz) {
+                                                       uint4 keep, uint4* __restrict__ vox, int N, int Nz, int pitch) {
     __shared__ uint4 tab[256];
-    __shared__ uint8_t tile[LT][LT + 4];
+    extern __shared__ uint8_t tile[];  // [LX][pitch]
     {
         uint4 e = lut[threadIdx.x];
         e.x &= keep.x;
@@ -214,24 +227,30 @@ __global__ __launch_bounds__(256) void k_labels_to_vox(const uint8_t* __restrict
         e.w &= keep.w;
         tab[threadIdx.x] = e;
     }
-    const int x0 = blockIdx.x * LT, z0 = blockIdx.y * LT, y = blockIdx.z;
+    const int x0 = blockIdx.x * LX, y = blockIdx.y;
+    const int nx = min(LX, N - x0);
     const int row = y + row0;
     const bool row_in = row >= 0 && row < nrow;
-    const int tx = threadIdx.x % LT, ty = threadIdx.x / LT;
-    const int col = x0 + tx + col0;
-    const bool col_in = row_in && col >= 0 && col < ncol;
-    for (int zz = ty; zz < LT; zz += 256 / LT) {
-        const int sl = z0 + zz + slice0;
-        uint8_t v = 0;
-        if (col_in && sl >= 0 && sl < nz) v = labels[((size_t)sl * nrow + row) * ncol + col];
-        tile[zz][tx] = v;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < LT * LT; i += 256) {
-        const int xx = i / LT, zz = i % LT;
-        const int x = x0 + xx, z = z0 + zz;
-        if (x >= N || z >= Nz) continue;
-        vox[((size_t)y * N + x) * Nz + z] = tab[tile[zz][xx]];
+    const size_t n2 = (size_t)nrow * ncol;
+    uint4* out = vox + ((size_t)y * N + x0) * Nz;
+    for (int zc = 0; zc < Nz; zc += LZC) {
+        const int nzc = min(LZC, Nz - zc);
+        for (int j = threadIdx.x; j < nzc * LX; j += 256) {
+            const int zz = j / LX, xx = j % LX;
+            const int sl = zc + zz + slice0, col = x0 + xx + col0;
+            uint8_t v = 0;
+            if (row_in && col >= 0 && col < ncol && sl >= 0 && sl < nz) v = labels[(size_t)sl * n2 + (size_t)row * ncol + col];
+            tile[xx * pitch + zz] = v;
+        }
+        __syncthreads();
+        // j -> (x, z) with z fastest; the FP32 quotient is exact for j < 32 * nzc and nzc <= 1024
+        const float inv = 1.0f / (float)nzc;
+        for (int j = threadIdx.x; j < nx * nzc; j += 256) {
+            const int xx = (int)(((float)j + 0.5f) * inv);
+            const int zz = j - xx * nzc;
+            out[(size_t)xx * Nz + zc + zz] = tab[tile[xx * pitch + zz]];
+        }
+        __syncthreads();
     }
 }
 
@@ -270,7 +289,7 @@ extern "C" int vcbct_labels_to_maps(const uint8_t* labels, int ncol, int nrow, i
     if (ncol < 1 || nrow < 1 || nz < 1 || nz > 65535 || (double)ncol * nrow >= 2147483648.0 - 65536.0 * 256)
         return fail(VCBCT_EINVAL, "labels_to_maps: label grid %d x %d x %d out of range", ncol, nrow, nz);
     const int n2 = ncol * nrow;
-    const unsigned bx = (unsigned)std::min((n2 + 255) / 256, 1024);
+    const unsigned bx = (unsigned)std::min((n2 + 255) / 256, std::max(1, (4096 + nz - 1) / nz));
     hipLaunchKernelGGL(k_labels_to_maps, dim3(bx, nz), dim3(256), 0, (hipStream_t)stream, labels, (const uint4*)lut,
                        maps, n2);
     VCBCT_HIP(hipGetLastError());
@@ -291,9 +310,10 @@ extern "C" int vcbct_projector_set_volume_labels(vcbct_projector* p, const uint8
     uint32_t k[4] = {0, 0, 0, 0};
     for (int m = 0; m < g.nmat; ++m) k[m / 2] |= 0xffffu << (16 * (m % 2));
     const uint4 keep = make_uint4(k[0], k[1], k[2], k[3]);
-    dim3 grid((g.N + LT - 1) / LT, (g.Nz + LT - 1) / LT, g.N);
-    hipLaunchKernelGGL(k_labels_to_vox, grid, dim3(256), 0, (hipStream_t)stream, labels, ncol, nrow, nz, col0, row0,
-                       slice0, (const uint4*)lut, keep, (uint4*)p->d_vox_u16, g.N, g.Nz);
+    dim3 grid((g.N + LX - 1) / LX, g.N);
+    const int pitch = label_pitch(std::min(g.Nz, LZC));
+    hipLaunchKernelGGL(k_labels_to_vox, grid, dim3(256), (size_t)LX * pitch, (hipStream_t)stream, labels, ncol, nrow,
+                       nz, col0, row0, slice0, (const uint4*)lut, keep, (uint4*)p->d_vox_u16, g.N, g.Nz, pitch);
     VCBCT_HIP(hipGetLastError());
     p->have_u16 = 1;
     return 0;
