@@ -341,6 +341,58 @@ def bench_phantom(line, td, dev, stream, e):
     P.close()
 
 
+def bench_iterative(line, dev, stream, e):
+    """Iterative-reconstruction pieces on a 256 x 256 x 64 image (1 mm voxels), 360 angles, 400 x 400 detector:
+    lengthproj (voxel-driven chord backprojection, FP64 geometry per voxel and angle) and one IR_ImageCBCT
+    update.  CPU beside it: the reference's own lengthproj (compiled from its sources, OpenMP over slices) on
+    16 slices x 36 angles of the same geometry."""
+    import torch
+    from oracle import oracle as O
+    from virtualcbct_amd import io, iterative, phantoms
+    from virtualcbct_amd.projector import Projector
+    N, Nz, H, W, nA = 256, 64, 400, 400, 360
+    ang, xs, ys = phantoms.flexmap_angles(nA)
+    c, s_, x, y = io.trig_table(W, H, ang, xs, ys)
+    P = Projector(N, 0.1, Nz, 0.1, H, W, nA, c, s_, x, y)
+    rng = np.random.default_rng(0)
+    proj_h = rng.integers(20000, 60000, (nA, H, W)).astype(np.uint16)
+    proj = torch.from_numpy(proj_h.view(np.int16)).to(dev)
+    npf = iterative.lengthproj(P, proj)
+    before = torch.from_numpy(rng.integers(0, 3000, (Nz, N, N)).astype(np.int16)).to(dev)
+    npf_re = npf * 1.01
+    out = iterative.IR_ImageCBCT(before, npf, npf_re, 0.1, 0.3, 0.05)
+    torch.cuda.synchronize()
+    reps = 3
+    e[0].record(stream)
+    for _ in range(reps):
+        iterative.lengthproj(P, proj, out=npf)
+    e[1].record(stream)
+    for _ in range(reps):
+        iterative.IR_ImageCBCT(before, npf, npf_re, 0.1, 0.3, 0.05, out=out)
+    e[2].record(stream)
+    torch.cuda.synchronize()
+    lp_ms = e[0].elapsed_time(e[1]) / reps
+    ir_ms = e[1].elapsed_time(e[2]) / reps
+    pairs = N * N * Nz * nA
+    res = {"workload": f"image {N}x{N}x{Nz} (1 mm), {nA} angles, {W}x{H} detector",
+           "lengthproj_ms": lp_ms, "lengthproj_voxel_angles_per_s": pairs / (lp_ms * 1e-3),
+           "ir_update_ms": ir_ms, "ir_update_voxels_per_s": N * N * Nz / (ir_ms * 1e-3),
+           "ir_update_gb_s": (N * N * Nz * (2 + 4 + 4 + 4)) / (ir_ms * 1e-3) / 1e9}
+    if O.ref_available():
+        nz_c, na_c = 16, 36
+        sel = np.arange(0, nA, nA // na_c)[:na_c]
+        g = O.Geometry(N=N, s=0.1, Nz=nz_c, t=0.1, H=H, W=W, cos_t=c[sel], sin_t=s_[sel], xc=x[sel], yc=y[sel])
+        t0 = time.perf_counter()
+        O.ref_lengthproj(g, proj_h[sel], ang[sel])
+        sec = time.perf_counter() - t0
+        res["cpu_reference"] = {"voxel_angles_per_s": N * N * nz_c * na_c / sec, "cores": min(nz_c, os.cpu_count()),
+                                "sample": f"P/lengthproj.cpp on {N}x{N}x{nz_c} x {na_c} angles, {sec:.2f} s"}
+    line["iterative"] = res
+    log(f"iterative: lengthproj {lp_ms:.2f} ms ({res['lengthproj_voxel_angles_per_s']:.3e} voxel-angles/s), "
+        f"IR update {ir_ms:.3f} ms")
+    P.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -348,7 +400,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline forward only (used under the profiler)")
-    ap.add_argument("--only", choices=["adjoint", "batched", "decompose", "fdk", "phantom"], default=None,
+    ap.add_argument("--only", choices=["adjoint", "batched", "decompose", "fdk", "phantom", "iterative"], default=None,
                     help="run one secondary section alone and print its JSON (used under the profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (scale for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -401,6 +453,8 @@ def main():
             bench_decompose(line, td, dev, stream, e)
         elif args.only == "phantom":
             bench_phantom(line, td, dev, stream, e)
+        elif args.only == "iterative":
+            bench_iterative(line, dev, stream, e)
         else:
             bench_batched(line, inp, dev, stream, e, NE)
         print(json.dumps(line), flush=True)
@@ -453,6 +507,7 @@ def main():
         bench_batched(line, inp, dev, stream, e, NE)
         bench_decompose(line, td, dev, stream, e)
         bench_phantom(line, td, dev, stream, e)
+        bench_iterative(line, dev, stream, e)
 
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

@@ -140,6 +140,25 @@ int vcbct_projector_set_volume_labels(vcbct_projector* p, const uint8_t* dev_lab
 int vcbct_projector_get_volume_u16(vcbct_projector* p, uint16_t* dev_vol, void* stream);
 
 /* ------------------------------------------------------------------------------------------------
+ * Iterative-reconstruction pieces of the projector tree (P/main.cpp includes them but no longer calls them).
+ *
+ * Replaces lengthproj (P/lengthproj.cpp:9-131): dev_npf [Nz][N][N] FP32 (overwritten) receives, per voxel,
+ * the sum over angles [a0, a1) (a1 = -1: all) of (float)((65535 - q) * chord), where chord is the voxel's
+ * length along the major axis of the source -> voxel-centre line and q the bilinear sample of dev_proj
+ * [n_angles][H][W] u16 where that line meets the detector (the reference's tap order and weights, its
+ * x_reg in place of y_reg in the u coordinate, and its registration reset for ite != 0).  Angles whose
+ * detector geometry divides by zero (sin = 0) contribute nothing, as on the reference's x86 build. */
+int vcbct_projector_lengthproj(vcbct_projector* p, int ite, const uint16_t* dev_proj, int a0, int a1, float* dev_npf,
+                               void* stream);
+/* Replaces IR_ImageCBCT (P/IR_ImageCBCT.cpp:19-94), one update of a [Nz][N][N] int16 image:
+ * out = before (wbefore + (1 - wbefore) npf_re / (npf + lambda)), lambda = weight_tv TV'(before) + 1e-6
+ * (the 13-neighbour TV gradient of :9-16, interior voxels only; the prior term is 0 as in the reference),
+ * clamped (< 0 -> 0, > 32767 -> 5000) and zeroed where the in-plane radius in cm reaches N/2. */
+int vcbct_ir_update(int recon_size, int recon_slices, double recon_scale, const int16_t* dev_before,
+                    const float* dev_npf, const float* dev_npf_re, double wbefore, double weight_tv,
+                    double weight_prior, float* dev_out, void* stream);
+
+/* ------------------------------------------------------------------------------------------------
  * Differentiable form (no reference counterpart; the reference's scatter U-Net, Unet_scatter.py, is the
  * intended consumer).  Log projection p = -log( sum_e pdf_e exp(-sum_m mu_me L_m/xf) / sum_e pdf_e ).
  */

@@ -894,3 +894,146 @@ int orc_labels_to_maps(const uint8_t* labels, int ncol, int nrow, int nz, const 
 }
 
 }  // extern "C"
+
+/* ------------------------------------------------------------------------------------------------
+ * Iterative-reconstruction pieces of the projector tree (not called by its main program).
+ *
+ * The (int) casts of the reference run on x86, where a NaN or out-of-range double converts to INT_MIN;
+ * every later test on such an index fails, so here an invalid conversion simply drops the term. */
+static inline bool x86_trunc(double v, int* out) {
+    if (!(v > -2147483649.0 && v < 2147483648.0)) return false;
+    *out = (int)v;
+    return true;
+}
+
+extern "C" {
+
+int orc_lengthproj(const orc_geom* gp, int ite, const double* cos_t, const double* sin_t, const double* xc,
+                   const double* yc, const uint16_t* proj, int a0, int a1, float* npf) {
+    /* P/lengthproj.cpp:9-131: for every voxel centre r and angle, the chord of the voxel along the major
+     * axis of the source-voxel line times (65535 - the projection sampled where that line meets the detector),
+     * accumulated in FP32 over the angles in order. */
+    const orc_geom& g = *gp;
+    double x_reg = g.x_reg, y_reg = g.y_reg, z_reg = g.z_reg;
+    if (ite != 0) { x_reg = 0.0; y_reg = 0.0; z_reg = 0.0; }
+    const int N = g.N, Nz = g.Nz, W = g.W, H = g.H;
+    const double s = g.s, t = g.t;
+    for (size_t i = 0; i < (size_t)N * N * Nz; i++) npf[i] = 0.0f;
+#pragma omp parallel for schedule(static)
+    for (int jz = 0; jz < Nz; jz++) {
+        const double rz = Nz * t * 0.5 - (jz + 0.5) * t;
+        for (int jy = 0; jy < N; jy++) {
+            const double ry = N * s * 0.5 - (jy + 0.5) * s;
+            for (int jx = 0; jx < N; jx++) {
+                const double rx = -N * s * 0.5 + (jx + 0.5) * s;
+                float acc = 0.0f;
+                for (int a = a0; a < a1; a++) {
+                    const double sx = -g.sad * sin_t[a] + x_reg;
+                    const double sy = g.sad * cos_t[a] + y_reg;
+                    const double sz = z_reg;
+                    double rr;
+                    if (fabs(rx - sx) > fabs(ry - sy)) {
+                        const double x0 = rx - 0.5 * s, x1 = rx + 0.5 * s;
+                        const double y0 = (x0 - sx) / (rx - sx) * (ry - sy) + sy;
+                        const double z0 = (x0 - sx) / (rx - sx) * (rz - sz) + sz;
+                        const double y1 = (x1 - sx) / (rx - sx) * (ry - sy) + sy;
+                        const double z1 = (x1 - sx) / (rx - sx) * (rz - sz) + sz;
+                        rr = sqrt((x0 - x1) * (x0 - x1) + (y0 - y1) * (y0 - y1) + (z0 - z1) * (z0 - z1));
+                    } else {
+                        const double y0 = ry - 0.5 * s, y1 = ry + 0.5 * s;
+                        const double x0 = (y0 - sy) / (ry - sy) * (rx - sx) + sx;
+                        const double z0 = (y0 - sy) / (ry - sy) * (rz - sz) + sz;
+                        const double x1 = (y1 - sy) / (ry - sy) * (rx - sx) + sx;
+                        const double z1 = (y1 - sy) / (ry - sy) * (rz - sz) + sz;
+                        rr = sqrt((x0 - x1) * (x0 - x1) + (y0 - y1) * (y0 - y1) + (z0 - z1) * (z0 - z1));
+                    }
+                    /* detector line through o, perpendicular to the central ray; p = its meet with s->r */
+                    const double ox = sx + g.det_pitch / g.det_pitch_iso * sin_t[a] * g.sad;
+                    const double oy = sy - g.det_pitch / g.det_pitch_iso * cos_t[a] * g.sad;
+                    const double kr = (sx - rx) / (sy - ry), ko = (oy - sy) / (ox - sx);
+                    const double py = (ox - rx + kr * ry + ko * oy) / (kr + ko);
+                    const double px = (kr * (py - ry) + rx);
+                    const double pz = ((sz - rz) / (sy - ry) * (py - ry) + rz);
+                    const double pix = g.det_pitch * (g.det_pixel_num / H);
+                    const double xshift = -(xc[a] + H / 2.0) * pix;
+                    const double yshift = -(yc[a] + H / 2.0) * pix;
+                    const double vv = -(pz - z_reg - yshift);
+                    /* the reference subtracts x_reg from py too */
+                    const double uu = (px - x_reg) * cos_t[a] + (py - x_reg) * sin_t[a] + xshift - (W - H) / 2 * pix;
+                    const double vfac = vv < 0 ? -0.5 : 0.5, ufac = uu < 0 ? -0.5 : 0.5;
+                    int v1, u1;
+                    if (!x86_trunc(vv / pix + vfac, &v1) || !x86_trunc(uu / pix + ufac, &u1)) continue;
+                    const double dv1 = vv / pix - v1, du1 = uu / pix - u1;
+                    /* the second tap steps toward the sign of (coordinate in cm - index) */
+                    const int v2 = fabs(vv - v1) < 0.000000001 ? v1 : v1 + ((vv - v1) > 0 ? 1 : -1);
+                    const int u2 = fabs(uu - u1) < 0.000000001 ? u1 : u1 + ((uu - u1) > 0 ? 1 : -1);
+                    const int h2 = H / 2, w2 = W / 2;
+                    if (v1 + h2 < 0 || v1 + h2 >= H || v2 + h2 < 0 || v2 + h2 >= H || u1 + w2 < 0 || u1 + w2 >= W ||
+                        u2 + w2 < 0 || u2 + w2 >= W)
+                        continue;
+                    const uint16_t* P = proj + (size_t)a * W * H;
+                    const double p11 = P[(v1 + h2) * W + u1 + w2], p12 = P[(v2 + h2) * W + u1 + w2];
+                    const double p21 = P[(v1 + h2) * W + u2 + w2], p22 = P[(v2 + h2) * W + u2 + w2];
+                    /* as written: the v fraction blends the u taps and the u fraction the v taps */
+                    const double q1 = ((1 - dv1) * p11 + dv1 * p21);
+                    const double q2 = ((1 - dv1) * p12 + dv1 * p22);
+                    const double q = ((1 - du1) * q1 + du1 * q2);
+                    acc += (float)((65535.0 - q) * rr);
+                }
+                npf[(size_t)jz * N * N + (size_t)jy * N + jx] = acc;
+            }
+        }
+    }
+    return 0;
+}
+
+/* The TV term's sums of squared differences are int arithmetic in the reference; on its x86 build they wrap
+ * modulo 2^32 when they pass INT_MAX (a negative sum, then a NaN square root).  The wrap is made explicit. */
+static inline int sq3_wrapped(int d1, int d2, int d3) {
+    const uint32_t u = (uint32_t)d1 * (uint32_t)d1 + (uint32_t)d2 * (uint32_t)d2 + (uint32_t)d3 * (uint32_t)d3;
+    return (int)u;
+}
+
+static double tv_term(int a, int ax, int ay, int az, int ax1, int ay1, int az1, int ayx1, int azx1, int axy1, int azy1,
+                      int axz1, int ayz1) {
+    /* P/IR_ImageCBCT.cpp:9-16 */
+    const double d1 = sqrt(double(sq3_wrapped(a - ax, a - ay, a - az))) + 0.00000001;
+    const double d2 = sqrt(double(sq3_wrapped(a - ax1, ax1 - ayx1, ax1 - azx1))) + 0.00000001;
+    const double d3 = sqrt(double(sq3_wrapped(ay1 - axy1, a - ay1, ay1 - azy1))) + 0.00000001;
+    const double d4 = sqrt(double(sq3_wrapped(az1 - axz1, az1 - ayz1, a - az1))) + 0.00000001;
+    return ((a - ax) + (a - ay) + (a - az)) / d1 + (a - ax1) / d2 + (a - ay1) / d3 + (a - az1) / d4;
+}
+
+int orc_ir_update(int N, int Nz, double scale, const int16_t* before, const float* npf, const float* npf_re,
+                  double wbefore, double weight_tv, double weight_prior, float* out) {
+    /* P/IR_ImageCBCT.cpp:19-94: x' = x (w + (1 - w) npf_re / (npf + lambda)), lambda = weight_tv TV'(x) + 1e-6
+     * (the prior term is 0), then clamped (< 0 -> 0, > 32767 -> 5000) and zeroed outside r >= N/2 (in cm). */
+    const double rmax = N * 0.5 * 1.0;
+    const size_t NN = (size_t)N * N;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < Nz; z++)
+        for (int y = 0; y < N; y++)
+            for (int x = 0; x < N; x++) {
+                const size_t k = (size_t)z * NN + (size_t)y * N + x;
+                double tv = 0.0, prior = 0.0;
+                if (z < Nz - 1 && y < N - 1 && x < N - 1 && x > 0 && y > 0 && z > 0) {
+                    auto at = [&](int dz, int dy, int dx) {
+                        return (int)before[(long long)k + (long long)dz * (long long)NN + (long long)dy * N + dx];
+                    };
+                    tv = tv_term(at(0, 0, 0), at(0, 0, 1), at(0, 1, 0), at(1, 0, 0), at(0, 0, -1), at(0, -1, 0),
+                                 at(-1, 0, 0), at(0, 1, -1), at(1, 0, -1), at(0, -1, 1), at(1, -1, 0), at(-1, 0, 1),
+                                 at(-1, 1, 0));
+                }
+                const double lambda = weight_tv * tv + weight_prior * prior + 0.000001;
+                float v = (float)(before[k] * (wbefore + (1 - wbefore) * npf_re[k] / (npf[k] + lambda)));
+                if (v < 0.0) v = 0.0f;
+                else if (v > 65535 / 2) v = 5000.0f;
+                const double xx = -N * scale * 0.5 + (x + 0.5) * scale;
+                const double yy = N * scale * 0.5 - (y + 0.5) * scale;
+                if (sqrt(xx * xx + yy * yy) >= rmax) v = 0.0f;
+                out[k] = v;
+            }
+    return 0;
+}
+
+}  // extern "C"

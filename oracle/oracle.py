@@ -453,3 +453,52 @@ def ref_to_material_run(labels, media_csv, organs_csv, workdir):
         md[iz] = np.fromfile(os.path.join(workdir, "AM_MD", f"AM_MD_{ncol}_{nrow}_{iz:03d}.raw"),
                              np.int16).reshape(6, nrow, ncol)
     return whole, md, r.stdout
+
+
+# --------------------------------------------------------------------------------------------------
+# Iterative-reconstruction pieces (P/lengthproj.cpp, P/IR_ImageCBCT.cpp).
+
+def lengthproj(g: Geometry, proj_u16, ite=0, a0=0, a1=None):
+    """npf [Nz][N][N] FP32: chord-weighted voxel-driven backprojection of (65535 - proj)."""
+    p = np.ascontiguousarray(proj_u16, np.uint16)
+    a1 = g.n_ang if a1 is None else a1
+    cs, sn, xc, yc = g.tables()
+    out = np.empty((g.Nz, g.N, g.N), np.float32)
+    lib().orc_lengthproj(C.byref(g.c()), int(ite), _p(cs, _D), _p(sn, _D), _p(xc, _D), _p(yc, _D), _p(p, _U16),
+                         int(a0), int(a1), _p(out, _F))
+    return out
+
+
+def ir_update(before_i16, npf, npf_re, scale, wbefore, weight_tv, weight_prior=0.0):
+    b = np.ascontiguousarray(before_i16, np.int16)
+    Nz, N, _ = b.shape
+    out = np.empty(b.shape, np.float32)
+    lib().orc_ir_update(N, Nz, C.c_double(scale), b.ctypes.data_as(C.POINTER(C.c_int16)),
+                        _p(np.ascontiguousarray(npf, np.float32), _F), _p(np.ascontiguousarray(npf_re, np.float32), _F),
+                        C.c_double(wbefore), C.c_double(weight_tv), C.c_double(weight_prior), _p(out, _F))
+    return out
+
+
+def ref_lengthproj(g: Geometry, proj_u16, angles, ite=0, a0=0, a1=None):
+    """The reference's lengthproj (built from its sources into oracle/_ref)."""
+    p = np.ascontiguousarray(proj_u16, np.uint16)
+    a1 = g.n_ang if a1 is None else a1
+    cs, sn, xc, yc = g.tables()
+    ang = np.ascontiguousarray(angles, np.float64)
+    out = np.empty((g.Nz, g.N, g.N), np.float32)
+    d = C.c_double
+    ref("proj").ref_lengthproj(int(ite), g.N, g.Nz, _p(p, _U16), _p(out, _F), d(g.s), d(g.t), g.W, g.H, int(a0),
+                               int(a1), _p(ang, _D), _p(cs, _D), _p(sn, _D), _p(xc, _D), _p(yc, _D), d(g.det_pitch),
+                               d(g.det_pitch_iso), d(g.sad), d(g.det_pixel_num), d(g.x_reg), d(g.y_reg), d(g.z_reg))
+    return out
+
+
+def ref_ir_update(before_i16, npf, npf_re, scale, wbefore, weight_tv, weight_prior=0.0):
+    b = np.ascontiguousarray(before_i16, np.int16)
+    Nz, N, _ = b.shape
+    out = np.empty(b.shape, np.float32)
+    d = C.c_double
+    ref("proj").ref_ir_update(N, Nz, d(scale), b.ctypes.data_as(C.POINTER(C.c_int16)), _p(out, _F),
+                              _p(np.ascontiguousarray(npf, np.float32), _F),
+                              _p(np.ascontiguousarray(npf_re, np.float32), _F), d(wbefore), d(weight_tv), d(weight_prior))
+    return out

@@ -7,6 +7,8 @@
  *   attenuation_coefficient  P/prior_weight_production.cpp:200-523
  *   prepareTrigonometricTable P/misc.cpp:9-37
  *   loadData                 P/projectionData.cpp:63-237
+ *   lengthproj               P/lengthproj.cpp:9-131
+ *   IR_ImageCBCT             P/IR_ImageCBCT.cpp:19-94
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -21,6 +23,8 @@
 #include "projectionData.h"
 #include "reprojection.h"
 #include "virtual_projection.h"
+#include "lengthproj.h"
+#include "IR_ImageCBCT.h"
 
 extern "C" {
 
@@ -106,6 +110,22 @@ int ref_proj_load(const char* txt, const char* workdir, int* ints /* 9 */, doubl
     if (vol && nv <= vol_cap) memcpy(vol, pd->matVolume, nv * sizeof(unsigned short));
     deleteProjectionData(pd);
     return n;
+}
+
+/* lengthproj over angles [a0, a1) of proj [nAng][H][W] u16; npf [Nz][N][N] (overwritten). */
+void ref_lengthproj(int ite, int N, int Nz, const unsigned short* proj, float* npf, double s, double t, int W, int H,
+                    int a0, int a1, const double* ang, const double* c, const double* sn, const double* xc,
+                    const double* yc, double pitch, double pitch_iso, double sad, double pixnum, double xr, double yr,
+                    double zr) {
+    double xalpha[1] = {0};
+    lengthproj(ite, N, Nz, NULL, NULL, npf, (unsigned short*)proj, s, t, W, H, a0, a1, (double*)ang, (double*)c,
+               (double*)sn, (double*)xc, (double*)yc, 1000.0, xalpha, pitch, pitch_iso, sad, pixnum, xr, yr, zr);
+}
+
+/* One IR_ImageCBCT update: before [Nz][N][N] short -> out [Nz][N][N] float. */
+void ref_ir_update(int N, int Nz, double s, const short* before, float* out, const float* npf, const float* npf_re,
+                   double wbefore, double wtv, double wprior) {
+    IR_ImageCBCT(0, N, Nz, s, (short*)before, out, (float*)npf, (float*)npf_re, wbefore, wtv, wprior, 1);
 }
 
 }  // extern "C"

@@ -51,6 +51,25 @@ def material_maps_case(seed=4242):
     return dict(seed=seed, labels=lab)
 
 
+def iterative_fixture():
+    """7. lengthproj and IR_ImageCBCT (P/lengthproj.cpp, P/IR_ImageCBCT.cpp) on a small seeded case."""
+    rng = np.random.default_rng(31337)
+    N, Nz, H, W, s, t = 20, 5, 24, 30, 0.6, 0.45
+    ang = np.deg2rad(np.array([-150.0001, -60.5, 0.0, 45.0, 100.25, 170.0]))
+    xs, ys = np.full(6, 0.021), np.full(6, -0.013)
+    cs, sn, xc, yc = O.ref_trig_table(W, H, ang, xs, ys)
+    g = O.Geometry(N=N, s=s, Nz=Nz, t=t, H=H, W=W, cos_t=cs, sin_t=sn, xc=xc, yc=yc, x_reg=0.2, z_reg=-0.3)
+    proj = rng.integers(0, 65536, (6, H, W)).astype(np.uint16)
+    npf = O.ref_lengthproj(g, proj, ang, 0)
+    npf1 = O.ref_lengthproj(g, proj, ang, 1)
+    before = rng.integers(0, 3000, (Nz, N, N)).astype(np.int16)
+    npf_re = (npf * rng.uniform(0.8, 1.2, npf.shape)).astype(np.float32)
+    ir = O.ref_ir_update(before, npf, npf_re, s, 0.3, 0.05)
+    np.savez_compressed(f"{OUT}/iterative_small.npz", N=N, Nz=Nz, H=H, W=W, s=s, t=t, x_reg=0.2, z_reg=-0.3, angles=ang,
+                        cos=cs, sin=sn, xc=xc, yc=yc, proj=proj, npf=npf, npf_ite1=npf1, before=before, npf_re=npf_re,
+                        ir=ir)
+
+
 def main():
     if not O.ref_available() or not O.ref_to_material_available():
         sys.exit("oracle/_ref not built (make -C oracle with the reference tree present)")
@@ -136,6 +155,7 @@ def main():
                         media_printed=media, md_sha256=np.bytes_(hashlib.sha256(md.tobytes()).hexdigest()),
                         whole_sha256=np.bytes_(hashlib.sha256(whole.tobytes()).hexdigest()),
                         md_sums=md.astype(np.int64).sum(axis=(2, 3)))
+    iterative_fixture()
     print("golden fixtures written to", OUT)
 
 

@@ -71,6 +71,9 @@ SIGNATURES = {
     "vcbct_projector_set_volume_labels": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
                                                   c_void_p, c_void_p]),
     "vcbct_projector_get_volume_u16": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "vcbct_projector_lengthproj": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "vcbct_ir_update": (c_int, [c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double,
+                                c_void_p, c_void_p]),
     "vcbct_projector_logproj": (c_int, [c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_projector_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_projector_logproj_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p]),

@@ -122,6 +122,7 @@ extern "C" void vcbct_projector_destroy(vcbct_projector* p) {
     (void)hipFree(p->d_vox_f32);
     (void)hipFree(p->d_grad_vox);
     (void)hipFree(p->d_coef);
+    (void)hipFree(p->d_lp_tab);
     (void)hipFree(p->d_mu);
     (void)hipFree(p->d_pdf);
     free(p);

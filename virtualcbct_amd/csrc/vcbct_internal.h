@@ -136,6 +136,7 @@ struct vcbct_projector {
     float* d_vox_f32;      // [N][N][Nz][8]
     float* d_grad_vox;     // adjoint scatter target, planar [mat][y][x][z]
     float* d_coef;         // adjoint per-ray coefficients [mat][angle][row][col]
+    void* d_lp_tab;        // lengthproj per-angle constants (two forms, see iterative.hip)
     double* d_mu;          // [nmat][NE]
     double* d_pdf;         // [NE]
     int NE;
