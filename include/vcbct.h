@@ -224,6 +224,16 @@ int vcbct_fdk_plan_execute(vcbct_fdk_plan* p, const float* dev_proj, float* dev_
                            void* stream);
 int vcbct_fdk_plan_slices_per_thread(const vcbct_fdk_plan* p);
 
+/* Panel-offset class of an FDK angle list (F/projectionDataMVkV.cpp:277-281): "S" when the mean panel x
+ * shift of the used projections is below 6, "L" above 15, else "M".  out: at least 2 chars. */
+int vcbct_fdk_size_class(int n_proj, const double* host_xshift, char* out);
+/* prepareAngles (F/misc.cpp:87-180): dev_mask [n_proj][R][R] u8 = 1 where angle k lies in voxel (i, j)'s
+ * window (th1 + 170 deg, th1 + 10 deg) through +-180 deg, th1 the voxel centre's angle from +y measured
+ * clockwise; all 1 for size class "S".  The reference builds this mask in its main program and never
+ * applies it.  Synchronises the stream (set-up call). */
+int vcbct_fdk_angle_mask(int n_proj, const double* host_angles_rad, int recon_size, double recon_scale,
+                         const char* size_class, uint8_t* dev_mask, void* stream);
+
 /* ------------------------------------------------------------------------------------------------
  * Batched launch (no reference counterpart): many forward problems — volumes of different slice counts,
  * angle counts and spectra, each set up as a vcbct_projector — in ONE persistent kernel launch with

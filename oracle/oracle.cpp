@@ -1037,3 +1037,52 @@ int orc_ir_update(int N, int Nz, double scale, const int16_t* before, const floa
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int orc_size_class(int n, const double* xshift, char* out) {
+    /* F/projectionDataMVkV.cpp:277-281: the mean panel x shift of the used projections */
+    double xa = 0.0;
+    for (int i = 0; i < n; i++) xa += xshift[i];
+    xa = xa / n;
+    strcpy(out, xa < 6 ? "S" : (xa > 15 ? "L" : "M"));
+    return 0;
+}
+
+int orc_angle_mask(int n, const double* angles, int R, double scale, const char* cls, char* mask) {
+    /* F/misc.cpp:87-180: for M and L panels, voxel (i, j) keeps angle k when k lies in the window
+     * (th1 + 170 deg, th1 + 10 deg) taken through +-180 deg, th1 the voxel's polar angle from +y (clockwise
+     * positive); an S panel keeps every angle. */
+    const double pi = 3.141592653589793238462643383279;
+    const size_t RR = (size_t)R * R;
+    if (strcmp(cls, "S") == 0) {
+        memset(mask, 1, (size_t)n * RR);
+        return 0;
+    }
+    for (int j = 0; j < R; j++) {
+        const double y = -(-R / 2 + j + 0.5) * scale;
+        for (int i = 0; i < R; i++) {
+            const double x = (-R / 2 + i + 0.5) * scale;
+            double th1 = 0.0;
+            if (y > 0) th1 = -atan(x / y);
+            else if (x < 0 && y < 0) th1 = (-atan(x / y) + pi);
+            else if (x > 0 && y < 0) th1 = -(atan(x / y) + pi);
+            double ts = th1 + pi - 10 * pi / 180;
+            double te = th1 + 10 * pi / 180;
+            if (ts < -pi) ts = ts + 2 * pi;
+            else if (ts > pi) ts = ts - 2 * pi;
+            if (te > pi) te = te - 2 * pi;
+            else if (te < -pi) te = te + 2 * pi;
+            for (int k = 0; k < n; k++) {
+                const double a = angles[k];
+                char v;
+                if (ts > te) v = ((a > ts && a < pi) || (a > -pi && a < te)) ? 1 : 0;
+                else v = (a > ts && a < te) ? 1 : 0;
+                mask[(size_t)k * RR + (size_t)j * R + i] = v;
+            }
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

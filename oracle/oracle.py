@@ -344,15 +344,15 @@ def ref_mask(R):
 def ref_fdk_load(txt, workdir, cap=4096):
     ints = np.zeros(8, np.int32)
     dbls = np.zeros(3)
-    strs = C.create_string_buffer(1024)
+    strs = C.create_string_buffer(1280)
     a, x, y = np.zeros(cap), np.zeros(cap), np.zeros(cap)
     n = ref("fdk").ref_fdk_load(txt.encode(), workdir.encode(), _p(ints, _I32), _p(dbls, _D), strs,
                                 _p(a, _D), _p(x, _D), _p(y, _D), cap)
     if n < 0:
         raise RuntimeError(f"ref_fdk_load failed {n}")
     raw = strs.raw
-    s = [raw[i * 256:(i + 1) * 256].split(b"\0")[0].decode() for i in range(4)]
-    return dict(ints=ints, dbls=dbls, strs=s, angles=a[:n], xshift=x[:n], yshift=y[:n])
+    s = [raw[i * 256:(i + 1) * 256].split(b"\0")[0].decode() for i in range(5)]
+    return dict(ints=ints, dbls=dbls, strs=s[:4], size_class=s[4], angles=a[:n], xshift=x[:n], yshift=y[:n])
 
 
 def ref_proj_load(txt, workdir, cap=4096, vol_cap=0):
@@ -502,3 +502,27 @@ def ref_ir_update(before_i16, npf, npf_re, scale, wbefore, weight_tv, weight_pri
                               _p(np.ascontiguousarray(npf, np.float32), _F),
                               _p(np.ascontiguousarray(npf_re, np.float32), _F), d(wbefore), d(weight_tv), d(weight_prior))
     return out
+
+
+# --------------------------------------------------------------------------------------------------
+# FDK angle mask (F/misc.cpp:87-180) and the panel-offset class (F/projectionDataMVkV.cpp:277-281).
+
+def size_class(xshift):
+    out = C.create_string_buffer(4)
+    x = np.ascontiguousarray(xshift, np.float64)
+    lib().orc_size_class(len(x), _p(x, _D), out)
+    return out.value.decode()
+
+
+def angle_mask(angles_rad, R, scale, cls):
+    a = np.ascontiguousarray(angles_rad, np.float64)
+    m = np.empty((len(a), R, R), np.uint8)
+    lib().orc_angle_mask(len(a), _p(a, _D), R, C.c_double(scale), cls.encode(), m.ctypes.data_as(_CH))
+    return m
+
+
+def ref_angle_mask(angles_rad, R, scale, cls):
+    a = np.ascontiguousarray(angles_rad, np.float64)
+    m = np.empty((len(a), R, R), np.uint8)
+    ref("fdk").ref_prepare_angles(len(a), _p(a, _D), R, C.c_double(scale), cls.encode(), m.ctypes.data_as(_CH))
+    return m

@@ -5,6 +5,7 @@
  *   prepareConstMap / filterSinogram   F/filterSinogram.cpp:65-186
  *   reconstructImage                   F/reconstructImage.cpp:10-184
  *   prepareMaskData, prepareTrigonometricTable  F/misc.cpp:9-37, 181-200
+ *   prepareAngles                      F/misc.cpp:87-180
  *   loadData                           F/projectionDataMVkV.cpp:51-321
  */
 #include <stdio.h>
@@ -78,7 +79,7 @@ void ref_mask(int R, char* out) {
 }
 
 /* loadData for the FDK txt, run in workdir.  ints: total, W, H, R, reconOffset, Z, xfactor, used;
- * dbls: step, scale, pitch_iso; strs: 4 x 256 (proj path, ext, flexmap, output). */
+ * dbls: step, scale, pitch_iso; strs: 5 x 256 (proj path, ext, flexmap, output, panel-offset class). */
 int ref_fdk_load(const char* txt, const char* workdir, int* ints, double* dbls, char* strs, double* ang, double* xs,
                  double* ys, int cap) {
     char cwd[4096];
@@ -95,11 +96,17 @@ int ref_fdk_load(const char* txt, const char* workdir, int* ints, double* dbls, 
     dbls[0] = pd->reconStep; dbls[1] = pd->reconScale; dbls[2] = pd->DETECTOR_PITCH_CM_AT_ISO_CENT;
     strncpy(strs, pd->projImgFileDirname, 255); strncpy(strs + 256, pd->projImgFileExt, 255);
     strncpy(strs + 512, pd->angleDataFilename, 255); strncpy(strs + 768, pd->reconDataFilename, 255);
+    strncpy(strs + 1024, pd->SMLClassString, 255);
     int n = pd->usedProjNumber < cap ? pd->usedProjNumber : cap;
     for (int i = 0; i < n; i++) { ang[i] = pd->anglesRad[i]; xs[i] = pd->xshift[i]; ys[i] = pd->yshift[i]; }
     /* projVolume is never allocated by the vCBCT loader; free the rest by hand */
     free(pd->anglesRad); free(pd->xshift); free(pd->yshift); free(pd);
     return n;
+}
+
+/* prepareAngles: mask [n][R][R] (1 = angle inside the voxel's window). */
+void ref_prepare_angles(int n, const double* ang, int R, double scale, const char* cls, char* mask) {
+    prepareAngles(n, (double*)ang, R, mask, scale, cls);
 }
 
 }  // extern "C"

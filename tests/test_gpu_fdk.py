@@ -96,3 +96,13 @@ def test_fdk_chain_c1_water(torch):
     assert err <= 1e-3 * water
     short = fdk.to_short(rec)
     assert short.dtype == np.int16
+
+
+@pytest.mark.parametrize("R,cls", [(17, "M"), (270, "L"), (64, "S")])
+def test_angle_mask_matches_oracle(torch, R, cls):
+    """prepareAngles (F/misc.cpp:87-180) on the device, bit-exact against the oracle."""
+    from virtualcbct_amd import fdk
+    rng = np.random.default_rng(R)
+    ang = np.concatenate([np.sort(rng.uniform(-np.pi, np.pi, 50)), [-np.pi, np.pi, 0.0]])
+    got = fdk.prepareAngles(ang, R, 0.1, cls).cpu().numpy()
+    assert np.array_equal(got, O.angle_mask(ang, R, 0.1, cls))

@@ -100,3 +100,45 @@ def test_iterative_fixture():
     assert np.array_equal(O.lengthproj(g, d["proj"], 0), d["npf"])
     assert np.array_equal(O.lengthproj(g, d["proj"], 1), d["npf_ite1"])
     assert np.array_equal(O.ir_update(d["before"], d["npf"], d["npf_re"], float(d["s"]), 0.3, 0.05), d["ir"])
+
+
+# ------------------------------------------------------------- FDK angle mask and panel-offset class
+
+@need_ref
+@pytest.mark.parametrize("R", [16, 17, 64])
+@pytest.mark.parametrize("cls", ["S", "M", "L"])
+def test_angle_mask_oracle_matches_reference(R, cls):
+    rng = np.random.default_rng(R)
+    ang = np.concatenate([np.sort(rng.uniform(-np.pi, np.pi, 30)), [-np.pi, np.pi, 0.0, np.pi / 2]])
+    assert np.array_equal(O.angle_mask(ang, R, 0.1, cls), O.ref_angle_mask(ang, R, 0.1, cls))
+
+
+def test_angle_mask_closed_form():
+    """A voxel near the +y axis (th1 = -atan(x/y), -3 deg) keeps the 200-degree window from th1 + 170 deg up
+    through +-180 deg to th1 + 10 deg and drops the rest; the mask of an S panel is all ones."""
+    R = 20
+    ang = np.deg2rad(np.array([-169.0, -90.0, 0.0, 6.0, 8.0, 90.0, 166.0, 168.0, 175.0]))
+    m = O.angle_mask(ang, R, 0.1, "M")
+    j, i = 0, R // 2  # x = 0.05, y = 0.95 (scale units)
+    th1 = -np.arctan(0.5 / 9.5)
+    ts, te = th1 + np.pi - np.pi / 18, th1 + np.pi / 18
+    expect = [bool(a > ts or a < te) for a in ang]
+    assert expect == [True, True, True, True, False, False, False, True, True]
+    assert list(m[:, j, i].astype(bool)) == expect
+    assert O.angle_mask(ang, R, 0.1, "S").all()
+
+
+def test_size_class_thresholds():
+    assert O.size_class([5.9] * 4) == "S" and O.size_class([6.0, 6.0]) == "M"
+    assert O.size_class([15.0]) == "M" and O.size_class([15.1, 15.1]) == "L"
+    from virtualcbct_amd import fdk
+    for xs in ([5.9] * 4, [6.0, 6.0], [15.0], [15.1, 15.1], [0.0, 30.0]):
+        assert fdk.size_class(xs) == O.size_class(xs)
+
+
+@need_ref
+def test_size_class_matches_reference_loader(tmp_path, ref_dir):
+    """The FDK loader's class for the shipped 36-angle txt (its flexmap shifts)."""
+    F = f"{ref_dir}/Release_recon_from_forward_pro_float_CM"
+    d = O.ref_fdk_load(f"{F}/txtfile/FDK_36pro_water20cm.txt", F)
+    assert d["size_class"] == O.size_class(d["xshift"])

@@ -74,6 +74,8 @@ SIGNATURES = {
     "vcbct_projector_lengthproj": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "vcbct_ir_update": (c_int, [c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double,
                                 c_void_p, c_void_p]),
+    "vcbct_fdk_size_class": (c_int, [c_int, P_D, C.c_char_p]),
+    "vcbct_fdk_angle_mask": (c_int, [c_int, P_D, c_int, c_double, c_char_p, c_void_p, c_void_p]),
     "vcbct_projector_logproj": (c_int, [c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_projector_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_projector_logproj_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p]),

@@ -104,6 +104,24 @@ def fdk_device(proj, zstart, reconStep, reconScale, reconSize, reconSlices, xfac
     return rec
 
 
+def size_class(xshift):
+    """Panel-offset class of the used projections' x shifts (F/projectionDataMVkV.cpp:277-281): "S", "M" or "L"."""
+    x = np.ascontiguousarray(xshift, np.float64)
+    out = C.create_string_buffer(4)
+    check(lib().vcbct_fdk_size_class(len(x), ptr(x, P_D), out))
+    return out.value.decode()
+
+
+def prepareAngles(anglesRad, reconSize, reconScale, SMLClassString, stream=None):
+    """The per-voxel angle mask of prepareAngles (F/misc.cpp:87-180): uint8 device tensor [n][R][R]."""
+    torch = _torch()
+    a = np.ascontiguousarray(anglesRad, np.float64)
+    out = torch.empty((len(a), reconSize, reconSize), dtype=torch.uint8, device="cuda")
+    check(lib().vcbct_fdk_angle_mask(len(a), ptr(a, P_D), int(reconSize), float(reconScale),
+                                     str(SMLClassString).encode(), C.c_void_p(out.data_ptr()), _stream(stream)))
+    return out
+
+
 def to_short(recon):
     """(short) truncation toward zero of the FP volume (F/main.cpp:179-185)."""
     r = np.asarray(recon)
