@@ -46,6 +46,21 @@ def test_filter_w400_matches_fp64_oracle(torch):
     assert err < 2e-5
 
 
+@pytest.mark.parametrize("W,nrows", [(33, 7), (700, 9), (1100, 5)])
+def test_filter_other_widths_match_fp64_oracle(torch, W, nrows):
+    """FFT sizes 128 (radix-2 last stage), 2048 (tables staged in LDS at the 64 KB limit) and 4096 (tables read
+    from global memory), odd row counts (the last row pair has one row) and block tails."""
+    from virtualcbct_amd import fdk
+    rng = np.random.default_rng(W)
+    proj = rng.uniform(0.02, 1.0, (2, nrows, W)).astype(np.float32)
+    xs, ys = np.array([0.4, -7.0]), np.array([0.1, 2.0])
+    out = fdk.filterSinogram(nrows, W, 2, proj, xs, ys)
+    ref = O.filter_sinogram(proj, xs, ys, acc64=True)
+    err = np.abs(out - ref).max() / np.abs(ref).max()
+    print(f"filter W={W} vs FP64: {err:.3e}")
+    assert err < 2e-5
+
+
 def test_backprojection_matches_reference_fixture(torch):
     from virtualcbct_amd import fdk
     d = np.load(os.path.join(GOLD, "fdk_small.npz"))

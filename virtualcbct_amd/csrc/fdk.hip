@@ -73,7 +73,7 @@ static void fft64(std::vector<std::complex<double>>& a) {
 struct FilterPlan {
     int W, L, logL;
     float2* d_H;    // [L] frequency response of the circularly wrapped h, scaled by 1/L
-    float2* d_tw;   // [L/2] exp(-2 pi i k / L)
+    float2* d_tw;   // [L] exp(-2 pi i m / L)
 };
 
 static std::mutex g_plan_mu;
@@ -104,37 +104,70 @@ static int get_plan(int W, double pitch_iso, FilterPlan** out) {
         h[P.L - d] = ramp_offcentre(-d, pitch_iso, W);
     }
     fft64(h);
-    std::vector<float2> Hf(P.L), tw(P.L / 2);
+    std::vector<float2> Hf(P.L), tw(P.L);
     for (int k = 0; k < P.L; ++k) Hf[k] = make_float2((float)(h[k].real() / P.L), (float)(h[k].imag() / P.L));
-    for (int k = 0; k < P.L / 2; ++k) tw[k] = make_float2((float)cos(-2.0 * kPI * k / P.L), (float)sin(-2.0 * kPI * k / P.L));
+    for (int k = 0; k < P.L; ++k) tw[k] = make_float2((float)cos(-2.0 * kPI * k / P.L), (float)sin(-2.0 * kPI * k / P.L));
     VCBCT_HIP(dev_malloc((void**)&P.d_H, P.L * sizeof(float2)));
-    VCBCT_HIP(dev_malloc((void**)&P.d_tw, P.L / 2 * sizeof(float2)));
+    VCBCT_HIP(dev_malloc((void**)&P.d_tw, P.L * sizeof(float2)));
     VCBCT_HIP(hipMemcpy(P.d_H, Hf.data(), P.L * sizeof(float2), hipMemcpyHostToDevice));
-    VCBCT_HIP(hipMemcpy(P.d_tw, tw.data(), P.L / 2 * sizeof(float2), hipMemcpyHostToDevice));
+    VCBCT_HIP(hipMemcpy(P.d_tw, tw.data(), P.L * sizeof(float2), hipMemcpyHostToDevice));
     g_plans[key] = P;
     *out = &g_plans[key];
     return 0;
 }
 
-// One block filters two detector rows at once: z = q1 + i q2 (real kernel, so the two convolutions
-// separate into the real and imaginary parts), radix-2 Stockham FFT in LDS, product with H, inverse
-// FFT, cosine weight Cm at the output pixel.
+// Two detector rows form one complex signal z = q1 + i q2 (the kernel is real, so the two convolutions
+// separate into the real and imaginary parts).  Each is zero-padded to L = 2^logL, transformed with a
+// Stockham FFT in LDS (radix-4 stages, one radix-2 stage when logL is odd; each of the 256 threads does one
+// butterfly per stage), multiplied by the response H, transformed back, and cosine-weighted at the output
+// pixel.  A block filters FSIG row pairs in turn, so the twiddle and response tables it stages in LDS are
+// read from L2 once per 2 FSIG rows (up to L = 2048; beyond, they are read from global memory).
 constexpr int FT = 256;
+constexpr int FSIG = 8;
+constexpr int FPF = 8;  // pixels per thread per row held in registers: W <= FPF * FT
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
+// x -> DFT(x) (INV: the unscaled inverse) through the ping-pong buffers x, y; returns the buffer holding it.
+// T[m] = exp(-2 pi i m / L).  Stage s of radix R has sub-transform size Ns = R^s: butterfly j reads
+// x[j + r L / R], scales input r by w^(k r), w = exp(-2 pi i / (R Ns)), k = j mod Ns, and writes
+// y[(j / Ns) R Ns + k + r Ns].
 template <bool INV>
-__device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* __restrict__ tw, int L, int logL) {
-    // Stockham autosort, radix 2: stage s has stride Ns = 2^s
-    const int half = L >> 1;
-    for (int s = 0, Ns = 1; s < logL; ++s, Ns <<= 1) {
+__device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T, int L, int logL) {
+    int Ns = 1;
+    const int q = L >> 2;
+    for (int s = 0; s < logL / 2; ++s) {
+        const int shift = logL - 2 - 2 * s;  // w^(k r) = T[(k r) << shift]
+        for (int j = threadIdx.x; j < q; j += FT) {
+            const int k = j & (Ns - 1);
+            float2 v0 = x[j], v1 = x[j + q], v2 = x[j + 2 * q], v3 = x[j + 3 * q];
+            float2 w1 = T[k << shift], w2 = T[(2 * k) << shift], w3 = T[(3 * k) << shift];
+            if (INV) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+            v1 = cmul(v1, w1);
+            v2 = cmul(v2, w2);
+            v3 = cmul(v3, w3);
+            const float2 a0 = make_float2(v0.x + v2.x, v0.y + v2.y), a1 = make_float2(v0.x - v2.x, v0.y - v2.y);
+            const float2 a2 = make_float2(v1.x + v3.x, v1.y + v3.y), d = make_float2(v1.x - v3.x, v1.y - v3.y);
+            const float2 a3 = INV ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);  // (+-i) (v1 - v3)
+            const int o = ((j - k) << 2) + k;
+            y[o] = make_float2(a0.x + a2.x, a0.y + a2.y);
+            y[o + Ns] = make_float2(a1.x + a3.x, a1.y + a3.y);
+            y[o + 2 * Ns] = make_float2(a0.x - a2.x, a0.y - a2.y);
+            y[o + 3 * Ns] = make_float2(a1.x - a3.x, a1.y - a3.y);
+        }
+        __syncthreads();
+        float2* t = x; x = y; y = t;
+        Ns <<= 2;
+    }
+    if (logL & 1) {  // last stage radix 2: Ns = L / 2, w^k = T[k]
+        const int half = L >> 1;
         for (int j = threadIdx.x; j < half; j += FT) {
-            int k = j & (Ns - 1);                  // position inside the current sub-transform
-            float2 a = x[j], b = x[j + half];
-            float2 w = tw[k * (half / Ns)];        // exp(-2 pi i k / (2 Ns))
+            const int k = j & (Ns - 1);
+            const float2 a = x[j];
+            float2 w = T[k];
             if (INV) w.y = -w.y;
-            float2 bw = cmul(b, w);
-            int o = (j - k) * 2 + k;               // (j / Ns) * 2 Ns + k
+            const float2 bw = cmul(x[j + half], w);
+            const int o = ((j - k) << 1) + k;
             y[o] = make_float2(a.x + bw.x, a.y + bw.y);
             y[o + Ns] = make_float2(a.x - bw.x, a.y - bw.y);
         }
@@ -147,40 +180,75 @@ __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* _
 __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, float* __restrict__ out, int H, int W,
                                                long nrows, int L, int logL, const float2* __restrict__ Hf,
                                                const float2* __restrict__ tw, const float* __restrict__ xc_tab,
-                                               const float* __restrict__ yc_tab, float pitchW) {
+                                               const float* __restrict__ yc_tab, float pitchW, int staged) {
+    // LDS: [A][B] ping-pong buffers, then (staged) the twiddle and response tables; without staging (L above
+    // 2048, where the four would exceed 64 KB) the tables are read from global memory
     extern __shared__ float2 lds[];
     float2* A = lds;
     float2* B = lds + L;
-    const long r1 = 2 * (long)blockIdx.x, r2 = r1 + 1;
-    const bool has2 = r2 < nrows;
-    const float* p1 = proj + r1 * W;
-    const float* p2 = proj + r2 * W;
-    for (int k = threadIdx.x; k < L; k += FT) {
-        float a = 0.f, b = 0.f;
-        if (k < W) {
-            a = -logf(p1[k]);
-            if (has2) b = -logf(p2[k]);
+    const float2* T = tw;
+    const float2* HL = Hf;
+    if (staged) {
+        float2* t = lds + 2 * L;
+        float2* h = lds + 3 * L;
+        for (int k = threadIdx.x; k < L; k += FT) {
+            t[k] = tw[k];
+            h[k] = Hf[k];
         }
-        A[k] = make_float2(a, b);
+        T = t;
+        HL = h;
     }
-    __syncthreads();
-    float2* X = fft_lds<false>(A, B, tw, L, logL);
-    float2* Y = (X == A) ? B : A;
-    for (int k = threadIdx.x; k < L; k += FT) X[k] = cmul(X[k], Hf[k]);
-    __syncthreads();
-    X = fft_lds<true>(X, Y, tw, L, logL);
-    // cosine weight at the output pixel (F/filterSinogram.cpp:141-154)
-    for (int s = 0; s < 2; ++s) {
-        const long r = r1 + s;
-        if (s == 1 && !has2) break;
-        const int th = (int)(r / H), n = (int)(r % H);
-        const float pos2 = fabsf(yc_tab[th] + (float)n) * pitchW;
-        for (int m = threadIdx.x; m < W; m += FT) {
-            float pos1 = fabsf(xc_tab[th] + (float)m) * pitchW;
-            float cm = 100.0f / sqrtf(10000.0f + pos1 * pos1 + pos2 * pos2);
-            float v = (s == 0) ? X[m].x : X[m].y;
-            out[r * W + m] = v * cm;
+    // the next row pair's pixels are loaded into registers while the current pair is transformed (the
+    // block barriers wait for LDS traffic only, so these loads stay in flight across them)
+    float pa[FPF], pb[FPF];
+    auto prefetch = [&](long r1) {
+        const float* p1 = proj + r1 * W;
+        const float* p2 = proj + (r1 + 1) * W;
+        const bool has2 = r1 + 1 < nrows;
+#pragma unroll
+        for (int i = 0; i < FPF; ++i) {
+            const int k = threadIdx.x + i * FT;
+            pa[i] = 1.f;
+            pb[i] = 1.f;
+            if (k < W) {
+                pa[i] = p1[k];
+                if (has2) pb[i] = p2[k];
+            }
         }
+    };
+    const long first = 2 * (long)blockIdx.x * FSIG;
+    if (first < nrows) prefetch(first);
+    for (int sig = 0; sig < FSIG; ++sig) {
+        const long r1 = first + 2 * sig, r2 = r1 + 1;
+        if (r1 >= nrows) break;  // uniform over the block
+        const bool has2 = r2 < nrows;
+#pragma unroll
+        for (int i = 0; i < FPF; ++i) {
+            const int k = threadIdx.x + i * FT;
+            if (k < W) A[k] = make_float2(-logf(pa[i]), has2 ? -logf(pb[i]) : 0.f);
+        }
+        for (int k = W + threadIdx.x; k < L; k += FT) A[k] = make_float2(0.f, 0.f);
+        __syncthreads();
+        if (sig + 1 < FSIG && r1 + 2 < nrows) prefetch(r1 + 2);
+        float2* X = fft_lds<false>(A, B, T, L, logL);
+        float2* Y = (X == A) ? B : A;
+        for (int k = threadIdx.x; k < L; k += FT) X[k] = cmul(X[k], HL[k]);
+        __syncthreads();
+        X = fft_lds<true>(X, Y, T, L, logL);
+        // cosine weight at the output pixel (F/filterSinogram.cpp:141-154)
+        for (int s = 0; s < 2; ++s) {
+            const long r = r1 + s;
+            if (s == 1 && !has2) break;
+            const int th = (int)(r / H), n = (int)(r % H);
+            const float pos2 = fabsf(yc_tab[th] + (float)n) * pitchW;
+            for (int m = threadIdx.x; m < W; m += FT) {
+                float pos1 = fabsf(xc_tab[th] + (float)m) * pitchW;
+                float cm = 100.0f / sqrtf(10000.0f + pos1 * pos1 + pos2 * pos2);
+                float v = (s == 0) ? X[m].x : X[m].y;
+                out[r * W + m] = v * cm;
+            }
+        }
+        __syncthreads();  // X is read above; the next pair overwrites A and B
     }
 }
 
@@ -311,11 +379,15 @@ static std::vector<float> filter_table(int n, int W, const double* xshift, const
 static int launch_filter(const FilterPlan* P, int H, int W, int n, const float* dp, float* dq, const float* d_ctab,
                          double pitch_iso, hipStream_t st) {
     const long nrows = (long)n * H;
-    const long nblk = (nrows + 1) / 2;
+    const long npairs = (nrows + 1) / 2;
+    const long nblk = (npairs + FSIG - 1) / FSIG;
     if (nblk > INT_MAX) return fail(VCBCT_EINVAL, "filter_sinogram: too many rows");
     const float pitchW = (float)(pitch_iso * 800 / W);
-    hipLaunchKernelGGL(k_filter, dim3((unsigned)nblk), dim3(FT), 2 * P->L * sizeof(float2), st, dp, dq, H, W, nrows,
-                       P->L, P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW);
+    const size_t sig_bytes = 2 * (size_t)P->L * sizeof(float2);  // the two signal buffers
+    const size_t tab_bytes = 2 * (size_t)P->L * sizeof(float2);
+    const int staged = sig_bytes + tab_bytes <= 65536;
+    hipLaunchKernelGGL(k_filter, dim3((unsigned)nblk), dim3(FT), sig_bytes + (staged ? tab_bytes : 0), st, dp, dq,
+                       H, W, nrows, P->L, P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, staged);
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
