@@ -114,6 +114,8 @@ extern "C" int vcbct_batch_add_forward(vcbct_batch* b, vcbct_projector* p, float
     P.vox = P.vox_is_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16;
     P.mu = p->d_mu;
     P.pdf = p->d_pdf;
+    P.muf = p->d_muf;
+    P.pdff = p->d_pdff;
     P.NE = p->NE;
     P.out = dev_out;
     P.inv_xf = 1.0 / xfactor;

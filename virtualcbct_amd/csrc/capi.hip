@@ -125,6 +125,8 @@ extern "C" void vcbct_projector_destroy(vcbct_projector* p) {
     (void)hipFree(p->d_lp_tab);
     (void)hipFree(p->d_mu);
     (void)hipFree(p->d_pdf);
+    (void)hipFree(p->d_muf);
+    (void)hipFree(p->d_pdff);
     free(p);
 }
 
@@ -183,14 +185,23 @@ extern "C" int vcbct_projector_set_spectrum(vcbct_projector* p, const float* att
     if (NE != p->NE) {
         (void)hipFree(p->d_mu);
         (void)hipFree(p->d_pdf);
+        (void)hipFree(p->d_muf);
+        (void)hipFree(p->d_pdff);
         p->d_mu = nullptr;
         p->d_pdf = nullptr;
+        p->d_muf = nullptr;
+        p->d_pdff = nullptr;
+        p->NE = 0;
         if (dev_malloc((void**)&p->d_mu, mu.size() * sizeof(double)) != hipSuccess ||
-            dev_malloc((void**)&p->d_pdf, pd.size() * sizeof(double)) != hipSuccess)
+            dev_malloc((void**)&p->d_pdf, pd.size() * sizeof(double)) != hipSuccess ||
+            dev_malloc((void**)&p->d_muf, mu.size() * sizeof(float)) != hipSuccess ||
+            dev_malloc((void**)&p->d_pdff, pd.size() * sizeof(float)) != hipSuccess)
             return fail(VCBCT_ENOMEM, "device allocation of the spectrum failed");
     }
     VCBCT_HIP(hipMemcpy(p->d_mu, mu.data(), mu.size() * sizeof(double), hipMemcpyHostToDevice));
     VCBCT_HIP(hipMemcpy(p->d_pdf, pd.data(), pd.size() * sizeof(double), hipMemcpyHostToDevice));
+    VCBCT_HIP(hipMemcpy(p->d_muf, atten, mu.size() * sizeof(float), hipMemcpyHostToDevice));
+    VCBCT_HIP(hipMemcpy(p->d_pdff, pdf, pd.size() * sizeof(float), hipMemcpyHostToDevice));
     p->NE = NE;
     return 0;
 }
@@ -207,6 +218,8 @@ static Spectrum spec_of(const vcbct_projector* p) {
     Spectrum s;
     s.mu = p->d_mu;
     s.pdf = p->d_pdf;
+    s.muf = p->d_muf;
+    s.pdff = p->d_pdff;
     s.NE = p->NE;
     return s;
 }
@@ -232,7 +245,7 @@ extern "C" int vcbct_projector_line_integrals(vcbct_projector* p, float* dev_L, 
     if (rc) return rc;
     if (!dev_L) return fail(VCBCT_EINVAL, "line_integrals: output is NULL");
     VCBCT_HIP(hipSetDevice(p->device));
-    Spectrum sp{nullptr, nullptr, 0};
+    Spectrum sp{nullptr, nullptr, nullptr, nullptr, 0};
     return launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
                           sp, 1.0, 0, p->g.nAng, nullptr, VCBCT_MODE_LINEINT, dev_L, (hipStream_t)stream);
 }
@@ -310,7 +323,7 @@ extern "C" int vcbct_projector_segment_counts(vcbct_projector* p, float* dev_cou
     if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
     if (!dev_counts) return fail(VCBCT_EINVAL, "segment_counts: output is NULL");
     VCBCT_HIP(hipSetDevice(p->device));
-    Spectrum sp{nullptr, nullptr, 0};
+    Spectrum sp{nullptr, nullptr, nullptr, nullptr, 0};
     return launch_forward(p->g, p->d_tables, nullptr, 0, sp, 1.0, 0, p->g.nAng, nullptr, VCBCT_MODE_SEGCOUNT,
                           dev_counts, (hipStream_t)stream);
 }

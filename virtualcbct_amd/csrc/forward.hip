@@ -211,7 +211,8 @@ __device__ __forceinline__ Tile tile_of(int nA, int W, int quads_per_xcd) {
 template <int MODE, class VoxT>
 __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __restrict__ tab,
                                              const VoxT* __restrict__ vox, const double* __restrict__ mu,
-                                             const double* __restrict__ pdf, int NE, double inv_xf, int a, int rg,
+                                             const double* __restrict__ pdf, const float* __restrict__ muf,
+                                             const float* __restrict__ pdff, int NE, double inv_xf, int a, int rg,
                                              int jj, float* __restrict__ out, float* __restrict__ aux,
                                              StepRec* steps) {
     const int lane = threadIdx.x & 63;
@@ -243,18 +244,23 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) zero = zero && (acc.L[m] == 0.0);
     if (MODE == VCBCT_MODE_INTENSITY) {
-        // sum_e float(pdf_e * exp(-transp_e / xf) * leng), FP32 accumulation in e order
-        // (P/reprojectionOnGPU.cu:432, 1038-1044)
+        // sum_e pdf_e * exp(-transp_e / xf) * leng, accumulated in FP32 in e order (P/reprojectionOnGPU.cu:432,
+        // 1038-1044).  A ray that charges nothing sums the reference's own terms, float(pdf_e * 1.0 * leng), bit
+        // for bit.  Otherwise the terms are FP32: the tables are FP32 as given, the line integrals FP32 sums,
+        // and each term carries ~1e-6 relative error against the reference's FP64 term rounded to float, far
+        // below the FP32 accumulation of the line integrals.  exp(-x) = 2^(-x log2 e).
         float s = 0.0f;
         if (zero) {
             for (int e = 0; e < NE; ++e) s += (float)(ldg(pdf, e) * 1.0 * leng);
         } else {
+            const float lf = (float)leng;
+            const float k2 = (float)(-inv_xf * 1.4426950408889634);
             for (int e = 0; e < NE; ++e) {
-                double tr = 0.0;
+                float tr = 0.0f;
 #pragma unroll
                 for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                    if (m < g.nmat) tr += ldg(mu, m * NE + e) * acc.L[m];
-                s += (float)(ldg(pdf, e) * exp(-tr * inv_xf) * leng);
+                    if (m < g.nmat) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
+                s += ldg(pdff, e) * exp2f(tr * k2) * lf;
             }
         }
         stg(out, o, s);
@@ -275,13 +281,14 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
 
 template <int MODE, class VoxT>
 __global__ __launch_bounds__(256) void k_forward(ProjGeom g, const double* __restrict__ tab, const VoxT* __restrict__ vox,
-                                                 const double* __restrict__ mu, const double* __restrict__ pdf, int NE,
+                                                 const double* __restrict__ mu, const double* __restrict__ pdf,
+                                                 const float* __restrict__ muf, const float* __restrict__ pdff, int NE,
                                                  double inv_xf, int a0, int nA, int quads_per_xcd,
                                                  float* __restrict__ out, float* __restrict__ aux) {
     __shared__ StepRec steps[4][64];
     const Tile t = tile_of(nA, g.W, quads_per_xcd);
     if (t.col >= g.W) return;  // whole wave (column) out of range: uniform exit
-    forward_wave<MODE, VoxT>(g, tab, vox, mu, pdf, NE, inv_xf, a0 + t.a, t.rg, t.col, out, aux,
+    forward_wave<MODE, VoxT>(g, tab, vox, mu, pdf, muf, pdff, NE, inv_xf, a0 + t.a, t.rg, t.col, out, aux,
                              steps[threadIdx.x >> 6]);
 }
 
@@ -326,8 +333,8 @@ __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __res
             const int a = ra % P.g.nAng, rg = ra / P.g.nAng;
             const int jj = q * P.cols_per_xcd + cl;
             if (jj >= P.g.W) continue;
-            forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.NE, P.inv_xf, a,
-                                                    rg, jj, P.out, nullptr, steps[threadIdx.x >> 6]);
+            forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.muf, P.pdff, P.NE,
+                                                     P.inv_xf, a, rg, jj, P.out, nullptr, steps[threadIdx.x >> 6]);
         }
     }
 }
@@ -353,8 +360,8 @@ __global__ __launch_bounds__(256) void k_forward_flat(const BatchProblem* __rest
     const int a = ra % P.g.nAng, rg = ra / P.g.nAng;
     const int jj = ((xcd * P.quads_per_xcd + q) << 2) + (threadIdx.x >> 6);
     if (jj >= P.g.W) return;
-    forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.NE, P.inv_xf, a, rg, jj,
-                                            P.out, nullptr, steps[threadIdx.x >> 6]);
+    forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.muf, P.pdff, P.NE,
+                                             P.inv_xf, a, rg, jj, P.out, nullptr, steps[threadIdx.x >> 6]);
 }
 
 int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks, int vox_is_f32, hipStream_t st) {
@@ -588,7 +595,7 @@ int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int
     const double inv_xf = 1.0 / xfactor;
 #define VCBCT_LAUNCH_FWD(M, T)                                                                              \
     hipLaunchKernelGGL((k_forward<M, T>), dim3(nb), dim3(256), 0, st, g, tables, (const T*)vox, sp.mu, sp.pdf, \
-                       sp.NE, inv_xf, a0, nA, qpx, out, aux)
+                       sp.muf, sp.pdff, sp.NE, inv_xf, a0, nA, qpx, out, aux)
     if (!vox_is_f32) {
         if (mode == VCBCT_MODE_INTENSITY) VCBCT_LAUNCH_FWD(VCBCT_MODE_INTENSITY, uint4);
         else if (mode == VCBCT_MODE_LINEINT) VCBCT_LAUNCH_FWD(VCBCT_MODE_LINEINT, uint4);

@@ -78,6 +78,8 @@ __device__ __forceinline__ void stg(T* p, size_t i, T v) {
 struct Spectrum {
     const double* mu;   // [nmat][NE]
     const double* pdf;  // [NE]
+    const float* muf;   // the same tables as given (FP32): the intensity projection's spectral sum
+    const float* pdff;
     int NE;
 };
 
@@ -90,6 +92,8 @@ struct BatchProblem {
     const void* vox;
     const double* mu;
     const double* pdf;
+    const float* muf;
+    const float* pdff;
     float* out;
     double inv_xf;
     int NE, vox_is_f32;
@@ -139,6 +143,8 @@ struct vcbct_projector {
     void* d_lp_tab;        // lengthproj per-angle constants (two forms, see iterative.hip)
     double* d_mu;          // [nmat][NE]
     double* d_pdf;         // [NE]
+    float* d_muf;          // [nmat][NE] as given
+    float* d_pdff;         // [NE] as given
     int NE;
     int have_u16, have_f32;
 };
