@@ -264,16 +264,17 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
             }
         }
         stg(out, o, s);
-    } else {  // VCBCT_MODE_LOGPROJ: p = -log(sum_e pdf_e e^{-mu.L/xf} / sum_e pdf_e), FP64
+    } else {  // VCBCT_MODE_LOGPROJ: p = -log(sum_e pdf_e e^{-mu.L/xf} / sum_e pdf_e); FP32 terms, FP64 sums
         double sw = 0.0, se = 0.0;
+        const float k2 = (float)(-inv_xf * 1.4426950408889634);
         for (int e = 0; e < NE; ++e) {
-            double tr = 0.0;
+            float tr = 0.0f;
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                if (m < g.nmat) tr += ldg(mu, m * NE + e) * acc.L[m];
+                if (m < g.nmat) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
             const double pe = ldg(pdf, e);
             sw += pe;
-            se += pe * exp(-tr * inv_xf);
+            se += pe * (double)exp2f(tr * k2);
         }
         stg(out, o, (float)(-log(se / sw)));
     }
@@ -453,7 +454,8 @@ struct ScatterG {
 template <int MODE>
 __global__ __launch_bounds__(256) void k_adjoint_coef(ProjGeom g, const double* __restrict__ tab,
                                                       const float4* __restrict__ vox, const double* __restrict__ mu,
-                                                      const double* __restrict__ pdf, int NE, double inv_xf,
+                                                      const double* __restrict__ pdf, const float* __restrict__ muf,
+                                                      int NE, double inv_xf,
                                                       int quads_per_xcd, const float* __restrict__ tg,
                                                       float* __restrict__ coef, double* __restrict__ loss) {
     __shared__ StepRec step_tabs[4][64];
@@ -476,14 +478,16 @@ __global__ __launch_bounds__(256) void k_adjoint_coef(ProjGeom g, const double* 
 
     double lossc = 0.0;
     if (active) {
+        // the exponent and its exponential in FP32 (relative error ~1e-6 per term), the sums in FP64
         double sw = 0.0, se = 0.0, mub[VCBCT_MAX_MAT] = {0, 0, 0, 0, 0, 0};
+        const float k2 = (float)(-inv_xf * 1.4426950408889634);
         for (int e = 0; e < NE; ++e) {
-            double tr = 0.0;
+            float tr = 0.0f;
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                if (m < g.nmat) tr += ldg(mu, m * NE + e) * acc.L[m];
+                if (m < g.nmat) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
             const double pe = ldg(pdf, e);
-            double we = pe * exp(-tr * inv_xf);
+            const double we = pe * (double)exp2f(tr * k2);
             sw += pe;
             se += we;
 #pragma unroll
@@ -619,10 +623,10 @@ int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, co
     const double inv_xf = 1.0 / xfactor;
     if (mode == VCBCT_ADJ_LOSS)
         hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_LOSS>, dim3(nb), dim3(256), 0, st, g, tables, (const float4*)vox,
-                           sp.mu, sp.pdf, sp.NE, inv_xf, qpx, target_or_gp, coef, loss);
+                           sp.mu, sp.pdf, sp.muf, sp.NE, inv_xf, qpx, target_or_gp, coef, loss);
     else
         hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_UPSTREAM>, dim3(nb), dim3(256), 0, st, g, tables,
-                           (const float4*)vox, sp.mu, sp.pdf, sp.NE, inv_xf, qpx, target_or_gp, coef, loss);
+                           (const float4*)vox, sp.mu, sp.pdf, sp.muf, sp.NE, inv_xf, qpx, target_or_gp, coef, loss);
     VCBCT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, grad_vox);
     VCBCT_HIP(hipGetLastError());
