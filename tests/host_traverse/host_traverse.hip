@@ -61,12 +61,12 @@ extern "C" int ht_line_integrals(int N, double s, int Nz, double t, int H, int W
                 }
                 bool same = true;
                 for (int l = 1; l < 64; ++l) same = same && r[l].swap == r[0].swap;
-                vcbct::StepRec tab[64];
+                vcbct::WaveLds tab;
                 for (int l = 0; l < 64; ++l) {
                     if (!act[l]) continue;
                     int j = rg * 64 + l;
                     HostAcc acc{vol, N, Nz, nmat, {0, 0, 0, 0, 0, 0}, 0};
-                    vcbct::traverse_lane(g, r[l], act[l], kmin, kmax, acc, (same && g_use_table) ? tab : nullptr);
+                    vcbct::traverse_lane(g, r[l], act[l], kmin, kmax, acc, (same && g_use_table) ? &tab : nullptr);
                     size_t o = (size_t)a * H * W + (size_t)(H - 1 - j) * W + jj;
                     for (int m = 0; m < nmat; ++m) L[o * nmat + m] = acc.L[m];
                     if (nseg) nseg[o] = acc.nseg;

@@ -214,7 +214,7 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
                                              const double* __restrict__ pdf, const float* __restrict__ muf,
                                              const float* __restrict__ pdff, int NE, double inv_xf, int a, int rg,
                                              int jj, float* __restrict__ out, float* __restrict__ aux,
-                                             StepRec* steps) {
+                                             WaveLds* steps) {
     const int lane = threadIdx.x & 63;
     const int j = rg * 64 + lane;
     const bool active = j < g.H;
@@ -286,11 +286,11 @@ __global__ __launch_bounds__(256) void k_forward(ProjGeom g, const double* __res
                                                  const float* __restrict__ muf, const float* __restrict__ pdff, int NE,
                                                  double inv_xf, int a0, int nA, int quads_per_xcd,
                                                  float* __restrict__ out, float* __restrict__ aux) {
-    __shared__ StepRec steps[4][64];
+    __shared__ WaveLds steps[4];
     const Tile t = tile_of(nA, g.W, quads_per_xcd);
     if (t.col >= g.W) return;  // whole wave (column) out of range: uniform exit
     forward_wave<MODE, VoxT>(g, tab, vox, mu, pdf, muf, pdff, NE, inv_xf, a0 + t.a, t.rg, t.col, out, aux,
-                             steps[threadIdx.x >> 6]);
+                             &steps[threadIdx.x >> 6]);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -313,7 +313,7 @@ __device__ __forceinline__ int find_problem(const BatchProblem* __restrict__ pr,
 template <class VoxT>
 __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __restrict__ pr, int np,
                                                        const int* __restrict__ xcd_total, int* __restrict__ counters) {
-    __shared__ StepRec steps[4][64];
+    __shared__ WaveLds steps[4];
     const int home = blockIdx.x & 7;
     const int lane = threadIdx.x & 63;
     for (int qi = 0; qi < 8; ++qi) {
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __res
             const int jj = q * P.cols_per_xcd + cl;
             if (jj >= P.g.W) continue;
             forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.muf, P.pdff, P.NE,
-                                                     P.inv_xf, a, rg, jj, P.out, nullptr, steps[threadIdx.x >> 6]);
+                                                     P.inv_xf, a, rg, jj, P.out, nullptr, &steps[threadIdx.x >> 6]);
         }
     }
 }
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __res
 // carries over).
 template <class VoxT>
 __global__ __launch_bounds__(256) void k_forward_flat(const BatchProblem* __restrict__ pr, int np) {
-    __shared__ StepRec steps[4][64];
+    __shared__ WaveLds steps[4];
     const int b = blockIdx.x;
     int lo = 0, hi = np - 1;
     while (lo < hi) {  // last problem whose first block <= b
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(256) void k_forward_flat(const BatchProblem* __rest
     const int jj = ((xcd * P.quads_per_xcd + q) << 2) + (threadIdx.x >> 6);
     if (jj >= P.g.W) return;
     forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.muf, P.pdff, P.NE,
-                                             P.inv_xf, a, rg, jj, P.out, nullptr, steps[threadIdx.x >> 6]);
+                                             P.inv_xf, a, rg, jj, P.out, nullptr, &steps[threadIdx.x >> 6]);
 }
 
 int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks, int vox_is_f32, hipStream_t st) {
@@ -458,8 +458,8 @@ __global__ __launch_bounds__(256) void k_adjoint_coef(ProjGeom g, const double* 
                                                       int NE, double inv_xf,
                                                       int quads_per_xcd, const float* __restrict__ tg,
                                                       float* __restrict__ coef, double* __restrict__ loss) {
-    __shared__ StepRec step_tabs[4][64];
-    StepRec* steps = step_tabs[threadIdx.x >> 6];
+    __shared__ WaveLds step_tabs[4];
+    WaveLds* steps = &step_tabs[threadIdx.x >> 6];
     const Tile t = tile_of(g.nAng, g.W, quads_per_xcd);
     const int lane = threadIdx.x & 63;
     const int j = t.rg * 64 + lane;
@@ -518,8 +518,8 @@ __global__ __launch_bounds__(256) void k_adjoint_coef(ProjGeom g, const double* 
 
 __global__ __launch_bounds__(256) void k_adjoint_scatter(ProjGeom g, const double* __restrict__ tab, int quads_per_xcd,
                                                          const float* __restrict__ coef, float* __restrict__ gvox) {
-    __shared__ StepRec step_tabs[4][64];
-    StepRec* steps = step_tabs[threadIdx.x >> 6];
+    __shared__ WaveLds step_tabs[4];
+    WaveLds* steps = &step_tabs[threadIdx.x >> 6];
     const Tile t = tile_of(g.nAng, g.W, quads_per_xcd);
     const int lane = threadIdx.x & 63;
     const int j = t.rg * 64 + lane;

@@ -373,6 +373,18 @@ struct StepRec {
 };
 enum { STEP_HV = 1 };
 
+// A lane's held z-split piece (see traverse_table).
+struct HeldPiece {
+    double a, b;  // the piece [a, b]
+    int maj;      // its step's major-axis term
+};
+
+// Per-wave scratch of the table walk (LDS on the GPU): the step table and one held piece per lane.
+struct WaveLds {
+    StepRec steps[64];
+    HeldPiece held[64];
+};
+
 __host__ __device__ __forceinline__ bool make_step(const ProjGeom& g, const RayCtx& r, const ChargeC& c, int k,
                                                    StepRec& t) {
 #pragma clang fp contract(fast)
@@ -406,13 +418,28 @@ __host__ __device__ __forceinline__ bool make_step(const ProjGeom& g, const RayC
     return inprev;
 }
 
+// True when the condition holds on any lane of the wave (the host build walks one lane at a time).
+__host__ __device__ __forceinline__ bool any_lane(bool b) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __any(b);
+#else
+    return b;
+#endif
+}
+
 // The aligned-grid walk with the wave-uniform step data computed 64 steps at a time, one step per lane, into
 // a per-wave table (`tab`, 64 records; LDS on the GPU), and read back by every lane in step order.  Only
 // the per-lane z part (z crossing, crossing order, segments) stays in the sequential loop.  Requires the
 // lanes to share their xy path bit for bit (caller checks).
+// A step's segments are those of the wave-uniform minor-axis split (one or two, the same on every lane),
+// one of which a lane's z crossing may split again.  The z crossings are sparse per lane (a few per cent of
+// the lanes in a step) but, over 64 rows, present in almost every step: so the second piece of a z split is
+// not handed over in the step (which would run a mostly idle extra slot for the whole wave) but held by the
+// lane (in LDS), and the wave hands over all held pieces together when a lane that holds one meets its
+// next z crossing, and at the end.  The segments are the same; only the order of their sums changes.
 template <bool MUL24, class V>
 __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
-                                                        int kmax, V& vis, StepRec* tab) {
+                                                        int kmax, V& vis, WaveLds* lds) {
 #pragma clang fp contract(fast)
     const double lim_u = g.lim_u;
     const double inv_t = g.inv_t;
@@ -435,6 +462,14 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
     }
     int zmaprev = (int)(((b0 - r.su) * r.dz + r.sz) * inv_t);
     const double dt = r.dirz * g.t;
+    const Seg none{0, 0.0, false};
+    StepRec* tab = lds->steps;
+#ifdef __HIP_DEVICE_COMPILE__
+    HeldPiece* hp = &lds->held[threadIdx.x & 63];  // this lane's held piece (kept in LDS, not registers)
+#else
+    HeldPiece* hp = &lds->held[0];
+#endif
+    bool held = false;
 
     for (;;) {
         // steps k .. k+63: record and validity (k' <= kmax and the reference's loop still running)
@@ -475,6 +510,11 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
                 continue;
             }
             const bool hz = zma != zmaprev;
+            if (any_lane(hz && held)) {  // a lane that holds a piece meets a z crossing: hand over every held
+                const HeldPiece q = *hp;  // piece first (at the start of the step, where few values are live)
+                vis.step(make_seg<true, MUL24>(g, r, c, q.a, q.b, held, q.maj, true), none, none);
+                held = false;
+            }
             const bool hv = (flags & STEP_HV) != 0;
             const double uv = t.uv;
             double uz = uk;
@@ -482,19 +522,37 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
                 const double zb = (zma * g.t) - ((zk < 0 && r.dirz < 0) || (zk > 0 && r.dirz > 0) ? 0.0 : dt);
                 uz = (zb - r.sz) * r.duz + r.su;
             }
+            // Crossings ordered by distance from the end plane (the farther one comes first; equal distances
+            // drop both, as no branch of the reference matches), a crossing at the end plane splits nothing.
             const double chv = fabs(uk - uv), chz = fabs(uk - uz);
-            double e1 = uk, e2 = uk;
-            if (hv && hz) {
-                if (chz > chv) { e1 = uz; e2 = uv; }
-                else if (chz < chv) { e1 = uv; e2 = uz; }
-            } else if (hz && chz != 0) {
-                e1 = uz;
-            } else if (hv && chv != 0) {
-                e1 = uv;
+            const bool splitv = hv && chv != 0;  // wave-uniform: uk and uv are
+            const bool splitz = hz && chz != 0;
+            double a1 = splitv ? uv : uk;        // first piece [b0, a1]
+            double c1 = uk;                      // second piece [uv, c1] (splitv only)
+            bool onb = on && splitv;
+            double h0 = 0.0, h1 = 0.0;
+            bool hold = false;
+            if (splitv && splitz && chz == chv) {  // both crossings dropped: one segment
+                a1 = uk;
+                onb = false;
+            } else if (splitz) {
+                if (!splitv || chz > chv) {  // z crossing inside the first piece
+                    h0 = uz;
+                    h1 = a1;
+                    a1 = uz;
+                } else {  // inside the second
+                    h0 = uz;
+                    h1 = uk;
+                    c1 = uz;
+                }
+                hold = on;
             }
-            vis.step(make_seg<true, MUL24>(g, r, c, b0, e1, on, majterm, true),
-                     make_seg<true, MUL24>(g, r, c, e1, e2, on && (e2 != e1), majterm, true),
-                     make_seg<true, MUL24>(g, r, c, e2, uk, on && (uk != e2), majterm, true));
+            if (hold) {
+                *hp = HeldPiece{h0, h1, majterm};
+                held = true;
+            }
+            vis.step(make_seg<true, MUL24>(g, r, c, b0, a1, on, majterm, true),
+                     make_seg<true, MUL24>(g, r, c, uv, c1, onb, majterm, true), none);
             b0 = uk;
             zmaprev = zma;
         }
@@ -504,17 +562,21 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
         if (nsteps < 64) break;
         k += 64;
     }
+    if (any_lane(held)) {
+        const HeldPiece q = *hp;
+        vis.step(make_seg<true, MUL24>(g, r, c, q.a, q.b, held, q.maj, true), none, none);
+    }
 }
 
 // Walks the compat traversal of one ray (lanes of a wave in lockstep over plane steps).
 // `active`: lane holds a real detector pixel.  The visitor accumulates.
 template <class V>
 __host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
-                                                       int kmax, V& vis, StepRec* tab = nullptr) {
+                                                       int kmax, V& vis, WaveLds* lds = nullptr) {
     if (kmin > kmax) return;  // no lane of this wave can reach the volume's z band
-    if (g.aligned && tab) {  // the caller vouches that the lanes share their xy path
-        if (g.mul24) traverse_table<true>(g, r, active, kmin, kmax, vis, tab);
-        else traverse_table<false>(g, r, active, kmin, kmax, vis, tab);
+    if (g.aligned && lds) {  // the caller vouches that the lanes share their xy path
+        if (g.mul24) traverse_table<true>(g, r, active, kmin, kmax, vis, lds);
+        else traverse_table<false>(g, r, active, kmin, kmax, vis, lds);
     } else if (g.aligned) {
         if (g.mul24) traverse_steps<true, true>(g, r, active, kmin, kmax, vis);
         else traverse_steps<true, false>(g, r, active, kmin, kmax, vis);
@@ -551,12 +613,12 @@ __device__ __forceinline__ void share_xy_path(RayCtx& r) {
     r.duv = first_lane(r.duv);
 }
 
-// `tab`: this wave's 64-record step table in LDS.  Waves whose lanes all chose the same box entry (every
-// wave but those of rays through a box corner) share their whole xy path and walk with the step table;
-// the others walk per lane.
+// `lds`: this wave's step table and held pieces in LDS.  Waves whose lanes all chose the same box entry
+// (every wave but those of rays through a box corner) share their whole xy path and walk with the step
+// table; the others walk per lane.
 template <class V>
 __device__ __forceinline__ double traverse_wave(const ProjGeom& g, double c, double sn, double xc, double yc, int j,
-                                                int jj, bool active, V& vis, StepRec* tab) {
+                                                int jj, bool active, V& vis, WaveLds* lds) {
     RayCtx r;
     ray_setup(g, c, sn, xc, yc, active ? j : 0, jj, r);
     share_xy_path(r);
@@ -565,7 +627,7 @@ __device__ __forceinline__ double traverse_wave(const ProjGeom& g, double c, dou
     const int kmin = wave_min(klo);
     const int kmax = wave_max(khi);
     const bool same = __all((int)r.swap == __builtin_amdgcn_readfirstlane((int)r.swap));
-    traverse_lane(g, r, active, kmin, kmax, vis, same ? tab : nullptr);
+    traverse_lane(g, r, active, kmin, kmax, vis, same ? lds : nullptr);
     return r.leng;
 }
 }  // namespace vcbct
