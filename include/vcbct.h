@@ -122,7 +122,8 @@ int vcbct_material_table(const char* media_csv, const char* organs_csv, int rand
 
 /* Replaces the voxel loop of R/ :212-226 and the slice writer's layout (:242-274): labels u8
  * [nz][nrow][ncol] -> maps u16 [nz][6][nrow][ncol] (slice iz of dev_maps is the reference's per-slice
- * material file MD_<ncol>_<nrow>_<iz>.raw).  dev_lut [256][8] as vcbct_material_table writes it. */
+ * material file MD_<ncol>_<nrow>_<iz>.raw).  dev_lut [256][8] as vcbct_material_table writes it, 16-byte
+ * aligned (it is read one 16-byte entry at a time). */
 int vcbct_labels_to_maps(const uint8_t* dev_labels, int ncol, int nrow, int nz, const uint16_t* dev_lut,
                          uint16_t* dev_maps, void* stream);
 
