@@ -117,23 +117,35 @@ __global__ __launch_bounds__(256) void k_untranspose_f32(const float* __restrict
 template <class VoxT>
 struct AccumL;
 
+// u16 maps: the first gather of a step is software-pipelined.  A step issues it (lanes that charge nothing
+// read voxel 0, so the load is unconditional and the wait before its use can count it exactly) and adds
+// the previous step's, which has had a whole step of arithmetic to arrive; finish() adds the last.
+// (Pipelining the second slot too costs registers down to 5 waves per SIMD and was measured slower.)
 template <>
 struct AccumL<uint4> {
     const uint4* __restrict__ vox;
     int Nz;
     float L[VCBCT_MAX_MAT];
+    uint3 p0 = make_uint3(0, 0, 0);  // the previous step's first gather
+    float r0 = 0.f;                  // and its length (0: nothing charged)
     __device__ __forceinline__ void step(const Seg& s0, const Seg& s1, const Seg& s2) {
 #pragma clang fp contract(fast)
-        uint3 q0, q1, q2;  // read only where the matching segment is charged
-        if (s0.ok) q0 = ld(s0);
-        if (s1.ok) q1 = ld(s1);
-        if (s2.ok) q2 = ld(s2);
-        if (s0.ok) add(q0, (float)s0.rr);
+        const uint3 q0 = ld(s0.ok ? s0.cell : 0);
+        uint3 q1, q2;
+        if (s1.ok) q1 = ld(s1.cell);
+        if (s2.ok) q2 = ld(s2.cell);
+        add(p0, r0);
         if (s1.ok) add(q1, (float)s1.rr);
         if (s2.ok) add(q2, (float)s2.rr);
+        p0 = q0;
+        r0 = s0.ok ? (float)s0.rr : 0.f;
     }
-    __device__ __forceinline__ uint3 ld(const Seg& s) const {
-        const uint4 q = ldg(vox, (size_t)s.cell);
+    __device__ __forceinline__ void finish() {
+        add(p0, r0);
+        r0 = 0.f;
+    }
+    __device__ __forceinline__ uint3 ld(int cell) const {
+        const uint4 q = ldg(vox, (size_t)cell);
         return make_uint3(q.x, q.y, q.z);
     }
     __device__ __forceinline__ void add(const uint3& q, float rr) {
@@ -165,6 +177,7 @@ struct AccumL<float4> {
         if (s1.ok) add(q1, (float)s1.rr);
         if (s2.ok) add(q2, (float)s2.rr);
     }
+    __device__ __forceinline__ void finish() {}
     __device__ __forceinline__ Q ld(const Seg& s) const {
         const size_t i = 2 * (size_t)s.cell;
         const float4 b = ldg(vox, i + 1);
@@ -234,6 +247,7 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0f;
     const double leng = traverse_wave(g, c, sn, xc, yc, j, jj, active, acc, steps);
+    acc.finish();
     if (!active) return;
 
     if (MODE == VCBCT_MODE_LINEINT) {
