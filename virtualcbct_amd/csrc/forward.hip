@@ -19,6 +19,14 @@
 
 namespace vcbct {
 
+// Occupancy of the ray kernels.  Their step loops wait on dependent LDS reads, scalar broadcasts and
+// gathers, and more resident waves hide that better than spilling a few registers costs: on the
+// 512x512x128 / 360-angle workload, k_forward takes 47.1 ms at its natural 75 VGPRs (6 waves per SIMD),
+// 45.0 ms held to 7 waves and 43.1 ms held to 8 (64 VGPRs, 56 bytes per lane spilled to scratch).  The
+// persistent batch kernel is fastest held to 7.
+#define VCBCT_WAVES8 __attribute__((amdgpu_waves_per_eu(8, 8)))
+#define VCBCT_WAVES7 __attribute__((amdgpu_waves_per_eu(7, 8)))
+
 // ------------------------------------------------------------------------------------------------
 // Volume re-layout.  Reference u16 [z][mat][y][x] -> device [y][x][z][8] (16-byte voxel, all materials;
 // the 64 rows of a wave read one contiguous run of a z column).  Tiled through LDS: a block reads a
@@ -295,7 +303,7 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
 }
 
 template <int MODE, class VoxT>
-__global__ __launch_bounds__(256) void k_forward(ProjGeom g, const double* __restrict__ tab, const VoxT* __restrict__ vox,
+__global__ __launch_bounds__(256) VCBCT_WAVES8 void k_forward(ProjGeom g, const double* __restrict__ tab, const VoxT* __restrict__ vox,
                                                  const double* __restrict__ mu, const double* __restrict__ pdf,
                                                  const float* __restrict__ muf, const float* __restrict__ pdff, int NE,
                                                  double inv_xf, int a0, int nA, int quads_per_xcd,
@@ -325,7 +333,7 @@ __device__ __forceinline__ int find_problem(const BatchProblem* __restrict__ pr,
 }
 
 template <class VoxT>
-__global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __restrict__ pr, int np,
+__global__ __launch_bounds__(256) VCBCT_WAVES7 void k_forward_batch(const BatchProblem* __restrict__ pr, int np,
                                                        const int* __restrict__ xcd_total, int* __restrict__ counters) {
     __shared__ WaveLds steps[4];
     const int home = blockIdx.x & 7;
@@ -358,7 +366,7 @@ __global__ __launch_bounds__(256) void k_forward_batch(const BatchProblem* __res
 // count is a multiple of 8, so block b still runs on XCD b % 8 and the per-problem XCD-aware tile mapping
 // carries over).
 template <class VoxT>
-__global__ __launch_bounds__(256) void k_forward_flat(const BatchProblem* __restrict__ pr, int np) {
+__global__ __launch_bounds__(256) VCBCT_WAVES8 void k_forward_flat(const BatchProblem* __restrict__ pr, int np) {
     __shared__ WaveLds steps[4];
     const int b = blockIdx.x;
     int lo = 0, hi = np - 1;
@@ -466,7 +474,7 @@ struct ScatterG {
 };
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_adjoint_coef(ProjGeom g, const double* __restrict__ tab,
+__global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, const double* __restrict__ tab,
                                                       const float4* __restrict__ vox, const double* __restrict__ mu,
                                                       const double* __restrict__ pdf, const float* __restrict__ muf,
                                                       int NE, double inv_xf,
@@ -530,7 +538,7 @@ __global__ __launch_bounds__(256) void k_adjoint_coef(ProjGeom g, const double* 
     }
 }
 
-__global__ __launch_bounds__(256) void k_adjoint_scatter(ProjGeom g, const double* __restrict__ tab, int quads_per_xcd,
+__global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_scatter(ProjGeom g, const double* __restrict__ tab, int quads_per_xcd,
                                                          const float* __restrict__ coef, float* __restrict__ gvox) {
     __shared__ WaveLds step_tabs[4];
     WaveLds* steps = &step_tabs[threadIdx.x >> 6];
