@@ -115,18 +115,50 @@ def bench_fdk(line, inp, dev, stream, out):
         torch.cuda.synchronize()
         tf.append(e[0].elapsed_time(e[1]))
         tb.append(e[1].elapsed_time(e[2]))
+    # half-fan weighting + padding of the same projections (host wall clock: table upload, kernel, output
+    # allocation, synchronise)
+    import time
+    tw = []
+    for _ in range(nrep):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fdk.offset_weight_device(proj, inp["xs_fdk"], inp["ys_fdk"])
+        tw.append((time.perf_counter() - t0) * 1e3)
     from virtualcbct_amd.batch import autotune_log
     updates = w["R"] * w["R"] * w["Z"] * w["n_ang"]
     line["fdk"] = {"gups": updates / (np.median(tb) * 1e-3) / 1e9, "bp_ms": float(np.median(tb)),
-                   "filter_ms": float(np.median(tf)),
+                   "filter_ms": float(np.median(tf)), "offset_weight_ms": float(np.median(tw)),
                    "filter_rows_per_s": w["n_ang"] * w["H"] / (np.median(tf) * 1e-3),
                    "updates": updates, "shape": [w["Z"], w["R"], w["R"]],
                    "bp_roofline_frac": (16.0 * updates / (np.median(tb) * 1e-3) / 1e9) / HBM_PEAK_GBS,
                    "autotune": [l for l in autotune_log().strip().split("\n") if l.startswith("bp ")]}
     log(f"fdk: filter {np.median(tf):.3f} ms, backprojection {np.median(tb):.3f} ms, "
-        f"{line['fdk']['gups']:.1f} GUPS")
+        f"{line['fdk']['gups']:.1f} GUPS, offset weighting {np.median(tw):.3f} ms")
     plan.close()
     del filt, rec
+
+
+def bench_detector(line, dev, stream, e, out):
+    import torch
+    from virtualcbct_amd import detector
+    # detector model on the headline projections (360 x 400 x 400): Poisson counts of I0 = 1e5 quanta per
+    # open-field pixel plus a flat 5 % scatter field, normalised for the filter
+    sc = torch.full_like(out, 0.05)
+    noisy = torch.empty_like(out)
+    detector.detector_counts(out, sc, I0=1.0e5, seed=1, out=noisy)
+    torch.cuda.synchronize()
+    t = []
+    for r in range(5):
+        e[0].record(stream)
+        detector.detector_counts(out, sc, I0=1.0e5, seed=2 + r, out=noisy)
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        t.append(e[0].elapsed_time(e[1]))
+    ms = float(np.median(t))
+    line["detector"] = {"ms": ms, "pixels": out.numel(), "pixels_per_s": out.numel() / (ms * 1e-3), "I0": 1.0e5,
+                        "scatter": "flat 0.05"}
+    log(f"detector model: {ms:.3f} ms for {out.numel()} pixels")
+    del sc, noisy
 
 
 def bench_adjoint(line, inp, dev, stream, e):
@@ -400,7 +432,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline forward only (used under the profiler)")
-    ap.add_argument("--only", choices=["adjoint", "batched", "decompose", "fdk", "phantom", "iterative"], default=None,
+    ap.add_argument("--only", choices=["adjoint", "batched", "decompose", "fdk", "phantom", "iterative",
+                                                 "detector"], default=None,
                     help="run one secondary section alone and print its JSON (used under the profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (scale for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -437,11 +470,14 @@ def main():
     NE = len(inp["pdf"])
     stream = torch.cuda.current_stream()
 
-    if args.only == "fdk":  # projections of the headline workload, then the FDK plan (filter + backprojection)
+    if args.only in ("fdk", "detector"):  # projections of the headline workload, then the stage after them
         out = torch.empty(P.shape_proj, dtype=torch.float32, device=dev)
         P.forward(1000.0, out=out)
-        line = {"section": "fdk"}
-        bench_fdk(line, inp, dev, stream, out)
+        line = {"section": args.only}
+        if args.only == "fdk":
+            bench_fdk(line, inp, dev, stream, out)
+        else:
+            bench_detector(line, dev, stream, [torch.cuda.Event(enable_timing=True) for _ in range(2)], out)
         print(json.dumps(line), flush=True)
         return
     if args.only:
@@ -501,6 +537,7 @@ def main():
     if not args.no_extras:
         bench_fdk(line, inp, dev, stream, out)
         e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        bench_detector(line, dev, stream, e, out)
         bench_adjoint(line, inp, dev, stream, e)
 
     if not args.no_extras:

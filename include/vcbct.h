@@ -234,6 +234,35 @@ int vcbct_fdk_size_class(int n_proj, const double* host_xshift, char* out);
  * applies it.  Synchronises the stream (set-up call). */
 int vcbct_fdk_angle_mask(int n_proj, const double* host_angles_rad, int recon_size, double recon_scale,
                          const char* size_class, uint8_t* dev_mask, void* stream);
+/* Offset-detector (half-fan) weighting, the step that the panel-offset class and angle mask above are computed
+ * for (F/main.cpp:57) and that the reference never takes: no reference counterpart.  Each column of
+ * projection k gets w(t) = sin^2(pi/4 (1 + t/tau)) on |t| < tau, 0 below, 1 above, t its distance in pixels
+ * from the central ray (column W/2 + xshift[k]/(0.05*800/W) - 0.5, the backprojector's column rule,
+ * F/reconstructImage.cpp:83) counted positive towards the long side, tau = min(taper_px, W/2 - |shift|)
+ * (taper_px <= 0: the whole overlap), so that conjugate rays sum to one (G. Wang, Med. Phys. 29 (2002)
+ * 1634).  The weight acts on the filter's line integral q = -log p, doubled to keep a full 360-degree scan's
+ * normalisation (p^(2w)).  The panel is padded on the short side with P = round(2 max|shift|) columns of
+ * p = 1, so that the filtered projection's tails reach the voxels only the long side sees: dev_out is
+ * [n][H][W + P] (out_width = vcbct_fdk_half_fan_width()), and host_xshift_out / host_yshift_out (n each) put
+ * the central ray and the filter's row centre where they were.  Filter and backproject dev_out with these
+ * shifts, width W + P and det_pitch_iso_cm * (W + P) / W (the same pixel size).  dev_out may alias dev_proj
+ * only when P = 0.  Errors: shifts of both signs, a shift that leaves no overlap, W + P > 4096.
+ * Synchronises the stream. */
+int vcbct_fdk_half_fan_width(int proj_width, int n_proj, const double* host_xshift);
+int vcbct_fdk_offset_weight(int proj_height, int proj_width, int n_proj, const float* dev_proj,
+                            const double* host_xshift, const double* host_yshift, double taper_px, int out_width,
+                            float* dev_out, double* host_xshift_out, double* host_yshift_out, void* stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * Detector model (no reference counterpart: the stage between the projector and FDK that the reference
+ * leaves to other programs — its projection file names, F/txtfile, and the scatter U-Net,
+ * Unet_scatter.py:349-423, on either side).  For each of the n pixels (any flat layout, e.g. [n][H][W]):
+ * lambda = I0 * (primary[i] + scatter[i]) quanta (dev_scatter may be NULL), k ~ Poisson(lambda) from
+ * Philox4x32-10 keyed by (seed, i) — inversion below lambda = 10, Hoermann's PTRS rejection above — and
+ * dev_out[i] = k (raw != 0) or max(k, 1) / I0 (normalised intensity for vcbct_filter_sinogram).  The draw
+ * of pixel i depends only on (seed, i, lambda).  dev_out may alias dev_primary.  Errors: I0 not positive. */
+int vcbct_detector_counts(size_t n, const float* dev_primary, const float* dev_scatter, double I0, uint64_t seed,
+                          int raw, float* dev_out, void* stream);
 
 /* ------------------------------------------------------------------------------------------------
  * Batched launch (no reference counterpart): many forward problems — volumes of different slice counts,

@@ -1085,4 +1085,158 @@ int orc_angle_mask(int n, const double* angles, int R, double scale, const char*
     return 0;
 }
 
+
+/* Offset-detector (half-fan) weighting; no reference counterpart (the reference computes the angle mask
+ * above for it and never applies either).  Column col of a projection with panel shift xshift (cm at the
+ * detector, shift = xshift / (0.05 * 800 / W) pixels) sees the central ray at c = W/2 + shift - 0.5
+ * (F/reconstructImage.cpp:83); t = side * (col - c), side = -1 when shift > 0 (long side towards column 0),
+ * else +1; tau = min(taper, W/2 - |shift|) (taper <= 0: W/2 - |shift|).  Returns 2 w(t):
+ * 0 for t <= -tau, 2 for t >= tau, 2 sin^2(pi/4 (1 + t/tau)) between.  -1 when the shift leaves no overlap. */
+double orc_offset_exponent(int W, double xshift, double taper, int col) {
+    const double sh = xshift / (0.05 * 800 / W);
+    const double delta = W / 2.0 - fabs(sh);
+    if (!(delta > 0.0)) return -1.0;
+    const double c = W / 2.0 + sh - 0.5;
+    const double tau = taper > 0.0 ? fmin(taper, delta) : delta;
+    const double t = (sh > 0.0 ? -1.0 : 1.0) * ((double)col - c);
+    if (t <= -tau) return 0.0;
+    if (t >= tau) return 2.0;
+    const double s = sin(0.78539816339744830962 * (1.0 + t / tau));
+    return 2.0 * (s * s);
+}
+
+/* Padding of the half-fan panel: P = round(2 max |shift|) columns on the short side (after the real columns
+ * when the shifts are positive, before when negative).  Returns W + P, or -1 when the shifts change sign or
+ * one leaves no overlap. */
+int orc_half_fan_width(int W, int n, const double* xshift) {
+    double mx = 0.0;
+    int sg = 0;
+    for (int k = 0; k < n; k++) {
+        const double sh = xshift[k] / (0.05 * 800 / W);
+        if (!(W / 2.0 - fabs(sh) > 0.0)) return -1;
+        const int s = sh > 0.0 ? 1 : (sh < 0.0 ? -1 : 0);
+        if (s != 0 && sg != 0 && s != sg) return -1;
+        if (s != 0) sg = s;
+        if (fabs(sh) > mx) mx = fabs(sh);
+    }
+    return W + (int)lround(2.0 * mx);
+}
+
+/* p' = p^(2w) per real column (the weight on q = -log p, doubled for the 360-degree normalisation), written
+ * into the padded panel out [n][H][Wo] (padding columns 1); xs_out / ys_out: the shifts that keep the central
+ * ray's column (Wo/2 + shift' - 0.5 = c + off) and the filter's row centre (-Wo/2 - yshift' = -W/2 - yshift,
+ * in pixels) on the padded panel. */
+int orc_offset_weight(int n, int H, int W, const float* p, const double* xshift, const double* yshift, double taper,
+                      int Wo, float* out, double* xs_out, double* ys_out) {
+    if (orc_half_fan_width(W, n, xshift) != Wo) return -1;
+    const int P = Wo - W;
+    int neg = 0;
+    for (int k = 0; k < n; k++)
+        if (xshift[k] < 0.0) neg = 1;
+    const int off = neg ? P : 0;
+    const double pix = 0.05 * 800 / W, pix_o = 0.05 * 800 / Wo;
+    for (int k = 0; k < n; k++) {
+        xs_out[k] = (xshift[k] / pix + off - P / 2.0) * pix_o;
+        ys_out[k] = (yshift[k] / pix - P / 2.0) * pix_o;
+        for (int col = 0; col < Wo; col++) {
+            const int rc = col - off;
+            const double e = (rc >= 0 && rc < W) ? orc_offset_exponent(W, xshift[k], taper, rc) : -2.0;
+            for (int j = 0; j < H; j++) {
+                float v = 1.0f;
+                if (e > -2.0) {
+                    const double x = (double)p[((size_t)k * H + j) * W + rc];
+                    v = (float)(e == 2.0 ? x * x : (e == 0.0 ? 1.0 : pow(x, e)));
+                }
+                out[((size_t)k * H + j) * Wo + col] = v;
+            }
+        }
+    }
+    return 0;
+}
+
+
+/* ---- Detector model: scatter composition and quantum noise (no reference counterpart) ----------------
+ * The reference leaves the stage between the forward projector and FDK to other programs (projection file
+ * names in F/txtfile, the scatter U-Net Unet_scatter.py:349-423).  Restated here as the library defines
+ * it: lambda = I0 (primary + scatter) expected quanta per pixel; a Poisson count k drawn from a
+ * counter-based generator keyed by (seed, flat pixel index), so every pixel's draw is independent of the
+ * launch shape; output k (raw) or max(k, 1) / I0 (normalised intensity, zero counts floored at one quantum).
+ *
+ * Philox4x32-10 (J. Salmon et al., "Parallel random numbers: as easy as 1, 2, 3", SC 2011): ten rounds of
+ * the two 32 x 32 -> 64 multiplies by 0xD2511F53 and 0xCD9E8D57, key bumped by 0x9E3779B9 / 0xBB67AE85. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* 53-bit uniform in [0, 1) from two words */
+static double orc_u53(uint32_t lo, uint32_t hi) {
+    return (double)((((uint64_t)hi << 32) | lo) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* Poisson(lambda) for the pixel with flat index i.  Block b = 0, 1, ... of the pixel's stream is Philox of
+ * counter (i_lo, i_hi, b, 0) under key (seed_lo, seed_hi); each block gives two uniforms (words 0-1, 2-3).
+ * lambda < 10: inversion by sequential search on the first uniform (capped at k = 256).
+ * lambda >= 10: PTRS transformed rejection (W. Hoermann, Insurance: Math. Econ. 12 (1993) 39-45), one block
+ * per trial (U from words 0-1, V from words 2-3), at most 64 trials (then floor(lambda + 0.5)). */
+long long orc_poisson(double lam, uint64_t i, uint64_t seed) {
+    if (!(lam > 0.0)) return 0;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t ctr[4] = {(uint32_t)i, (uint32_t)(i >> 32), 0u, 0u}, r[4];
+    if (lam < 10.0) {
+        orc_philox4x32_10(ctr, key, r);
+        const double u = orc_u53(r[0], r[1]);
+        double p = exp(-lam), F = p;
+        long long k = 0;
+        while (u > F && k < 256) {
+            k++;
+            p = p * lam / (double)k;
+            F = F + p;
+        }
+        return k;
+    }
+    const double slam = sqrt(lam), loglam = log(lam);
+    const double b = 0.931 + 2.53 * slam;
+    const double a = -0.059 + 0.02483 * b;
+    const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    for (uint32_t t = 0; t < 64; t++) {
+        ctr[2] = t;
+        orc_philox4x32_10(ctr, key, r);
+        const double U = orc_u53(r[0], r[1]) - 0.5;
+        const double V = orc_u53(r[2], r[3]);
+        const double us = 0.5 - fabs(U);
+        const double k = floor((2.0 * a / us + b) * U + lam + 0.43);
+        if (us >= 0.07 && V <= vr) return (long long)k;
+        if (k < 0.0 || (us < 0.013 && V > us)) continue;
+        if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lam + k * loglam - lgamma(k + 1.0))
+            return (long long)k;
+    }
+    return (long long)floor(lam + 0.5);
+}
+
+/* out[i] = k (raw != 0) or max(k, 1) / I0, k ~ Poisson(I0 (primary[i] + scatter[i])) (scatter may be NULL). */
+int orc_detector_counts(size_t n, const float* primary, const float* scatter, double I0, uint64_t seed, int raw,
+                        float* out) {
+    for (size_t i = 0; i < n; i++) {
+        const double lam = I0 * ((double)primary[i] + (scatter ? (double)scatter[i] : 0.0));
+        const long long k = orc_poisson(lam, i, seed);
+        out[i] = raw ? (float)k : (float)((double)(k < 1 ? 1 : k) / I0);
+    }
+    return 0;
+}
+
 }  // extern "C"

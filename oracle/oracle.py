@@ -521,6 +521,65 @@ def angle_mask(angles_rad, R, scale, cls):
     return m
 
 
+def offset_exponents(W, xshift, taper=0.0):
+    """[n][W] exponents 2 w of the offset-detector weighting (orc_offset_exponent); -1 where no overlap."""
+    xs = np.atleast_1d(np.asarray(xshift, np.float64))
+    f = lib().orc_offset_exponent
+    f.restype = C.c_double
+    return np.array([[f(int(W), C.c_double(x), C.c_double(taper), c) for c in range(W)] for x in xs])
+
+
+def half_fan_width(W, xshift):
+    xs = np.ascontiguousarray(xshift, np.float64)
+    return int(lib().orc_half_fan_width(int(W), len(xs), _p(xs, _D)))
+
+
+def offset_weight(p, xshift, yshift, taper=0.0):
+    """(padded panel [n][H][W + P], xshift', yshift') of the half-fan weighting (orc_offset_weight)."""
+    p = np.ascontiguousarray(p, np.float32)
+    n, H, W = p.shape
+    xs = np.ascontiguousarray(xshift, np.float64)
+    ys = np.ascontiguousarray(yshift, np.float64)
+    Wo = half_fan_width(W, xs)
+    if Wo < 0:
+        raise ValueError("offset weighting: shifts of both signs, or one leaves no overlap")
+    out = np.empty((n, H, Wo), np.float32)
+    xo, yo = np.empty(n), np.empty(n)
+    rc = lib().orc_offset_weight(n, H, W, _p(p, _F), _p(xs, _D), _p(ys, _D), C.c_double(taper), Wo, _p(out, _F),
+                                 _p(xo, _D), _p(yo, _D))
+    if rc != 0:
+        raise ValueError("offset weighting failed")
+    return out, xo, yo
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 of one 4-word counter under a 2-word key (orc_philox4x32_10)."""
+    c = (C.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    k = (C.c_uint32 * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return [int(x) for x in o]
+
+
+def poisson(lam, index, seed):
+    """The detector model's Poisson draw for one pixel (orc_poisson)."""
+    f = lib().orc_poisson
+    f.restype = C.c_longlong
+    f.argtypes = [C.c_double, C.c_uint64, C.c_uint64]
+    return int(f(float(lam), int(index), int(seed)))
+
+
+def detector_counts(primary, scatter, I0, seed, raw=False):
+    """Poisson counts of I0 (primary + scatter) per pixel, raw or as max(k, 1) / I0 (orc_detector_counts)."""
+    p = np.ascontiguousarray(primary, np.float32)
+    s = None if scatter is None else np.ascontiguousarray(scatter, np.float32)
+    out = np.empty_like(p)
+    f = lib().orc_detector_counts
+    f.argtypes = [C.c_size_t, _F, _F, C.c_double, C.c_uint64, C.c_int, _F]
+    f(p.size, _p(p, _F), _p(s, _F), float(I0), int(seed), int(bool(raw)), _p(out, _F))
+    return out
+
+
 def ref_angle_mask(angles_rad, R, scale, cls):
     a = np.ascontiguousarray(angles_rad, np.float64)
     m = np.empty((len(a), R, R), np.uint8)

@@ -76,6 +76,11 @@ SIGNATURES = {
                                 c_void_p, c_void_p]),
     "vcbct_fdk_size_class": (c_int, [c_int, P_D, C.c_char_p]),
     "vcbct_fdk_angle_mask": (c_int, [c_int, P_D, c_int, c_double, c_char_p, c_void_p, c_void_p]),
+    "vcbct_fdk_half_fan_width": (c_int, [c_int, c_int, P_D]),
+    "vcbct_fdk_offset_weight": (c_int, [c_int, c_int, c_int, c_void_p, P_D, P_D, c_double, c_int, c_void_p, P_D, P_D,
+                                        c_void_p]),
+    "vcbct_detector_counts": (c_int, [C.c_size_t, c_void_p, c_void_p, c_double, C.c_uint64, c_int, c_void_p,
+                                      c_void_p]),
     "vcbct_projector_logproj": (c_int, [c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_projector_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_projector_logproj_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p]),

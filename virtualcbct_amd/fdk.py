@@ -122,6 +122,50 @@ def prepareAngles(anglesRad, reconSize, reconScale, SMLClassString, stream=None)
     return out
 
 
+def half_fan_width(projWidth, xshift):
+    """Columns of the padded half-fan panel: W + round(2 max |shift|) (vcbct_fdk_half_fan_width)."""
+    xs = np.ascontiguousarray(xshift, np.float64)
+    w = lib().vcbct_fdk_half_fan_width(int(projWidth), len(xs), ptr(xs, P_D))
+    if w < 0:
+        check(w)
+    return w
+
+
+def offset_weight_device(proj, xshift, yshift, taper_px=0.0, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, stream=None):
+    """Offset-detector (half-fan) weighting of device projections [n][H][W] before the filter (no reference
+    counterpart: the step that the size class and angle mask above are computed for, F/main.cpp:57).  Column
+    weights w(t) = sin^2(pi/4 (1 + t/tau)) across the overlap, 0 / 1 outside it, with t the column's distance
+    from the central ray (positive towards the long side) and tau = min(taper_px, W/2 - |shift|); conjugate
+    rays sum to one.  Acts on the line integral (p ** (2 w)), and pads the short side with columns of p = 1 so
+    the panel is symmetric about the central ray.
+
+    Returns (proj_padded [n][H][W + P], xshift_padded, yshift_padded, pitch_iso_padded): filter and
+    backproject the padded panel with these (same pixel size, same central ray and row centre)."""
+    torch = _torch()
+    if proj.dtype != torch.float32 or not proj.is_cuda or proj.dim() != 3 or not proj.is_contiguous():
+        raise ValueError("proj must be a contiguous float32 CUDA tensor [n][H][W]")
+    n, H, W = proj.shape
+    xs = np.ascontiguousarray(xshift, np.float64)
+    ys = np.ascontiguousarray(yshift, np.float64)
+    if xs.shape != (n,) or ys.shape != (n,):
+        raise ValueError(f"xshift and yshift must have {n} entries")
+    Wo = half_fan_width(W, xs)
+    out = torch.empty((n, H, Wo), dtype=torch.float32, device=proj.device)
+    xo, yo = np.empty(n), np.empty(n)
+    check(lib().vcbct_fdk_offset_weight(int(H), int(W), int(n), C.c_void_p(proj.data_ptr()), ptr(xs, P_D),
+                                        ptr(ys, P_D), float(taper_px), int(Wo), C.c_void_p(out.data_ptr()),
+                                        ptr(xo, P_D), ptr(yo, P_D), _stream(stream)))
+    return out, xo, yo, DETECTOR_PITCH_CM_AT_ISO_CENT * Wo / W
+
+
+def fdk_half_fan_device(proj, zstart, reconStep, reconScale, reconSize, reconSlices, xfactor, anglesRad, xshift,
+                        yshift, taper_px=0.0, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, stream=None):
+    """fdk_device for a half-fan (offset-panel) scan: weighting and padding, then filter + backprojection."""
+    p, xs, ys, pit = offset_weight_device(proj, xshift, yshift, taper_px, DETECTOR_PITCH_CM_AT_ISO_CENT, stream)
+    return fdk_device(p, zstart, reconStep, reconScale, reconSize, reconSlices, xfactor, anglesRad, xs, ys, pit,
+                      stream=stream)
+
+
 def to_short(recon):
     """(short) truncation toward zero of the FP volume (F/main.cpp:179-185)."""
     r = np.asarray(recon)
