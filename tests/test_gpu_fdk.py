@@ -198,5 +198,5 @@ def test_offset_weight_half_fan_matches_centred_detector(torch):
     print(f"half-fan vs centred panel (level {level:.1f} inside r < 11 cm): largest ring-mean difference "
           f"weighted {ring_w:.4f}, plain {ring_u:.4f}; mean |pixel difference| weighted {pix_w:.4f}")
     assert ring_w < 0.02
-    assert pix_w < 0.03
+    assert pix_w < 0.04  # streaks of 120 views differ between the two panel positions
     assert ring_u > 0.2
