@@ -18,6 +18,7 @@
 #ifndef VCBCT_H
 #define VCBCT_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus

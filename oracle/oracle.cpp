@@ -1187,11 +1187,23 @@ static double orc_u53(uint32_t lo, uint32_t hi) {
     return (double)((((uint64_t)hi << 32) | lo) >> 11) * (1.0 / 9007199254740992.0);
 }
 
+/* log k! for integral k >= 0: exact factorials below 10, Stirling's series to the 1/x^5 term above (x = k + 1;
+ * the omitted terms are below 2e-12 of log k!). */
+double orc_log_factorial(double k) {
+    if (k < 10.0) {
+        static const double f[10] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0, 40320.0, 362880.0};
+        return log(f[(int)k]);
+    }
+    const double x = k + 1.0, r = 1.0 / x, r2 = r * r;
+    return (x - 0.5) * log(x) - x + 0.91893853320467274178 + r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0)));
+}
+
 /* Poisson(lambda) for the pixel with flat index i.  Block b = 0, 1, ... of the pixel's stream is Philox of
  * counter (i_lo, i_hi, b, 0) under key (seed_lo, seed_hi); each block gives two uniforms (words 0-1, 2-3).
  * lambda < 10: inversion by sequential search on the first uniform (capped at k = 256).
  * lambda >= 10: PTRS transformed rejection (W. Hoermann, Insurance: Math. Econ. 12 (1993) 39-45), one block
- * per trial (U from words 0-1, V from words 2-3), at most 64 trials (then floor(lambda + 0.5)). */
+ * per trial (U from words 0-1, V from words 2-3), at most 64 trials (then floor(lambda + 0.5)); the full test
+ * as log(V / alpha / (a/us^2 + b)) <= k log(lambda) - lambda - log k!. */
 long long orc_poisson(double lam, uint64_t i, uint64_t seed) {
     if (!(lam > 0.0)) return 0;
     const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
@@ -1222,7 +1234,7 @@ long long orc_poisson(double lam, uint64_t i, uint64_t seed) {
         const double k = floor((2.0 * a / us + b) * U + lam + 0.43);
         if (us >= 0.07 && V <= vr) return (long long)k;
         if (k < 0.0 || (us < 0.013 && V > us)) continue;
-        if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lam + k * loglam - lgamma(k + 1.0))
+        if (log(V * invalpha / (a / (us * us) + b)) <= -lam + k * loglam - orc_log_factorial(k))
             return (long long)k;
     }
     return (long long)floor(lam + 0.5);

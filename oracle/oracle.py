@@ -561,6 +561,14 @@ def philox4x32_10(ctr, key):
     return [int(x) for x in o]
 
 
+def log_factorial(k):
+    """log k! as the detector model's sampler forms it (orc_log_factorial)."""
+    f = lib().orc_log_factorial
+    f.restype = C.c_double
+    f.argtypes = [C.c_double]
+    return float(f(float(k)))
+
+
 def poisson(lam, index, seed):
     """The detector model's Poisson draw for one pixel (orc_poisson)."""
     f = lib().orc_poisson

@@ -21,6 +21,12 @@ def test_philox_known_answers(ctr, key, want):
     assert O.philox4x32_10(ctr, key) == want
 
 
+def test_log_factorial_against_lgamma():
+    """The sampler's log k! (factorial table below 10, Stirling's series above) against the C library's lgamma."""
+    for k in list(range(0, 40)) + [99, 1000, 12345, 10 ** 6, 3 * 10 ** 7]:
+        assert O.log_factorial(k) == pytest.approx(math.lgamma(k + 1.0), rel=1e-11, abs=1e-14), k
+
+
 def _counts(lam, n, seed=3):
     prim = np.full(n, lam, np.float32)
     return O.detector_counts(prim, None, 1.0, seed, raw=True).astype(np.float64)

@@ -11,10 +11,14 @@
 // Sampler: lambda < 10, inversion by sequential search on the first uniform (k capped at 256); lambda >= 10,
 // PTRS transformed rejection (W. Hoermann, Insurance: Math. Econ. 12 (1993) 39-45), one block per trial, at
 // most 64 trials (acceptance is above 0.9 per trial; the cap only bounds the loop: floor(lambda + 0.5)).
-// FP64 throughout; exp, log and lgamma are the device's FP64 functions, so a draw can differ from a host
-// restatement only where a comparison falls within an ulp or two of its threshold.
+// The rejection test compares log(V alpha^-1 / (a/us^2 + b)) with log of the pmf, k log(lambda) - lambda -
+// log k!, with log k! exact below k = 10 (a table of factorials) and Stirling's series to the 1/x^5 term
+// above (x = k + 1; the next term is below 2e-12 of log k!).  FP64 throughout; exp and log are the device's
+// FP64 functions, so a draw can differ from a host restatement only where a comparison falls within an ulp
+// or two of its threshold.
 //
-// Bound: FP64 issue (Philox's 80 integer operations and the sampler's transcendental functions per pixel),
+// Bound: instruction issue (Philox's 20 integer multiplies per block, the sampler's FP64 square root,
+// divisions and logarithms),
 // 8-12 B of HBM traffic per pixel.
 #include <hip/hip_runtime.h>
 
@@ -50,6 +54,17 @@ __device__ __forceinline__ double u53(uint32_t lo, uint32_t hi) {
     return (double)((((uint64_t)hi << 32) | lo) >> 11) * (1.0 / 9007199254740992.0);
 }
 
+// log k! (k >= 0, integral): the factorial table below 10, Stirling's series above.
+__device__ __forceinline__ double log_factorial(double k) {
+    if (k < 10.0) {
+        const double f[10] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0, 40320.0, 362880.0};
+        return log(f[(int)k]);
+    }
+    const double x = k + 1.0, r = 1.0 / x, r2 = r * r;
+    return (x - 0.5) * log(x) - x + 0.91893853320467274178 +
+           r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0)));
+}
+
 __device__ double poisson_draw(double lam, uint64_t i, uint32_t k0, uint32_t k1) {
     if (!(lam > 0.0)) return 0.0;
     const uint32_t i0 = (uint32_t)i, i1 = (uint32_t)(i >> 32);
@@ -65,10 +80,9 @@ __device__ double poisson_draw(double lam, uint64_t i, uint32_t k0, uint32_t k1)
         }
         return (double)k;
     }
-    const double slam = sqrt(lam), loglam = log(lam);
+    const double slam = sqrt(lam);
     const double b = 0.931 + 2.53 * slam;
     const double a = -0.059 + 0.02483 * b;
-    const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
     const double vr = 0.9277 - 3.6224 / (b - 2.0);
     for (uint32_t t = 0; t < 64; ++t) {
         const Philox4 r = philox4x32_10(i0, i1, t, 0u, k0, k1);
@@ -76,9 +90,11 @@ __device__ double poisson_draw(double lam, uint64_t i, uint32_t k0, uint32_t k1)
         const double V = u53(r.w[2], r.w[3]);
         const double us = 0.5 - fabs(U);
         const double k = floor((2.0 * a / us + b) * U + lam + 0.43);
-        if (us >= 0.07 && V <= vr) return k;
+        if (us >= 0.07 && V <= vr) return k;  // the fast acceptance (most trials)
         if (k < 0.0 || (us < 0.013 && V > us)) continue;
-        if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lam + k * loglam - lgamma(k + 1.0)) return k;
+        // the full test; its constants are formed here, off the fast path (same values)
+        const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+        if (log(V * invalpha / (a / (us * us) + b)) <= -lam + k * log(lam) - log_factorial(k)) return k;
     }
     return floor(lam + 0.5);
 }
