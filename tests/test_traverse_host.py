@@ -50,7 +50,8 @@ ANGLES = [-170.0001, -90.00001, -40.00001, 9.99999, 45.0, 59.99999, 133.3, 179.9
     (32, 8, 0.5, 0.5, 24, 24, 0.3, 0.0123),
     (32, 8, 0.5, 0.5, 24, 24, 0.0, 0.0),       # centre column on the rotation axis: exact y = 0 ties
     (64, 16, 0.25, 0.3, 80, 70, -0.7, 0.3),    # z_reg shift, non-square detector
-    (48, 100, 0.3, 0.05, 130, 20, 0.0, 0.0),   # many thin slices: many z crossings per ray
+    (48, 100, 0.3, 0.05, 130, 20, 0.0, -1.6923),  # many thin slices: many z crossings per ray (the panel
+    #                                              shift puts the central ray, at column H/2 + shift, on the panel)
     (41, 9, 0.37, 0.21, 33, 29, 0.11, -0.2),   # odd sizes, box half-width not on the plane grid
     (16, 4, 0.8, 1.2, 14, 14, 0.0, 0.01),      # planes offset by s/2 from the voxel grid: every midpoint a tie
 ])
