@@ -225,6 +225,19 @@ int vcbct_fdk_plan_backproject(vcbct_fdk_plan* p, const float* dev_filtered, flo
 int vcbct_fdk_plan_execute(vcbct_fdk_plan* p, const float* dev_proj, float* dev_filtered, float* dev_recon,
                            void* stream);
 int vcbct_fdk_plan_slices_per_thread(const vcbct_fdk_plan* p);
+/* Half-fan FDK plan (no reference counterpart; the weighting of vcbct_fdk_offset_weight below, inside the
+ * plan): the plan runs on the padded panel (its width: vcbct_fdk_plan_width), and its filter call weights and
+ * pads the caller's [n][H][W] projections into a buffer the plan owns before filtering, so dev_filtered is
+ * [n][H][vcbct_fdk_plan_width(p)].  Execution still allocates nothing and does not synchronise (capturable);
+ * calls on one half-fan plan must not overlap (they share its padded buffer).  Errors as
+ * vcbct_fdk_offset_weight. */
+int vcbct_fdk_plan_create_half_fan(vcbct_fdk_plan** out, int proj_height, int proj_width, int n_proj,
+                                   int recon_size, int recon_slices, int zstart_cm, double recon_step,
+                                   double recon_scale, int xfactor, const double* host_angles_rad,
+                                   const double* host_xshift, const double* host_yshift, double det_pitch_iso_cm,
+                                   double taper_px, int device);
+/* Columns of the panel the plan filters and backprojects (proj_width, or the padded width of a half-fan plan). */
+int vcbct_fdk_plan_width(const vcbct_fdk_plan* p);
 
 /* Panel-offset class of an FDK angle list (F/projectionDataMVkV.cpp:277-281): "S" when the mean panel x
  * shift of the used projections is below 6, "L" above 15, else "M".  out: at least 2 chars. */

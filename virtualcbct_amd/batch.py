@@ -28,6 +28,9 @@ class Batch:
         self._keep += [projector, out]
 
     def add_fdk(self, plan, proj, filtered, recon):
+        plan._check(proj, plan.shape_proj, "proj")
+        plan._check(filtered, plan.shape_filtered, "filtered")
+        plan._check(recon, plan.shape_recon, "recon")
         check(lib().vcbct_batch_add_fdk(self.h, plan.h, C.c_void_p(proj.data_ptr()), C.c_void_p(filtered.data_ptr()),
                                         C.c_void_p(recon.data_ptr()), recon.numel() * 4))
         self._keep += [plan, proj, filtered, recon]

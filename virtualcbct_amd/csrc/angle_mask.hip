@@ -52,13 +52,6 @@ __global__ __launch_bounds__(256) void k_fill_u8(uint8_t* __restrict__ p, size_t
 // the voxels that only the long side sees), so the panel is also padded on the short side with P =
 // round(2 max|shift|) columns of p = 1 (no attenuation), making it symmetric about the central ray; the
 // filter and the backprojector then run on the W + P columns with the same pixel size.
-struct OffsetPar {
-    double c;     // column of the central ray (unpadded panel)
-    double tau;   // transition half-width (pixels)
-    double side;  // +1: the long side is towards higher columns; -1: towards lower
-    double pad;
-};
-
 __host__ __device__ inline double offset_exponent(const OffsetPar& q, int col) {
     const double t = q.side * ((double)col - q.c);
     if (t <= -q.tau) return 0.0;
@@ -180,6 +173,42 @@ extern "C" int vcbct_fdk_angle_mask(int n, const double* angles, int R, double s
     return rc;
 }
 
+namespace vcbct {
+
+int half_fan_params(int W, int n, const double* xshift, const double* yshift, double taper_px, int& Wo, int& off,
+                    OffsetPar* par, double* xs_out, double* ys_out) {
+    int P, side;
+    if (!half_fan_pad(W, n, xshift, P, side))
+        return fail(VCBCT_EINVAL, "half-fan weighting: the shifts change sign or one leaves no overlap");
+    Wo = W + P;
+    if (Wo > 4096) return fail(VCBCT_EINVAL, "half-fan weighting: padded width %d above 4096", Wo);
+    off = side < 0 ? P : 0;
+    const double pix = 0.05 * 800 / W, pix_o = 0.05 * 800 / Wo;
+    for (int k = 0; k < n; ++k) {
+        const double sh = xshift[k] / pix;       // panel shift in pixels
+        const double delta = W / 2.0 - fabs(sh);  // central ray to the short edge
+        par[k].c = W / 2.0 + sh - 0.5;
+        par[k].tau = taper_px > 0.0 ? fmin(taper_px, delta) : delta;
+        par[k].side = sh > 0.0 ? -1.0 : 1.0;
+        par[k].pad = 0.0;
+        // the same central-ray column on the padded panel (Wo/2 + shift' - 0.5 = c + off), and the row
+        // centre the filter takes from the panel width (-Wo/2 - yshift' = -W/2 - yshift) kept
+        xs_out[k] = (sh + off - P / 2.0) * pix_o;
+        ys_out[k] = (yshift[k] / pix - P / 2.0) * pix_o;
+    }
+    return 0;
+}
+
+int launch_offset_weight(const float* in, float* out, int H, int W, int Wo, int off, int n, const OffsetPar* d_par,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(k_offset_weight, dim3((unsigned)((H + OW_ROWS - 1) / OW_ROWS), (unsigned)n), dim3(256),
+                       (size_t)W * sizeof(double), st, in, out, H, W, Wo, off, d_par);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace vcbct
+
 extern "C" int vcbct_fdk_half_fan_width(int W, int n, const double* xshift) {
     if (W < 1 || n < 1 || !xshift) return fail(VCBCT_EINVAL, "fdk_half_fan_width: bad arguments");
     int P, side;
@@ -194,42 +223,25 @@ extern "C" int vcbct_fdk_offset_weight(int H, int W, int n, const float* dev_pro
     if (H < 1 || W < 1 || W > 4096 || n < 1 || n > 65535 || !dev_proj || !dev_out || !xshift || !yshift ||
         !xshift_out || !yshift_out)
         return fail(VCBCT_EINVAL, "fdk_offset_weight: bad arguments");
-    int P, side;
-    if (!half_fan_pad(W, n, xshift, P, side))
-        return fail(VCBCT_EINVAL, "fdk_offset_weight: the shifts change sign or one leaves no overlap");
-    const int Wo = W + P;
+    std::vector<OffsetPar> par(n);
+    int Wo, off;
+    int rc = half_fan_params(W, n, xshift, yshift, taper_px, Wo, off, par.data(), xshift_out, yshift_out);
+    if (rc) return rc;
     if (out_width != Wo)
         return fail(VCBCT_EINVAL, "fdk_offset_weight: out_width %d, the padded panel has %d columns", out_width, Wo);
-    if (Wo > 4096) return fail(VCBCT_EINVAL, "fdk_offset_weight: padded width %d above 4096", Wo);
-    if ((const void*)dev_proj == (const void*)dev_out && P > 0)
+    if ((const void*)dev_proj == (const void*)dev_out && Wo != W)
         return fail(VCBCT_EINVAL, "fdk_offset_weight: in place only when nothing is padded");
-    const int off = side < 0 ? P : 0;
-    std::vector<OffsetPar> par(n);
-    const double pix = 0.05 * 800 / W, pix_o = 0.05 * 800 / Wo;
-    for (int k = 0; k < n; ++k) {
-        const double sh = xshift[k] / pix;       // panel shift in pixels
-        const double delta = W / 2.0 - fabs(sh);  // central ray to the short edge
-        par[k].c = W / 2.0 + sh - 0.5;
-        par[k].tau = taper_px > 0.0 ? fmin(taper_px, delta) : delta;
-        par[k].side = sh > 0.0 ? -1.0 : 1.0;
-        par[k].pad = 0.0;
-        // the same central-ray column on the padded panel (Wo/2 + shift' - 0.5 = c + off), and the row
-        // centre the filter takes from the panel width (-Wo/2 - yshift' = -W/2 - yshift) kept
-        xshift_out[k] = (sh + off - P / 2.0) * pix_o;
-        yshift_out[k] = (yshift[k] / pix - P / 2.0) * pix_o;
-    }
     hipStream_t st = (hipStream_t)stream;
     void* d_par = nullptr;
     if (dev_malloc(&d_par, (size_t)n * sizeof(OffsetPar)) != hipSuccess)
         return fail(VCBCT_ENOMEM, "fdk_offset_weight: device allocation failed");
     hipError_t e = hipMemcpy(d_par, par.data(), (size_t)n * sizeof(OffsetPar), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_offset_weight, dim3((unsigned)((H + OW_ROWS - 1) / OW_ROWS), (unsigned)n), dim3(256),
-                           (size_t)W * sizeof(double), st, dev_proj, dev_out, H, W, Wo, off, (const OffsetPar*)d_par);
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        rc = launch_offset_weight(dev_proj, dev_out, H, W, Wo, off, n, (const OffsetPar*)d_par, st);
+        if (!rc) e = hipStreamSynchronize(st);
     }
     (void)hipFree(d_par);
+    if (rc) return rc;
     if (e != hipSuccess) return fail(VCBCT_EHIP, "fdk_offset_weight: %s", hipGetErrorString(e));
     return 0;
 }

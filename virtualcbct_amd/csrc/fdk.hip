@@ -420,6 +420,11 @@ struct vcbct_fdk_plan {
     vcbct::FilterPlan* fp;
     float* d_ctab;
     double* d_atab;
+    // half-fan plans: W is the padded panel; the caller's projections have hf_W columns, weighted and padded
+    // into d_pad (the plan's own [n][H][W] buffer) before the filter
+    int hf_W, hf_off;
+    float* d_pad;
+    vcbct::OffsetPar* d_opar;
 };
 
 namespace vcbct {
@@ -521,9 +526,46 @@ extern "C" int vcbct_fdk_plan_create(vcbct_fdk_plan** out, int H, int W, int n, 
     return 0;
 }
 
+extern "C" int vcbct_fdk_plan_create_half_fan(vcbct_fdk_plan** out, int H, int W, int n, int R, int Z, int zstart,
+                                              double step, double scale, int xfactor, const double* angles,
+                                              const double* xshift, const double* yshift, double pitch_iso,
+                                              double taper_px, int device) {
+    if (!out) return fail(VCBCT_EINVAL, "fdk_plan_create_half_fan: out is NULL");
+    *out = nullptr;
+    if (H < 2 || W < 2 || W > 4096 || n < 2 || n > 65535 || !angles || !xshift || !yshift)
+        return fail(VCBCT_EINVAL, "fdk_plan_create_half_fan: bad arguments");
+    std::vector<OffsetPar> par(n);
+    std::vector<double> xs(n), ys(n);
+    int Wo, off;
+    int rc = half_fan_params(W, n, xshift, yshift, taper_px, Wo, off, par.data(), xs.data(), ys.data());
+    if (rc) return rc;
+    // the plan proper runs on the padded panel: same pixel size, shifts that keep the central ray
+    vcbct_fdk_plan* p = nullptr;
+    rc = vcbct_fdk_plan_create(&p, H, Wo, n, R, Z, zstart, step, scale, xfactor, angles, xs.data(), ys.data(),
+                               pitch_iso * Wo / W, device);
+    if (rc) return rc;
+    p->hf_W = W;
+    p->hf_off = off;
+    if (dev_malloc((void**)&p->d_opar, (size_t)n * sizeof(OffsetPar)) != hipSuccess ||
+        dev_malloc((void**)&p->d_pad, (size_t)n * H * Wo * sizeof(float)) != hipSuccess)
+        rc = fail(VCBCT_ENOMEM, "fdk_plan_create_half_fan: device allocation of the padded panel failed");
+    if (!rc && hipMemcpy(p->d_opar, par.data(), (size_t)n * sizeof(OffsetPar), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(VCBCT_EHIP, "fdk_plan_create_half_fan: upload failed");
+    if (rc) {
+        vcbct_fdk_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return 0;
+}
+
+extern "C" int vcbct_fdk_plan_width(const vcbct_fdk_plan* p) { return p ? p->W : -1; }
+
 extern "C" void vcbct_fdk_plan_destroy(vcbct_fdk_plan* p) {
     if (!p) return;
     (void)hipSetDevice(p->device);
+    (void)hipFree(p->d_pad);
+    (void)hipFree(p->d_opar);
     (void)hipFree(p->d_ctab);
     (void)hipFree(p->d_atab);
     free(p);
@@ -531,6 +573,12 @@ extern "C" void vcbct_fdk_plan_destroy(vcbct_fdk_plan* p) {
 
 extern "C" int vcbct_fdk_plan_filter(vcbct_fdk_plan* p, const float* dev_proj, float* dev_filtered, void* stream) {
     if (!p || !dev_proj || !dev_filtered) return fail(VCBCT_EINVAL, "fdk_plan_filter: NULL argument");
+    if (p->d_pad) {  // half-fan: weight and pad into the plan's buffer, then filter the padded panel
+        int rc = launch_offset_weight(dev_proj, p->d_pad, p->H, p->hf_W, p->W, p->hf_off, p->n, p->d_opar,
+                                      (hipStream_t)stream);
+        if (rc) return rc;
+        dev_proj = p->d_pad;
+    }
     return launch_filter(p->fp, p->H, p->W, p->n, dev_proj, dev_filtered, p->d_ctab, p->pitch, (hipStream_t)stream);
 }
 

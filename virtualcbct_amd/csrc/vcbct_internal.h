@@ -118,6 +118,22 @@ int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, co
 // Zero-fill kernel (bytes must be a multiple of 4).
 int launch_zero(void* p, size_t bytes, hipStream_t st);
 
+// Half-fan (offset-panel) weighting and short-side padding (angle_mask.hip).  Per projection: the central
+// ray's column on the unpadded panel, the transition half-width, the side of the long half.
+struct OffsetPar {
+    double c;     // column of the central ray (unpadded panel)
+    double tau;   // transition half-width (pixels)
+    double side;  // +1: the long side is towards higher columns; -1: towards lower
+    double pad;
+};
+// The padded width Wo, the offset of the real columns in it, the per-projection parameters (n entries) and
+// the shifts of the padded panel (xs_out / ys_out, n entries each).  Returns 0 or an error code.
+int half_fan_params(int W, int n, const double* xshift, const double* yshift, double taper_px, int& Wo, int& off,
+                    OffsetPar* par, double* xs_out, double* ys_out);
+// in [n][H][W] -> out [n][H][Wo]; d_par: n parameters in device memory.  Allocates nothing.
+int launch_offset_weight(const float* in, float* out, int H, int W, int Wo, int off, int n, const OffsetPar* d_par,
+                         hipStream_t st);
+
 // Fault injection for allocation failure (VCBCT_FAULT_ALLOC=<n>: the n-th device allocation fails).
 hipError_t dev_malloc(void** p, size_t bytes);
 

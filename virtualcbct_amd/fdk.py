@@ -178,19 +178,32 @@ class FdkPlan:
     thread) is autotuned at creation."""
 
     def __init__(self, projHeight, projWidth, nProj, reconSize, reconSlices, zstart, reconStep, reconScale, xfactor,
-                 anglesRad, xshift, yshift, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, device=0):
+                 anglesRad, xshift, yshift, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, device=0, half_fan=False,
+                 taper_px=0.0):
+        """half_fan=True: an offset-panel scan; the plan weights and pads the projections inside its filter call
+        (see offset_weight_device) and filters / backprojects the padded panel (width `self.width`)."""
         a, xs, ys = [np.ascontiguousarray(v, np.float64) for v in (anglesRad, xshift, yshift)]
         if min(a.size, xs.size, ys.size) < nProj:
             raise ValueError("angles/xshift/yshift need one entry per projection")
         h = C.c_void_p()
-        check(lib().vcbct_fdk_plan_create(C.byref(h), int(projHeight), int(projWidth), int(nProj), int(reconSize),
-                                          int(reconSlices), int(zstart), float(reconStep), float(reconScale),
-                                          int(xfactor), ptr(a, P_D), ptr(xs, P_D), ptr(ys, P_D),
-                                          float(DETECTOR_PITCH_CM_AT_ISO_CENT), int(device)))
+        args = (int(projHeight), int(projWidth), int(nProj), int(reconSize), int(reconSlices), int(zstart),
+                float(reconStep), float(reconScale), int(xfactor), ptr(a, P_D), ptr(xs, P_D), ptr(ys, P_D),
+                float(DETECTOR_PITCH_CM_AT_ISO_CENT))
+        if half_fan:
+            check(lib().vcbct_fdk_plan_create_half_fan(C.byref(h), *args, float(taper_px), int(device)))
+        else:
+            check(lib().vcbct_fdk_plan_create(C.byref(h), *args, int(device)))
         self.h = h
+        self.width = lib().vcbct_fdk_plan_width(h)
         self.shape_proj = (int(nProj), int(projHeight), int(projWidth))
+        self.shape_filtered = (int(nProj), int(projHeight), int(self.width))
         self.shape_recon = (int(reconSlices), int(reconSize), int(reconSize))
         self.device = int(device)
+
+    def _check(self, t, shape, name):
+        torch = _torch()
+        if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != shape:
+            raise ValueError(f"{name} must be a contiguous float32 CUDA tensor of shape {shape}")
 
     @property
     def slices_per_thread(self):
@@ -199,25 +212,34 @@ class FdkPlan:
     def filter(self, proj, filtered=None, stream=None):
         """filterSinogram on the device: -log, ramp convolution, cosine weight; returns the filtered set."""
         torch = _torch()
+        proj = proj.contiguous()
+        self._check(proj, self.shape_proj, "proj")
         if filtered is None:
-            filtered = torch.empty(self.shape_proj, dtype=torch.float32, device=proj.device)
-        check(lib().vcbct_fdk_plan_filter(self.h, C.c_void_p(proj.contiguous().data_ptr()),
+            filtered = torch.empty(self.shape_filtered, dtype=torch.float32, device=proj.device)
+        self._check(filtered, self.shape_filtered, "filtered")
+        check(lib().vcbct_fdk_plan_filter(self.h, C.c_void_p(proj.data_ptr()),
                                           C.c_void_p(filtered.data_ptr()), _stream(stream)))
         return filtered
 
     def backproject(self, filtered, recon, stream=None):
         """reconstructImage on the device: accumulates (+=) into recon (FP32 [Z][R][R])."""
-        check(lib().vcbct_fdk_plan_backproject(self.h, C.c_void_p(filtered.contiguous().data_ptr()),
+        filtered = filtered.contiguous()
+        self._check(filtered, self.shape_filtered, "filtered")
+        self._check(recon, self.shape_recon, "recon")
+        check(lib().vcbct_fdk_plan_backproject(self.h, C.c_void_p(filtered.data_ptr()),
                                               C.c_void_p(recon.data_ptr()), _stream(stream)))
         return recon
 
     def execute(self, proj, filtered=None, recon=None, stream=None):
         """filter + backprojection; recon (FP32 [Z][R][R]) is accumulated into (zeroed if created here)."""
         torch = _torch()
+        self._check(proj, self.shape_proj, "proj")
         if filtered is None:
-            filtered = torch.empty(self.shape_proj, dtype=torch.float32, device=proj.device)
+            filtered = torch.empty(self.shape_filtered, dtype=torch.float32, device=proj.device)
         if recon is None:
             recon = torch.zeros(self.shape_recon, dtype=torch.float32, device=proj.device)
+        self._check(filtered, self.shape_filtered, "filtered")
+        self._check(recon, self.shape_recon, "recon")
         check(lib().vcbct_fdk_plan_execute(self.h, C.c_void_p(proj.data_ptr()), C.c_void_p(filtered.data_ptr()),
                                            C.c_void_p(recon.data_ptr()), _stream(stream)))
         return recon
