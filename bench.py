@@ -124,16 +124,38 @@ def bench_fdk(line, inp, dev, stream, out):
         t0 = time.perf_counter()
         fdk.offset_weight_device(proj, inp["xs_fdk"], inp["ys_fdk"])
         tw.append((time.perf_counter() - t0) * 1e3)
+    # the same scan through a half-fan plan (weighting + padding to 630 columns inside the filter call)
+    plan_hf = fdk.FdkPlan(w["H"], w["W"], w["n_ang"], w["R"], w["Z"], w["zstart"], w["zstep"], w["rscale"], 1000,
+                          inp["a_fdk"], inp["xs_fdk"], inp["ys_fdk"], half_fan=True)
+    filt_hf = torch.empty(plan_hf.shape_filtered, dtype=torch.float32, device=dev)
+    plan_hf.filter(proj, filt_hf)
+    torch.cuda.synchronize()
+    thf, thb = [], []
+    for _ in range(nrep):
+        rec.zero_()
+        e[0].record(stream)
+        plan_hf.filter(proj, filt_hf)
+        e[1].record(stream)
+        plan_hf.backproject(filt_hf, rec)
+        e[2].record(stream)
+        torch.cuda.synchronize()
+        thf.append(e[0].elapsed_time(e[1]))
+        thb.append(e[1].elapsed_time(e[2]))
+    plan_hf.close()
+    del filt_hf
     from virtualcbct_amd.batch import autotune_log
     updates = w["R"] * w["R"] * w["Z"] * w["n_ang"]
     line["fdk"] = {"gups": updates / (np.median(tb) * 1e-3) / 1e9, "bp_ms": float(np.median(tb)),
                    "filter_ms": float(np.median(tf)), "offset_weight_ms": float(np.median(tw)),
+                   "half_fan_plan": {"filter_ms": float(np.median(thf)), "bp_ms": float(np.median(thb)),
+                                     "width": plan_hf.width},
                    "filter_rows_per_s": w["n_ang"] * w["H"] / (np.median(tf) * 1e-3),
                    "updates": updates, "shape": [w["Z"], w["R"], w["R"]],
                    "bp_roofline_frac": (16.0 * updates / (np.median(tb) * 1e-3) / 1e9) / HBM_PEAK_GBS,
                    "autotune": [l for l in autotune_log().strip().split("\n") if l.startswith("bp ")]}
     log(f"fdk: filter {np.median(tf):.3f} ms, backprojection {np.median(tb):.3f} ms, "
-        f"{line['fdk']['gups']:.1f} GUPS, offset weighting {np.median(tw):.3f} ms")
+        f"{line['fdk']['gups']:.1f} GUPS, offset weighting {np.median(tw):.3f} ms; half-fan plan: filter "
+        f"{np.median(thf):.3f} ms, backprojection {np.median(thb):.3f} ms")
     plan.close()
     del filt, rec
 
