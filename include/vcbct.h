@@ -225,6 +225,10 @@ int vcbct_fdk_plan_backproject(vcbct_fdk_plan* p, const float* dev_filtered, flo
 int vcbct_fdk_plan_execute(vcbct_fdk_plan* p, const float* dev_proj, float* dev_filtered, float* dev_recon,
                            void* stream);
 int vcbct_fdk_plan_slices_per_thread(const vcbct_fdk_plan* p);
+/* 1 when the plan's backprojector stages the projection footprint of each block in LDS (picked with the
+ * slices per thread by the autotuner; VCBCT_BP_LDS=0 / 1 restricts it), 0 otherwise.  Both forms give the
+ * same result bit for bit. */
+int vcbct_fdk_plan_bp_staged(const vcbct_fdk_plan* p);
 /* Half-fan FDK plan (no reference counterpart; the weighting of vcbct_fdk_offset_weight below, inside the
  * plan): the plan runs on the padded panel (its width: vcbct_fdk_plan_width), and its filter call weights and
  * pads the caller's [n][H][W] projections into a buffer the plan owns before filtering, so dev_filtered is

@@ -333,6 +333,156 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
 }
 
 
+// The same backprojection with the projection data staged in LDS.  Per angle, the rows and columns that the
+// block's 16 x 16 columns and ZB slices can sample (its footprint on the panel) are loaded into an LDS tile by
+// the whole block, coalesced along the rows, and the taps are read from the tile.  The footprint's columns
+// come from a per-(block, angle) table built with the plan: U is a linear-fractional function of (x, y), so
+// over the block's square its extremes lie at the four corner voxels; the base column floor(W/2 + U + xs -
+// 0.5) is taken there in the same FP64 expression, widened by one column before and two after.  Its rows
+// come from the extremes of 1 / (L2 pitch) at the corners (also in the table) and the block's slice range,
+// widened likewise.  A tap outside the tile (a footprint larger than the tile, or a rounding beyond the
+// margins) is read from global memory at the address the unstaged kernel reads, so the result is identical
+// to k_backproject's bit for bit.  Two tiles alternate, so one barrier per angle suffices: a tile is
+// rewritten two angles after it was read, and every thread has passed the barrier in between.
+struct BpFoot {
+    int u0, u1;      // columns [u0, u1] of the footprint (clamped to the panel; u0 > u1: none)
+    float ilo, ihi;  // extremes of (1 / L2) / pitch over the block's corner voxels
+};
+constexpr int BP_TILE = 2048;  // floats per staged tile (8 KB; two per block)
+
+template <int ZB>
+__global__ __launch_bounds__(256) void k_backproject_lds(float* __restrict__ recon, const float* __restrict__ sino,
+                                                         int R, int Z, double zstart, double step, double scale,
+                                                         int H, int W, int n, const double* __restrict__ atab,
+                                                         double xfactor, double pitchW,
+                                                         const BpFoot* __restrict__ foot) {
+    __shared__ float tiles[2][BP_TILE];
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int z0 = blockIdx.z * ZB;
+    const double rad = (double)(R - 1) / 2.0;
+    // every thread stages and meets the barriers; only the voxels inside the mask accumulate
+    const bool active = x < R && y < R &&
+                        ((double)y - rad) * ((double)y - rad) + ((double)x - rad) * ((double)x - rad) <= rad * rad;
+    const double centerPos = (double)R / 2.0 - 0.5;
+    const double xx = (x - centerPos) * scale;
+    const double yy = -(y - centerPos) * scale;
+    float zz[ZB];
+    float acc[ZB];
+#pragma unroll
+    for (int i = 0; i < ZB; ++i) {
+        zz[i] = (float)(zstart + (double)(z0 + i) * step);
+        acc[i] = 0.f;
+    }
+    const double hW = (double)(W / 2);
+    const float hH = (float)(H / 2);
+    const float ipW = (float)(1.0 / pitchW);
+    const BpFoot* fb = foot + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * n;
+    const float zlo = zz[0], zhi = zz[ZB - 1];
+    for (int th = 0; th < n; ++th) {
+        const BpFoot f = fb[th];
+        const float e0 = zlo * f.ilo, e1 = zlo * f.ihi, e2 = zhi * f.ilo, e3 = zhi * f.ihi;
+        const int r0 = max(0, (int)floorf(hH + fminf(fminf(e0, e1), fminf(e2, e3))) - 1);
+        const int r1 = min(H - 1, (int)floorf(hH + fmaxf(fmaxf(e0, e1), fmaxf(e2, e3))) + 2);
+        const int tw = f.u1 - f.u0 + 1, tr = r1 - r0 + 1;
+        const bool use_tile = tw > 0 && tr > 0 && tw * tr <= BP_TILE;  // uniform over the block
+        float* tile = tiles[th & 1];
+        const float* P = sino + (size_t)th * H * W;
+        if (use_tile) {
+            for (int i = threadIdx.x; i < tw * tr; i += 256) {
+                const int rr = i / tw, cc = i - rr * tw;
+                tile[i] = P[(size_t)(r0 + rr) * W + f.u0 + cc];
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
+        const double c = atab[th], s = atab[n + th], dbeta = atab[2 * n + th], xs = atab[3 * n + th];
+        double X = xx * c + yy * s;
+        double Y = -xx * s + yy * c;
+        double L2 = 1.0 - Y / 100.0;
+        double U = X / L2 / pitchW;
+        double fU = floor(U);
+        double w = U - fU;
+        int UU = (int)floor(hW + U + xs - 0.5);
+        const bool ucol = (UU >= 0 && UU < W - 1), uedge = (UU == W - 1);
+        if (!ucol && !uedge) continue;
+        const float rL2 = 1.0f / (float)L2;
+        const float iL = rL2 * ipW;
+        const float wgt = rL2 * rL2 * (float)(dbeta * xfactor);
+        const float wf = (float)w;
+        auto tap = [&](int r, int col) -> float {
+            const int a = r - r0, b = col - f.u0;
+            return (use_tile && (unsigned)a < (unsigned)tr && (unsigned)b < (unsigned)tw) ? tile[a * tw + b]
+                                                                                      : P[(size_t)r * W + col];
+        };
+#pragma unroll
+        for (int i = 0; i < ZB; ++i) {
+            const float ZETA = hH + zz[i] * iL;
+            const float fz = floorf(ZETA);
+            const int NZ = (int)fz;
+            const float wz = ZETA - fz;
+            float sum;
+            if (NZ >= 0 && NZ < H - 1) {
+                if (ucol) {
+                    float a0 = tap(NZ, UU), a1 = tap(NZ, UU + 1), b0 = tap(NZ + 1, UU), b1 = tap(NZ + 1, UU + 1);
+                    sum = (a0 * (1.f - wf) + a1 * wf) * (1.f - wz) + (b0 * (1.f - wf) + b1 * wf) * wz;
+                } else {
+                    sum = tap(NZ, UU) * (1.f - wz) + tap(NZ + 1, UU) * wz;
+                }
+            } else if (NZ == H - 1) {
+                sum = ucol ? (tap(NZ, UU) * (1.f - wf) + tap(NZ, UU + 1) * wf) : tap(NZ, UU);
+            } else {
+                continue;
+            }
+            acc[i] += sum * wgt;
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int i = 0; i < ZB; ++i)
+        if (z0 + i < Z) recon[((size_t)(z0 + i) * R + y) * R + x] += acc[i];
+}
+
+// Footprint table of the staged backprojector: per (16 x 16 column block, angle), in block-row order.
+static std::vector<BpFoot> bp_foot_table(int R, int W, int n, double scale, double pitch_iso, const double* tab) {
+    const int nb = (R + 15) / 16;
+    const double pitchW = pitch_iso * 800 / W;
+    const double hW = (double)(W / 2);
+    const float ipW = (float)(1.0 / pitchW);
+    const double centerPos = (double)R / 2.0 - 0.5;
+    std::vector<BpFoot> f((size_t)nb * nb * n);
+    for (int by = 0; by < nb; ++by)
+        for (int bx = 0; bx < nb; ++bx)
+            for (int th = 0; th < n; ++th) {
+                const double c = tab[th], s = tab[n + th], xs = tab[3 * n + th];
+                int umin = INT_MAX, umax = INT_MIN;
+                float ilo = 1e30f, ihi = -1e30f;
+                for (int cy = 0; cy < 2; ++cy)
+                    for (int cx = 0; cx < 2; ++cx) {
+                        const int x = cx ? std::min(bx * 16 + 15, R - 1) : bx * 16;
+                        const int y = cy ? std::min(by * 16 + 15, R - 1) : by * 16;
+                        const double xx = (x - centerPos) * scale;
+                        const double yy = -(y - centerPos) * scale;
+                        double X = xx * c + yy * s;
+                        double Y = -xx * s + yy * c;
+                        double L2 = 1.0 - Y / 100.0;
+                        double U = X / L2 / pitchW;
+                        const int UU = (int)floor(hW + U + xs - 0.5);
+                        umin = std::min(umin, UU);
+                        umax = std::max(umax, UU);
+                        const float iL = (1.0f / (float)L2) * ipW;
+                        ilo = std::min(ilo, iL);
+                        ihi = std::max(ihi, iL);
+                    }
+                BpFoot& e = f[((size_t)by * nb + bx) * n + th];
+                e.u0 = std::max(0, umin - 1);
+                e.u1 = std::min(W - 1, umax + 2);
+                e.ilo = ilo;
+                e.ihi = ihi;
+            }
+    return f;
+}
+
 // ------------------------------------------------------------------------------------------------
 // Plans: per-geometry device tables built once, so that execution allocates nothing and synchronises
 // nothing (it can be captured in a hipGraph).
@@ -392,17 +542,29 @@ static int launch_filter(const FilterPlan* P, int H, int W, int n, const float* 
     return 0;
 }
 
+// zb: slices per thread; d_foot non-null: the LDS-staged form with that footprint table.
 static int launch_bp(int zb, int R, int Z, int zstart, double step, double scale, float* drec, int H, int W, int n,
-                     const float* ds, const double* d_atab, int xfactor, double pitch_iso, hipStream_t st) {
+                     const float* ds, const double* d_atab, int xfactor, double pitch_iso, hipStream_t st,
+                     const BpFoot* d_foot = nullptr) {
     dim3 grid((R + 15) / 16, (R + 15) / 16, (Z + zb - 1) / zb);
     const double pitchW = pitch_iso * 800 / W;
 #define VCBCT_BP(ZBV)                                                                                                  \
     hipLaunchKernelGGL(k_backproject<ZBV>, grid, dim3(256), 0, st, drec, ds, R, Z, (double)zstart, step, scale, H, W, n, \
                        d_atab, (double)xfactor, pitchW)
-    if (zb == 2) VCBCT_BP(2);
-    else if (zb == 8) VCBCT_BP(8);
-    else VCBCT_BP(4);
+#define VCBCT_BPL(ZBV)                                                                                                 \
+    hipLaunchKernelGGL(k_backproject_lds<ZBV>, grid, dim3(256), 0, st, drec, ds, R, Z, (double)zstart, step, scale, H, \
+                       W, n, d_atab, (double)xfactor, pitchW, d_foot)
+    if (d_foot) {
+        if (zb == 2) VCBCT_BPL(2);
+        else if (zb == 8) VCBCT_BPL(8);
+        else VCBCT_BPL(4);
+    } else {
+        if (zb == 2) VCBCT_BP(2);
+        else if (zb == 8) VCBCT_BP(8);
+        else VCBCT_BP(4);
+    }
 #undef VCBCT_BP
+#undef VCBCT_BPL
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
@@ -425,24 +587,32 @@ struct vcbct_fdk_plan {
     int hf_W, hf_off;
     float* d_pad;
     vcbct::OffsetPar* d_opar;
+    // backprojector form: LDS-staged (with its footprint table) or not, picked with zb by the autotuner
+    int bp_lds;
+    vcbct::BpFoot* d_foot;
 };
 
 namespace vcbct {
 
 static int tune_bp(vcbct_fdk_plan* p) {
-    char key[128];
-    snprintf(key, sizeof key, "bp R%d Z%d n%d H%d W%d", p->R, p->Z, p->n, p->H, p->W);
+    // VCBCT_BP_LDS=0 / 1 restricts the candidates to the unstaged / LDS-staged form
+    const char* lds_env = getenv("VCBCT_BP_LDS");
+    const int lds_only = (lds_env && (*lds_env == '0' || *lds_env == '1')) ? *lds_env - '0' : -1;
+    char key[160];
+    snprintf(key, sizeof key, "bp R%d Z%d n%d H%d W%d lds%d", p->R, p->Z, p->n, p->H, p->W, lds_only);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_bp_zb.find(key);
         if (it != g_bp_zb.end()) {
-            p->zb = it->second;
+            p->zb = it->second % 100;
+            p->bp_lds = it->second / 100;
             return 0;
         }
     }
     const char* off = getenv("VCBCT_NO_AUTOTUNE");
     if (off && *off == '1') {
         p->zb = 4;
+        p->bp_lds = lds_only == 1 ? 1 : 0;
         return 0;
     }
     float *ds = nullptr, *dr = nullptr;
@@ -457,34 +627,42 @@ static int tune_bp(vcbct_fdk_plan* p) {
     VCBCT_HIP(hipEventCreate(&e0));
     VCBCT_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    int best_zb = 4;
+    int best_code = 4;
     std::string rep = std::string(key) + ":";
-    for (int zb : {2, 4, 8}) {
-        if (rc) break;
-        rc = launch_bp(zb, p->R, p->Z, p->zstart, p->step, p->scale, dr, p->H, p->W, p->n, ds, p->d_atab, p->xfactor,
-                       p->pitch, nullptr);  // warm
-        float ms = 0;
-        (void)hipEventRecord(e0, nullptr);
-        for (int r = 0; r < 2 && !rc; ++r)
-            rc = launch_bp(zb, p->R, p->Z, p->zstart, p->step, p->scale, dr, p->H, p->W, p->n, ds, p->d_atab, p->xfactor,
-                           p->pitch, nullptr);
-        (void)hipEventRecord(e1, nullptr);
-        (void)hipEventSynchronize(e1);
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        char b[64];
-        snprintf(b, sizeof b, " zb%d=%.3fms", zb, ms / 2);
-        rep += b;
-        if (ms < best) { best = ms; best_zb = zb; }
+    for (int lds = 0; lds < 2; ++lds) {
+        if (lds_only >= 0 && lds != lds_only) continue;
+        const BpFoot* foot = lds ? p->d_foot : nullptr;
+        for (int zb : {2, 4, 8}) {
+            if (rc) break;
+            rc = launch_bp(zb, p->R, p->Z, p->zstart, p->step, p->scale, dr, p->H, p->W, p->n, ds, p->d_atab,
+                           p->xfactor, p->pitch, nullptr, foot);  // warm
+            float ms = 0;
+            (void)hipEventRecord(e0, nullptr);
+            for (int r = 0; r < 2 && !rc; ++r)
+                rc = launch_bp(zb, p->R, p->Z, p->zstart, p->step, p->scale, dr, p->H, p->W, p->n, ds, p->d_atab,
+                               p->xfactor, p->pitch, nullptr, foot);
+            (void)hipEventRecord(e1, nullptr);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            char b[64];
+            snprintf(b, sizeof b, " %szb%d=%.3fms", lds ? "lds-" : "", zb, ms / 2);
+            rep += b;
+            if (ms < best) {
+                best = ms;
+                best_code = zb + 100 * lds;
+            }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(ds);
     (void)hipFree(dr);
     if (rc) return rc;
-    p->zb = best_zb;
+    p->zb = best_code % 100;
+    p->bp_lds = best_code / 100;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_bp_zb[key] = best_zb;
-    g_tune_log += rep + " -> zb" + std::to_string(best_zb) + "\n";
+    g_bp_zb[key] = best_code;
+    g_tune_log += rep + " -> " + (p->bp_lds ? "lds-" : "") + "zb" + std::to_string(p->zb) + "\n";
     return 0;
 }
 
@@ -515,10 +693,18 @@ extern "C" int vcbct_fdk_plan_create(vcbct_fdk_plan** out, int H, int W, int n, 
     if (!rc && (hipMemcpy(p->d_ctab, ct.data(), ct.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(p->d_atab, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
         rc = fail(VCBCT_EHIP, "fdk plan upload");
+    if (!rc) {  // footprint table of the LDS-staged backprojector
+        std::vector<BpFoot> ft = bp_foot_table(R, W, n, scale, pitch_iso, at.data());
+        if (dev_malloc((void**)&p->d_foot, ft.size() * sizeof(BpFoot)) != hipSuccess)
+            rc = fail(VCBCT_ENOMEM, "fdk plan footprint table");
+        else if (hipMemcpy(p->d_foot, ft.data(), ft.size() * sizeof(BpFoot), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(VCBCT_EHIP, "fdk plan upload");
+    }
     if (!rc) rc = tune_bp(p);
     if (rc) {
         (void)hipFree(p->d_ctab);
         (void)hipFree(p->d_atab);
+        (void)hipFree(p->d_foot);
         free(p);
         return rc;
     }
@@ -566,6 +752,7 @@ extern "C" void vcbct_fdk_plan_destroy(vcbct_fdk_plan* p) {
     (void)hipSetDevice(p->device);
     (void)hipFree(p->d_pad);
     (void)hipFree(p->d_opar);
+    (void)hipFree(p->d_foot);
     (void)hipFree(p->d_ctab);
     (void)hipFree(p->d_atab);
     free(p);
@@ -585,8 +772,10 @@ extern "C" int vcbct_fdk_plan_filter(vcbct_fdk_plan* p, const float* dev_proj, f
 extern "C" int vcbct_fdk_plan_backproject(vcbct_fdk_plan* p, const float* dev_filtered, float* dev_recon, void* stream) {
     if (!p || !dev_filtered || !dev_recon) return fail(VCBCT_EINVAL, "fdk_plan_backproject: NULL argument");
     return launch_bp(p->zb, p->R, p->Z, p->zstart, p->step, p->scale, dev_recon, p->H, p->W, p->n, dev_filtered,
-                     p->d_atab, p->xfactor, p->pitch, (hipStream_t)stream);
+                     p->d_atab, p->xfactor, p->pitch, (hipStream_t)stream, p->bp_lds ? p->d_foot : nullptr);
 }
+
+extern "C" int vcbct_fdk_plan_bp_staged(const vcbct_fdk_plan* p) { return p ? p->bp_lds : -1; }
 
 extern "C" int vcbct_fdk_plan_execute(vcbct_fdk_plan* p, const float* dev_proj, float* dev_filtered, float* dev_recon,
                                       void* stream) {

@@ -209,6 +209,11 @@ class FdkPlan:
     def slices_per_thread(self):
         return lib().vcbct_fdk_plan_slices_per_thread(self.h)
 
+    @property
+    def bp_staged(self):
+        """True when the backprojector stages each block's projection footprint in LDS (autotuned)."""
+        return bool(lib().vcbct_fdk_plan_bp_staged(self.h))
+
     def filter(self, proj, filtered=None, stream=None):
         """filterSinogram on the device: -log, ramp convolution, cosine weight; returns the filtered set."""
         torch = _torch()
