@@ -143,12 +143,31 @@ def bench_fdk(line, inp, dev, stream, out):
         thb.append(e[1].elapsed_time(e[2]))
     plan_hf.close()
     del filt_hf
+    # the reference's CPU backprojection (F/reconstructImage.cpp, built from its sources; OpenMP over slices)
+    # on a bounded sample of the same reconstruction: whole slices from the centre up, all 360 views, a
+    # multiple of the thread count (one slice per thread at a time)
+    cpu_ref = None
+    from oracle import oracle as O
+    if O.ref_available():
+        fh = filt.cpu().numpy()
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        _, sec1 = O.ref_reconstruct_image(w["R"], 1, 0, w["zstep"], w["rscale"], fh, 1000, inp["a_fdk"],
+                                          inp["xs_fdk"])
+        nz = int(min(w["Z"], nthreads * max(1, round(6.0 / max(sec1, 1e-3)))))
+        _, sec = O.ref_reconstruct_image(w["R"], nz, 0, w["zstep"], w["rscale"], fh, 1000, inp["a_fdk"],
+                                         inp["xs_fdk"])
+        upd = w["R"] * w["R"] * nz * w["n_ang"]
+        cpu_ref = {"gups": upd / sec / 1e9, "cores": nthreads, "kind": "reference",
+                   "sample": f"{nz} slices of the 270^2 reconstruction from all 360 views, {sec:.1f} s"}
+        log(f"cpu reference backprojection: {cpu_ref['gups']:.3f} GUPS ({cpu_ref['sample']})")
+        del fh
     from virtualcbct_amd.batch import autotune_log
     updates = w["R"] * w["R"] * w["Z"] * w["n_ang"]
     line["fdk"] = {"gups": updates / (np.median(tb) * 1e-3) / 1e9, "bp_ms": float(np.median(tb)),
                    "filter_ms": float(np.median(tf)), "offset_weight_ms": float(np.median(tw)),
                    "half_fan_plan": {"filter_ms": float(np.median(thf)), "bp_ms": float(np.median(thb)),
                                      "width": plan_hf.width},
+                   "cpu_reference_backprojection": cpu_ref,
                    "filter_rows_per_s": w["n_ang"] * w["H"] / (np.median(tf) * 1e-3),
                    "updates": updates, "shape": [w["Z"], w["R"], w["R"]],
                    "bp_roofline_frac": (16.0 * updates / (np.median(tb) * 1e-3) / 1e9) / HBM_PEAK_GBS,
