@@ -264,8 +264,8 @@ int vcbct_fdk_angle_mask(int n_proj, const double* host_angles_rad, int recon_si
  * [n][H][W + P] (out_width = vcbct_fdk_half_fan_width()), and host_xshift_out / host_yshift_out (n each) put
  * the central ray and the filter's row centre where they were.  Filter and backproject dev_out with these
  * shifts, width W + P and det_pitch_iso_cm * (W + P) / W (the same pixel size).  dev_out may alias dev_proj
- * only when P = 0.  Errors: shifts of both signs, a shift that leaves no overlap, W + P > 4096.
- * Synchronises the stream. */
+ * only when P = 0.  Errors: shifts of both signs, a shift that leaves no overlap, W + P > 4096,
+ * n_proj > 65535.  Synchronises the stream. */
 int vcbct_fdk_half_fan_width(int proj_width, int n_proj, const double* host_xshift);
 int vcbct_fdk_offset_weight(int proj_height, int proj_width, int n_proj, const float* dev_proj,
                             const double* host_xshift, const double* host_yshift, double taper_px, int out_width,
