@@ -73,7 +73,7 @@ static void fft64(std::vector<std::complex<double>>& a) {
 struct FilterPlan {
     int W, L, logL;
     float2* d_H;    // [L] frequency response of the circularly wrapped h, scaled by 1/L
-    float2* d_tw;   // [L] exp(-2 pi i m / L)
+    float2* d_tw;   // twiddles per stage, contiguous in k (see fft_lds); L entries
 };
 
 static std::mutex g_plan_mu;
@@ -107,10 +107,23 @@ static int get_plan(int W, double pitch_iso, FilterPlan** out) {
     std::vector<float2> Hf(P.L), tw(P.L);
     for (int k = 0; k < P.L; ++k) Hf[k] = make_float2((float)(h[k].real() / P.L), (float)(h[k].imag() / P.L));
     for (int k = 0; k < P.L; ++k) tw[k] = make_float2((float)cos(-2.0 * kPI * k / P.L), (float)sin(-2.0 * kPI * k / P.L));
+    // per-stage layout: radix-4 stage s (Ns = 4^s) holds w^(k r) = tw[(k r) << (logL - 2 - 2s)] at
+    // [base_s + (r - 1) Ns + k], r = 1..3, k < Ns; a closing radix-2 stage holds tw[k], k < L/2, after them
+    std::vector<float2> tc(P.L, make_float2(0.f, 0.f));
+    int base = 0, Ns = 1;
+    for (int st = 0; st < P.logL / 2; ++st) {
+        const int shift = P.logL - 2 - 2 * st;
+        for (int r = 1; r <= 3; ++r)
+            for (int k = 0; k < Ns; ++k) tc[base + (r - 1) * Ns + k] = tw[(k * r) << shift];
+        base += 3 * Ns;
+        Ns <<= 2;
+    }
+    if (P.logL & 1)
+        for (int k = 0; k < Ns; ++k) tc[base + k] = tw[k];
     VCBCT_HIP(dev_malloc((void**)&P.d_H, P.L * sizeof(float2)));
     VCBCT_HIP(dev_malloc((void**)&P.d_tw, P.L * sizeof(float2)));
     VCBCT_HIP(hipMemcpy(P.d_H, Hf.data(), P.L * sizeof(float2), hipMemcpyHostToDevice));
-    VCBCT_HIP(hipMemcpy(P.d_tw, tw.data(), P.L * sizeof(float2), hipMemcpyHostToDevice));
+    VCBCT_HIP(hipMemcpy(P.d_tw, tc.data(), P.L * sizeof(float2), hipMemcpyHostToDevice));
     g_plans[key] = P;
     *out = &g_plans[key];
     return 0;
@@ -129,19 +142,21 @@ constexpr int FPF = 8;  // pixels per thread per row held in registers: W <= FPF
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
 // x -> DFT(x) (INV: the unscaled inverse) through the ping-pong buffers x, y; returns the buffer holding it.
-// T[m] = exp(-2 pi i m / L).  Stage s of radix R has sub-transform size Ns = R^s: butterfly j reads
-// x[j + r L / R], scales input r by w^(k r), w = exp(-2 pi i / (R Ns)), k = j mod Ns, and writes
-// y[(j / Ns) R Ns + k + r Ns].
+// Stage s of radix R has sub-transform size Ns = R^s: butterfly j reads x[j + r L / R], scales input r by
+// w^(k r), w = exp(-2 pi i / (R Ns)), k = j mod Ns, and writes y[(j / Ns) R Ns + k + r Ns].  T holds each
+// stage's twiddles contiguous in k (w^(k r) at [base + (r - 1) Ns + k], base advancing by 3 Ns per radix-4
+// stage), so the lanes of a wave read consecutive or equal addresses: no LDS bank conflicts (a single
+// exp(-2 pi i m / L) table read at (k r) << shift had lanes 16 to 64 float2 apart, all in one bank).
 template <bool INV>
 __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T, int L, int logL) {
-    int Ns = 1;
+    int Ns = 1, base = 0;
     const int q = L >> 2;
     for (int s = 0; s < logL / 2; ++s) {
-        const int shift = logL - 2 - 2 * s;  // w^(k r) = T[(k r) << shift]
+        const float2* Ts = T + base;
         for (int j = threadIdx.x; j < q; j += FT) {
             const int k = j & (Ns - 1);
             float2 v0 = x[j], v1 = x[j + q], v2 = x[j + 2 * q], v3 = x[j + 3 * q];
-            float2 w1 = T[k << shift], w2 = T[(2 * k) << shift], w3 = T[(3 * k) << shift];
+            float2 w1 = Ts[k], w2 = Ts[Ns + k], w3 = Ts[2 * Ns + k];
             if (INV) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
             v1 = cmul(v1, w1);
             v2 = cmul(v2, w2);
@@ -157,14 +172,15 @@ __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T
         }
         __syncthreads();
         float2* t = x; x = y; y = t;
+        base += 3 * Ns;
         Ns <<= 2;
     }
-    if (logL & 1) {  // last stage radix 2: Ns = L / 2, w^k = T[k]
+    if (logL & 1) {  // last stage radix 2: Ns = L / 2, w^k at [base + k]
         const int half = L >> 1;
         for (int j = threadIdx.x; j < half; j += FT) {
             const int k = j & (Ns - 1);
             const float2 a = x[j];
-            float2 w = T[k];
+            float2 w = T[base + k];
             if (INV) w.y = -w.y;
             const float2 bw = cmul(x[j + half], w);
             const int o = ((j - k) << 1) + k;
