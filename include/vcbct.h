@@ -270,6 +270,10 @@ int vcbct_fdk_half_fan_width(int proj_width, int n_proj, const double* host_xshi
 int vcbct_fdk_offset_weight(int proj_height, int proj_width, int n_proj, const float* dev_proj,
                             const double* host_xshift, const double* host_yshift, double taper_px, int out_width,
                             float* dev_out, double* host_xshift_out, double* host_yshift_out, void* stream);
+/* Host-array form of vcbct_fdk_offset_weight for the host programs (host_out: [n][H][out_width]). */
+int vcbct_fdk_offset_weight_host(int proj_height, int proj_width, int n_proj, const float* host_proj,
+                                 const double* host_xshift, const double* host_yshift, double taper_px, int out_width,
+                                 float* host_out, double* host_xshift_out, double* host_yshift_out);
 
 /* ------------------------------------------------------------------------------------------------
  * Detector model (no reference counterpart: the stage between the projector and FDK that the reference
@@ -281,6 +285,9 @@ int vcbct_fdk_offset_weight(int proj_height, int proj_width, int n_proj, const f
  * of pixel i depends only on (seed, i, lambda).  dev_out may alias dev_primary.  Errors: I0 not positive. */
 int vcbct_detector_counts(size_t n, const float* dev_primary, const float* dev_scatter, double I0, uint64_t seed,
                           int raw, float* dev_out, void* stream);
+/* Host-array form of vcbct_detector_counts (host_scatter may be NULL; host_out may alias host_primary). */
+int vcbct_detector_counts_host(size_t n, const float* host_primary, const float* host_scatter, double I0,
+                               uint64_t seed, int raw, float* host_out);
 
 /* ------------------------------------------------------------------------------------------------
  * Batched launch (no reference counterpart): many forward problems — volumes of different slice counts,

@@ -84,3 +84,38 @@ def test_dropin_projector_and_fdk(torch, tmp_path):
     run(REF_FDK, d2, "f.txt")
     rec_theirs = np.fromfile(os.path.join(d2, "rec_short.raw"), np.int16).reshape(8, 48, 48)
     assert np.array_equal(rec_theirs, rec_mine)
+
+
+def test_dropin_noise_and_half_fan_options(torch, tmp_path):
+    """The host programs' opt-in stages (not in the reference): VCBCT_NOISE_I0 / VCBCT_NOISE_SEED make
+    vcbct_proj write the detector model's counts of its plain output, equal to the library call on the device;
+    VCBCT_HALF_FAN=1 makes vcbct_fdk weight and pad an offset panel, equal to fdk_half_fan_device before the
+    int16 truncation (both use the one-shot filter and backprojector)."""
+    if not (os.path.exists(MY_PROJ) and os.path.exists(MY_FDK)):
+        pytest.skip("host programs not built")
+    from virtualcbct_amd import detector, fdk
+    d = str(tmp_path / "opt")
+    setup_dir(d)
+    run(MY_PROJ, d, "p.txt")
+    plain = np.fromfile(os.path.join(d, "proj.raw"), np.float32).reshape(NA, H, W)
+    env = dict(os.environ, VCBCT_NOISE_I0="20000", VCBCT_NOISE_SEED="7")
+    r = subprocess.run([MY_PROJ, "p.txt"], cwd=d, env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    noisy = np.fromfile(os.path.join(d, "proj.raw"), np.float32).reshape(NA, H, W)
+    want = detector.detector_counts(torch.from_numpy(plain).cuda(), I0=20000.0, seed=7).cpu().numpy()
+    assert np.array_equal(noisy, want)
+    assert not np.array_equal(noisy, plain)
+    # the FDK program on an offset panel (12 of the 40 half-panel pixels: 6.0 in the FDK's units, 0.5 per pixel)
+    phantoms.write_flexmap(os.path.join(d, "m.flexmap"), NA, 6.0)
+    env = dict(os.environ, VCBCT_HALF_FAN="1")
+    r = subprocess.run([MY_FDK, "f.txt"], cwd=d, env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rec_prog = np.fromfile(os.path.join(d, "rec_short.raw"), np.int16).reshape(8, 48, 48)
+    a_f, xs_f, ys_f = io.read_fdk_angles(os.path.join(d, "m.flexmap"), NA)
+    rec = fdk.fdk_half_fan_device(torch.from_numpy(noisy).cuda(), -2, 0.5, 0.3, 48, 8, 1000, a_f, xs_f, ys_f)
+    want_rec = np.trunc(rec.cpu().numpy().astype(np.float64)).astype(np.int32)
+    # the program accumulates in FP64 on the host path, the device path in FP32: allow one unit of truncation
+    assert np.abs(rec_prog.astype(np.int32) - want_rec).max() <= 1
+    assert np.abs(rec_prog).max() > 10
+    flt = np.fromfile(os.path.join(d, "fltsngrm.raw"), np.float32)
+    assert flt.size == NA * H * fdk.half_fan_width(W, xs_f)

@@ -102,6 +102,9 @@ SIGNATURES = {
                                                c_double, c_int, P_D, P_D, P_D, c_double, c_double, c_int]),
     "vcbct_fdk_plan_width": (c_int, [c_void_p]),
     "vcbct_fdk_plan_bp_staged": (c_int, [c_void_p]),
+    "vcbct_fdk_offset_weight_host": (c_int, [c_int, c_int, c_int, c_void_p, P_D, P_D, c_double, c_int, c_void_p, P_D,
+                                             P_D]),
+    "vcbct_detector_counts_host": (c_int, [C.c_size_t, c_void_p, c_void_p, c_double, C.c_uint64, c_int, c_void_p]),
     "vcbct_batch_create": (c_int, [C.POINTER(c_void_p), c_int]),
     "vcbct_batch_destroy": (None, [c_void_p]),
     "vcbct_batch_add_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_double]),

@@ -245,3 +245,27 @@ extern "C" int vcbct_fdk_offset_weight(int H, int W, int n, const float* dev_pro
     if (e != hipSuccess) return fail(VCBCT_EHIP, "fdk_offset_weight: %s", hipGetErrorString(e));
     return 0;
 }
+
+extern "C" int vcbct_fdk_offset_weight_host(int H, int W, int n, const float* host_proj, const double* xshift,
+                                            const double* yshift, double taper_px, int out_width, float* host_out,
+                                            double* xshift_out, double* yshift_out) {
+    if (H < 1 || W < 1 || n < 1 || out_width < W || !host_proj || !host_out)
+        return fail(VCBCT_EINVAL, "fdk_offset_weight_host: bad arguments");
+    float *dp = nullptr, *dq = nullptr;
+    const size_t in_b = (size_t)n * H * W * sizeof(float), out_b = (size_t)n * H * out_width * sizeof(float);
+    if (dev_malloc((void**)&dp, in_b) != hipSuccess || dev_malloc((void**)&dq, out_b) != hipSuccess) {
+        (void)hipFree(dp);
+        return fail(VCBCT_ENOMEM, "fdk_offset_weight_host: device allocation failed");
+    }
+    int rc = 0;
+    if (hipMemcpy(dp, host_proj, in_b, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(VCBCT_EHIP, "fdk_offset_weight_host: upload failed");
+    if (!rc)
+        rc = vcbct_fdk_offset_weight(H, W, n, dp, xshift, yshift, taper_px, out_width, dq, xshift_out, yshift_out,
+                                     nullptr);
+    if (!rc && hipMemcpy(host_out, dq, out_b, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(VCBCT_EHIP, "fdk_offset_weight_host: download failed");
+    (void)hipFree(dp);
+    (void)hipFree(dq);
+    return rc;
+}

@@ -124,3 +124,26 @@ extern "C" int vcbct_detector_counts(size_t n, const float* dev_primary, const f
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
+
+extern "C" int vcbct_detector_counts_host(size_t n, const float* host_primary, const float* host_scatter, double I0,
+                                          uint64_t seed, int raw, float* host_out) {
+    if (n == 0) return 0;
+    if (!host_primary || !host_out) return fail(VCBCT_EINVAL, "detector_counts_host: NULL argument");
+    float *dp = nullptr, *ds = nullptr;
+    const size_t bytes = n * sizeof(float);
+    if (dev_malloc((void**)&dp, bytes) != hipSuccess ||
+        (host_scatter && dev_malloc((void**)&ds, bytes) != hipSuccess)) {
+        (void)hipFree(dp);
+        return fail(VCBCT_ENOMEM, "detector_counts_host: device allocation failed");
+    }
+    int rc = 0;
+    if (hipMemcpy(dp, host_primary, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        (ds && hipMemcpy(ds, host_scatter, bytes, hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(VCBCT_EHIP, "detector_counts_host: upload failed");
+    if (!rc) rc = vcbct_detector_counts(n, dp, ds, I0, seed, raw, dp, nullptr);
+    if (!rc && hipMemcpy(host_out, dp, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(VCBCT_EHIP, "detector_counts_host: download failed");
+    (void)hipFree(dp);
+    (void)hipFree(ds);
+    return rc;
+}

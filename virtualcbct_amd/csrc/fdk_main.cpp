@@ -1,6 +1,9 @@
 // vcbct_fdk <info.txt>: drop-in for the reference's cbct2 (F/main.cpp:13-247).  Reads <dir>.<ext>
 // (FP32 [th][row][col] projections), filters, writes fltsngrm.raw, backprojects and writes the int16
 // volume ((short) truncation of the FP64 accumulation, F/main.cpp:179-190).
+// VCBCT_HALF_FAN=1 (not in the reference) weights an offset-panel scan and pads its short side before the
+// filter (vcbct_fdk_offset_weight; VCBCT_HALF_FAN_TAPER=<pixels> narrows the transition); fltsngrm.raw then
+// holds the padded panel.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -31,14 +34,33 @@ int main(int argc, char** argv) {
         fprintf(stderr, "ERROR: projection file %s not readable\n", fn);
         return 1;
     }
-    std::vector<float> proj((size_t)n * H * W), flt((size_t)n * H * W);
+    std::vector<float> proj((size_t)n * H * W), flt;
     size_t got = fread(proj.data(), sizeof(float), proj.size(), fp);
     fclose(fp);
     if (got != proj.size()) {
         fprintf(stderr, "ERROR: projection file %s is short (%zu of %zu floats)\n", fn, got, proj.size());
         return 1;
     }
-    if (vcbct_filter_sinogram_host(H, W, n, proj.data(), flt.data(), xs.data(), ys.data(), c.det_pitch_iso_cm))
+    int Wf = W;  // columns of the panel that is filtered and backprojected
+    double pitch = c.det_pitch_iso_cm;
+    const char* hf = getenv("VCBCT_HALF_FAN");
+    if (hf && hf[0] == '1') {
+        const char* tp = getenv("VCBCT_HALF_FAN_TAPER");
+        Wf = vcbct_fdk_half_fan_width(W, n, xs.data());
+        if (Wf < 0) return die("half-fan weighting");
+        std::vector<float> padded((size_t)n * H * Wf);
+        std::vector<double> xs2(n), ys2(n);
+        if (vcbct_fdk_offset_weight_host(H, W, n, proj.data(), xs.data(), ys.data(), tp ? atof(tp) : 0.0, Wf,
+                                         padded.data(), xs2.data(), ys2.data()))
+            return die("half-fan weighting");
+        proj.swap(padded);
+        xs.swap(xs2);
+        ys.swap(ys2);
+        pitch = c.det_pitch_iso_cm * Wf / W;  // the same pixel size on the wider panel
+        fprintf(stderr, "half-fan weighting: panel padded from %d to %d columns\n", W, Wf);
+    }
+    flt.resize((size_t)n * H * Wf);
+    if (vcbct_filter_sinogram_host(H, Wf, n, proj.data(), flt.data(), xs.data(), ys.data(), pitch))
         return die("filterSinogram");
     if ((fp = fopen("fltsngrm.raw", "wb")) == nullptr) {
         fprintf(stderr, "ERROR: fltsngrm file NOT writable\n");
@@ -47,8 +69,8 @@ int main(int argc, char** argv) {
     fwrite(flt.data(), sizeof(float), flt.size(), fp);
     fclose(fp);
     std::vector<double> rec((size_t)Z * R * R, 0.0);
-    if (vcbct_reconstruct_image_host(R, Z, c.recon_offset_cm, c.recon_step, c.recon_scale, rec.data(), H, W, n,
-                                     flt.data(), c.xfactor, ang.data(), xs.data(), c.det_pitch_iso_cm))
+    if (vcbct_reconstruct_image_host(R, Z, c.recon_offset_cm, c.recon_step, c.recon_scale, rec.data(), H, Wf, n,
+                                     flt.data(), c.xfactor, ang.data(), xs.data(), pitch))
         return die("reconstructImage");
     std::vector<short> s(rec.size());
     for (size_t i = 0; i < rec.size(); ++i) s[i] = (short)rec[i];

@@ -1,4 +1,5 @@
 // vcbct_proj <info.txt>: drop-in for the reference's proj.exe (P/main.cpp:15-175), GPU build.
+// VCBCT_NOISE_I0 / VCBCT_NOISE_SEED (not in the reference): Poisson quantum noise on the output (below).
 // Reads the same txt, spectrum CSV, xraydb tables (attenuation_xraydb/ relative to the working directory),
 // flexmap and elemental slices, and writes <outputname>.raw (FP32 [angle][row][col]).
 #include <cstdio>
@@ -76,6 +77,16 @@ int main(int argc, char** argv) {
         return die("reprojectionOnGPU");
     }
     vcbct_projector_destroy(p);
+    // VCBCT_NOISE_I0=<quanta> (not in the reference): Poisson counts of I0 x the projection per pixel, written
+    // as max(k, 1) / I0 (vcbct_detector_counts; VCBCT_NOISE_SEED=<integer> picks the generator's key)
+    const char* ni = getenv("VCBCT_NOISE_I0");
+    if (ni && atof(ni) > 0.0) {
+        const char* nsd = getenv("VCBCT_NOISE_SEED");
+        const unsigned long long seed = nsd ? strtoull(nsd, nullptr, 10) : 0ull;
+        if (vcbct_detector_counts_host(out.size(), out.data(), nullptr, atof(ni), seed, 0, out.data()))
+            return die("detector model");
+        fprintf(stderr, "detector model: Poisson counts at I0 = %g, seed %llu\n", atof(ni), seed);
+    }
     char fn[1024];
     snprintf(fn, sizeof fn, "%s.raw", c.outputname);
     FILE* fp = fopen(fn, "wb");
