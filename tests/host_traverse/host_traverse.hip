@@ -121,3 +121,20 @@ extern "C" int ht_trace_ray(int N, double s, int Nz, double t, int H, int W, int
     vcbct::traverse_lane(g, me, true, kmin, kmax, rec);
     return rec.n;
 }
+
+// Quotients a / b over n pseudo-random a (mantissas and exponents drawn from a 64-bit LCG, |a| in
+// [2^-lo, 2^hi)): how many differ from the IEEE division when formed by div_known.
+extern "C" long ht_div_known_mismatches(double b, long n, unsigned long long seed, int lo, int hi) {
+    const double rb = 1.0 / b;
+    unsigned long long x = seed * 6364136223846793005ull + 1442695040888963407ull;
+    long bad = 0;
+    for (long i = 0; i < n; ++i) {
+        x = x * 6364136223846793005ull + 1442695040888963407ull;
+        const double m = 1.0 + (double)(x >> 11) * (1.0 / 9007199254740992.0);  // [1, 2)
+        x = x * 6364136223846793005ull + 1442695040888963407ull;
+        const int e = (int)((x >> 33) % (unsigned)(lo + hi)) - lo;
+        const double a = ldexp((x & 1) ? -m : m, e);
+        if (vcbct::div_known(a, b, rb) != a / b) ++bad;
+    }
+    return bad;
+}

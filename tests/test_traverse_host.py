@@ -119,3 +119,19 @@ def test_host_traversal_c2_offset_geometry(build):
     Lh, _ = host_line_integrals(g, vol, LIBS[build])
     den = np.maximum(np.abs(Lo).sum(-1), 1e-9)
     assert (np.abs(Lo - Lh).sum(-1) / den).max() < 1e-8
+
+
+@pytest.mark.parametrize("build", ["plain", "fma"])
+@pytest.mark.parametrize("b", [100.0, 0.032509752 * 800 / 400, 0.032509752 * 800 / 630, 0.032509752 * 800 / 64,
+                               0.032509752 * 800 / 33, 0.1, 3.0])
+def test_division_by_known_divisor_is_exact(build, b):
+    """div_known (the backprojector's divisions by the source distance and the pixel pitch): the product with
+    the correctly rounded reciprocal plus one FMA correction equals the IEEE quotient on 1e6 pseudo-random
+    dividends spanning 2^-30 .. 2^30 (Markstein's theorem; a dropped correction fails here)."""
+    lib = LIBS[build]
+    if not os.path.exists(lib):
+        pytest.skip(f"host traversal harness not built: {lib}")
+    f = C.CDLL(lib).ht_div_known_mismatches
+    f.restype = C.c_long
+    f.argtypes = [C.c_double, C.c_long, C.c_ulonglong, C.c_int, C.c_int]
+    assert f(b, 1000000, 11, 30, 30) == 0

@@ -300,14 +300,15 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
     const double hW = (double)(W / 2);
     const float hH = (float)(H / 2);
     const float ipW = (float)(1.0 / pitchW);
+    const double rpW = 1.0 / pitchW;  // correctly rounded: the divisions by pitchW below are exact products
     for (int th = 0; th < n; ++th) {
         const double c = atab[th], s = atab[n + th], dbeta = atab[2 * n + th], xs = atab[3 * n + th];
         // horizontal: FP64 in the reference's operation order (its base column floor(W/2 + U + xs - 0.5)
         // and weight frac(U) jump at different points, so U must round as the reference's does)
         double X = xx * c + yy * s;
         double Y = -xx * s + yy * c;
-        double L2 = 1.0 - Y / 100.0;
-        double U = X / L2 / pitchW;
+        double L2 = 1.0 - div_known(Y, 100.0, 0.01);  // = 1.0 - Y / 100.0 bit for bit
+        double U = div_known(X / L2, pitchW, rpW);    // = X / L2 / pitchW bit for bit
         double fU = floor(U);
         double w = U - fU;
         int UU = (int)floor(hW + U + xs - 0.5);
@@ -393,6 +394,7 @@ __global__ __launch_bounds__(256) void k_backproject_lds(float* __restrict__ rec
     const double hW = (double)(W / 2);
     const float hH = (float)(H / 2);
     const float ipW = (float)(1.0 / pitchW);
+    const double rpW = 1.0 / pitchW;  // correctly rounded: the divisions by pitchW below are exact products
     const BpFoot* fb = foot + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * n;
     const float zlo = zz[0], zhi = zz[ZB - 1];
     for (int th = 0; th < n; ++th) {
@@ -415,8 +417,8 @@ __global__ __launch_bounds__(256) void k_backproject_lds(float* __restrict__ rec
         const double c = atab[th], s = atab[n + th], dbeta = atab[2 * n + th], xs = atab[3 * n + th];
         double X = xx * c + yy * s;
         double Y = -xx * s + yy * c;
-        double L2 = 1.0 - Y / 100.0;
-        double U = X / L2 / pitchW;
+        double L2 = 1.0 - div_known(Y, 100.0, 0.01);  // = 1.0 - Y / 100.0 bit for bit
+        double U = div_known(X / L2, pitchW, rpW);    // = X / L2 / pitchW bit for bit
         double fU = floor(U);
         double w = U - fU;
         int UU = (int)floor(hW + U + xs - 0.5);

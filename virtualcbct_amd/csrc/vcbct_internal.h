@@ -115,6 +115,16 @@ int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int
 int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double xfactor,
                    const float* target_or_gp, int mode, float* coef, float* grad_vox, double* loss, hipStream_t st);
 
+// a / b for a divisor b known in advance, with rb = 1 / b correctly rounded: the product a * rb is within an
+// ulp of a / b, and one correction with the exact remainder a - q b (an FMA) rounds it correctly (Markstein's
+// theorem, round-to-nearest, no overflow or underflow), so the result equals the IEEE quotient bit for bit in
+// three operations instead of a full division sequence.
+__host__ __device__ inline double div_known(double a, double b, double rb) {
+    const double q = a * rb;
+    const double r = fma(-q, b, a);
+    return fma(r, rb, q);
+}
+
 // Zero-fill kernel (bytes must be a multiple of 4).
 int launch_zero(void* p, size_t bytes, hipStream_t st);
 
