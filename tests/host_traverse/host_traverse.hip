@@ -138,3 +138,21 @@ extern "C" long ht_div_known_mismatches(double b, long n, unsigned long long see
     }
     return bad;
 }
+
+// The same with a pseudo-random divisor per quotient (its reciprocal formed by one IEEE division).
+extern "C" long ht_div_known_random_mismatches(long n, unsigned long long seed, int lo, int hi) {
+    unsigned long long x = seed * 6364136223846793005ull + 1442695040888963407ull;
+    auto draw = [&]() {
+        x = x * 6364136223846793005ull + 1442695040888963407ull;
+        const double m = 1.0 + (double)(x >> 11) * (1.0 / 9007199254740992.0);
+        x = x * 6364136223846793005ull + 1442695040888963407ull;
+        const int e = (int)((x >> 33) % (unsigned)(lo + hi)) - lo;
+        return ldexp((x & 1) ? -m : m, e);
+    };
+    long bad = 0;
+    for (long i = 0; i < n; ++i) {
+        const double a = draw(), b = draw();
+        if (vcbct::div_known(a, b, 1.0 / b) != a / b) ++bad;
+    }
+    return bad;
+}

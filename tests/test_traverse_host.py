@@ -135,3 +135,17 @@ def test_division_by_known_divisor_is_exact(build, b):
     f.restype = C.c_long
     f.argtypes = [C.c_double, C.c_long, C.c_ulonglong, C.c_int, C.c_int]
     assert f(b, 1000000, 11, 30, 30) == 0
+
+
+@pytest.mark.parametrize("build", ["plain", "fma"])
+def test_division_with_a_shared_reciprocal_is_exact(build):
+    """The same identity with a divisor drawn afresh for each quotient and its reciprocal formed by one IEEE
+    division (lengthproj's shared divisors): 4e6 pseudo-random pairs over 2^-40 .. 2^40 and over 2^-4 .. 2^4."""
+    lib = LIBS[build]
+    if not os.path.exists(lib):
+        pytest.skip(f"host traversal harness not built: {lib}")
+    f = C.CDLL(lib).ht_div_known_random_mismatches
+    f.restype = C.c_long
+    f.argtypes = [C.c_long, C.c_ulonglong, C.c_int, C.c_int]
+    assert f(2000000, 3, 40, 40) == 0
+    assert f(2000000, 9, 4, 4) == 0

@@ -19,6 +19,7 @@ namespace vcbct {
 // per-angle constants of lengthproj (host-computed)
 struct LpAngle {
     double sx, sy, sz, ox, oy, c, s, xshift, yshift;
+    double ko;  // (oy - sy) / (ox - sx), the detector line's slope: per angle, formed once on the host
 };
 
 // (int)v as the reference's x86 build converts it; false where that conversion yields INT_MIN (NaN or out
@@ -42,32 +43,40 @@ __global__ __launch_bounds__(256) void k_lengthproj(const LpAngle* __restrict__ 
     const double rx = -N * s * 0.5 + (x + 0.5) * s;
     const int h2 = H / 2, w2 = W / 2;
     const double wh = (double)((W - H) / 2);
+    const double rpix = 1.0 / pix;
     float acc = 0.0f;
     for (int a = a0; a < a1; ++a) {
         const LpAngle A = tab[a];
         double rr;
+        // Quotients that share a divisor (or divide by the constant pitch) are formed from one correctly
+        // rounded reciprocal with div_known: the IEEE quotients bit for bit, one division instead of two.
+        // (A zero divisor gives NaN there where the division gives an infinity; it occurs only where the
+        // reference's result is NaN or rejected anyway: the angle then contributes nothing on both builds.)
         if (fabs(rx - A.sx) > fabs(ry - A.sy)) {
             const double x0 = rx - 0.5 * s, x1 = rx + 0.5 * s;
-            const double f0 = (x0 - A.sx) / (rx - A.sx), f1 = (x1 - A.sx) / (rx - A.sx);
+            const double dx = rx - A.sx, rdx = 1.0 / dx;
+            const double f0 = div_known(x0 - A.sx, dx, rdx), f1 = div_known(x1 - A.sx, dx, rdx);
             const double y0 = f0 * (ry - A.sy) + A.sy, z0 = f0 * (rz - A.sz) + A.sz;
             const double y1 = f1 * (ry - A.sy) + A.sy, z1 = f1 * (rz - A.sz) + A.sz;
             rr = sqrt((x0 - x1) * (x0 - x1) + (y0 - y1) * (y0 - y1) + (z0 - z1) * (z0 - z1));
         } else {
             const double y0 = ry - 0.5 * s, y1 = ry + 0.5 * s;
-            const double f0 = (y0 - A.sy) / (ry - A.sy), f1 = (y1 - A.sy) / (ry - A.sy);
+            const double dy = ry - A.sy, rdy = 1.0 / dy;
+            const double f0 = div_known(y0 - A.sy, dy, rdy), f1 = div_known(y1 - A.sy, dy, rdy);
             const double x0 = f0 * (rx - A.sx) + A.sx, z0 = f0 * (rz - A.sz) + A.sz;
             const double x1 = f1 * (rx - A.sx) + A.sx, z1 = f1 * (rz - A.sz) + A.sz;
             rr = sqrt((x0 - x1) * (x0 - x1) + (y0 - y1) * (y0 - y1) + (z0 - z1) * (z0 - z1));
         }
         // meet of the source -> voxel line with the detector line through o
-        const double kr = (A.sx - rx) / (A.sy - ry), ko = (A.oy - A.sy) / (A.ox - A.sx);
+        const double dys = A.sy - ry, rdys = 1.0 / dys;
+        const double kr = div_known(A.sx - rx, dys, rdys), ko = A.ko;
         const double py = (A.ox - rx + kr * ry + ko * A.oy) / (kr + ko);
         const double px = (kr * (py - ry) + rx);
-        const double pz = ((A.sz - rz) / (A.sy - ry) * (py - ry) + rz);
+        const double pz = (div_known(A.sz - rz, dys, rdys) * (py - ry) + rz);
         const double vv = -(pz - z_reg - A.yshift);
         const double uu = (px - x_reg) * A.c + (py - x_reg) * A.s + A.xshift - wh * pix;
         const double vfac = vv < 0 ? -0.5 : 0.5, ufac = uu < 0 ? -0.5 : 0.5;
-        const double vq = vv / pix, uq = uu / pix;
+        const double vq = div_known(vv, pix, rpix), uq = div_known(uu, pix, rpix);
         int v1, u1;
         if (!trunc_index(vq + vfac, &v1) || !trunc_index(uq + ufac, &u1)) continue;
         const double dv1 = vq - v1, du1 = uq - u1;
@@ -166,6 +175,7 @@ extern "C" int vcbct_projector_lengthproj(vcbct_projector* p, int ite, const uin
                 A.s = sn[a];
                 A.xshift = -(xc[a] + g.H / 2.0) * pix;
                 A.yshift = -(yc[a] + g.H / 2.0) * pix;
+                A.ko = (A.oy - A.sy) / (A.ox - A.sx);
             }
         }
         void* d = nullptr;
