@@ -176,6 +176,15 @@ int vcbct_projector_adjoint(vcbct_projector* p, const float* dev_target, float* 
  * dev_grad [mat][z][y][x] (overwritten) = sum_rays gp * mubar_m/xf * rr. */
 int vcbct_projector_logproj_backward(vcbct_projector* p, const float* dev_gp, float* dev_grad, double xfactor,
                                      void* stream);
+/* The log projection that also keeps its six line integrals per ray (dev_L: FP32 [nmat][n_angles*H*W], the
+ * rays in the projection layout), and the backward pass from them: the coefficients are formed per ray from
+ * dev_L (the same FP32 sums the backward's own traversal would form, so the same values) and only the
+ * scatter traverses the rays again.  No reference counterpart; the autograd path uses this pair (24 B per ray
+ * kept instead of a second traversal; the per-energy terms are never stored).  The backward does not read the
+ * volume. */
+int vcbct_projector_logproj_save(vcbct_projector* p, float* dev_p, float* dev_L, double xfactor, void* stream);
+int vcbct_projector_logproj_backward_saved(vcbct_projector* p, const float* dev_L, const float* dev_gp,
+                                           float* dev_grad, double xfactor, void* stream);
 
 /* ------------------------------------------------------------------------------------------------
  * FDK.

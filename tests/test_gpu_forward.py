@@ -271,3 +271,33 @@ def test_forward_odd_shapes_match_oracle(torch, case):
         print(f"segment counts differing from the host build: {int((diff > 0).sum())} of {diff.size} rays")
         assert (diff > 0).mean() < 3e-3 and diff.max() <= 2
     P.close()
+
+
+def test_saved_line_integrals_backward_equals_recompute(torch):
+    """The log projection that keeps its line integrals gives the same projection and the same sums as the
+    line-integral mode (bit for bit: the same traversal), and the backward pass from them equals the
+    backward pass that traverses the rays again (to the atomics' order of addition)."""
+    rng = np.random.default_rng(12)
+    N, Nz = 24, 6
+    ang = np.deg2rad(np.array([-170.0001, -100.0001, -40.00001, 9.99999, 79.99999, 150.0]))
+    cs, sn, xc, yc = O.trig_table_proj(20, 20, ang, np.full(6, 0.02), np.zeros(6))
+    g = O.Geometry(N=N, s=0.6, Nz=Nz, t=0.8, H=20, W=20, cos_t=cs, sin_t=sn, xc=xc, yc=yc, z_reg=0.1)
+    maps = rng.uniform(0.0, 1.2, (6, Nz, N, N)).astype(np.float32)
+    NE = 7
+    atten = rng.uniform(0.1, 0.6, (6, NE)).astype(np.float32)
+    pdf = rng.uniform(0.2, 1, NE).astype(np.float32)
+    P = make_projector(g)
+    P.set_volume(torch.from_numpy(maps).cuda())
+    P.set_spectrum(atten, pdf)
+    p1 = P.logproj(1.0)
+    p2, L = P.logproj_save(1.0)
+    assert torch.equal(p1, p2)
+    Lref = P.line_integrals()  # [angle][row][col][mat]
+    assert torch.equal(L, Lref.reshape(-1, 6).t().contiguous())
+    gp = torch.from_numpy(rng.normal(0, 1, p1.shape).astype(np.float32)).cuda()
+    g1 = P.logproj_backward(gp, 1.0)
+    g2 = P.logproj_backward_saved(L, gp, 1.0)
+    err = float((g1 - g2).abs().max()) / float(g1.abs().max())
+    print(f"saved vs recomputed backward: {err:.3e}")
+    assert err < 1e-5
+    P.close()

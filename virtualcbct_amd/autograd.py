@@ -2,10 +2,10 @@
 
 `poly_logproj(maps, projector, xfactor)` returns the polychromatic log projection
     p = -log( sum_e pdf_e exp(-sum_m mu_me L_m / xfactor) / sum_e pdf_e )
-of six FP32 elemental density maps [mat][z][y][x] (a torch tensor on the device).  Its backward runs the
-hand-written adjoint kernel: the per-ray line integrals are recomputed (not stored) and
-dL/dp * mubar_m / xfactor is scattered along the same ray traversal, with mubar_m the transmission-weighted
-spectral mean of mu_m.  There is no reference counterpart; the reference's scatter U-Net (Unet_scatter.py)
+of six FP32 elemental density maps [mat][z][y][x] (a torch tensor on the device).  The forward keeps the six
+per-ray material line integrals (24 B per ray; the per-energy terms are never stored), and the backward runs
+the hand-written adjoint kernels: from them, the per-ray coefficients dL/dp * mubar_m / xfactor (mubar_m the
+transmission-weighted spectral mean of mu_m), scattered along a second traversal of the rays.  There is no reference counterpart; the reference's scatter U-Net (Unet_scatter.py)
 consumes projections of this kind.
 """
 from __future__ import annotations
@@ -22,18 +22,18 @@ class PolyLogProjection(torch.autograd.Function):
             raise TypeError("maps must be an FP32 tensor on the device")
         m = maps.contiguous()
         projector.set_volume(m)
-        p = projector.logproj(xfactor)
+        p, L = projector.logproj_save(xfactor)
         ctx.projector = projector
         ctx.xfactor = xfactor
-        ctx.save_for_backward(m)
+        ctx.save_for_backward(L)
         return p
 
     @staticmethod
     def backward(ctx, grad_p):
-        (m,) = ctx.saved_tensors
-        P = ctx.projector
-        P.set_volume(m)  # the projector may have been used with other maps since forward
-        grad = P.logproj_backward(grad_p.contiguous(), ctx.xfactor)
+        (L,) = ctx.saved_tensors
+        # the scatter reads only the geometry and the spectrum (fixed per projector), not the maps, so the
+        # projector may have been used with other maps since forward
+        grad = ctx.projector.logproj_backward_saved(L, grad_p.contiguous(), ctx.xfactor)
         return grad, None, None
 
 

@@ -319,6 +319,41 @@ extern "C" int vcbct_projector_logproj_backward(vcbct_projector* p, const float*
     return run_adjoint(p, dev_gp, dev_grad, nullptr, xfactor, VCBCT_ADJ_UPSTREAM, stream);
 }
 
+extern "C" int vcbct_projector_logproj_save(vcbct_projector* p, float* dev_p, float* dev_L, double xfactor,
+                                            void* stream) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    const bool use_f32 = p->have_f32;
+    int rc = need(p, !use_f32, use_f32, true);
+    if (rc) return rc;
+    if (!dev_p || !dev_L) return fail(VCBCT_EINVAL, "logproj_save: NULL argument");
+    if (!(xfactor > 0)) return fail(VCBCT_EINVAL, "xfactor must be positive");
+    VCBCT_HIP(hipSetDevice(p->device));
+    return launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
+                          spec_of(p), xfactor, 0, p->g.nAng, dev_p, VCBCT_MODE_LOGPROJ, dev_L, (hipStream_t)stream);
+}
+
+extern "C" int vcbct_projector_logproj_backward_saved(vcbct_projector* p, const float* dev_L, const float* dev_gp,
+                                                      float* dev_grad, double xfactor, void* stream) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    int rc = need(p, false, false, true);  // the volume itself is not read: the line integrals are given
+    if (rc) return rc;
+    if (!dev_L || !dev_gp || !dev_grad) return fail(VCBCT_EINVAL, "logproj_backward_saved: NULL argument");
+    if (!(xfactor > 0)) return fail(VCBCT_EINVAL, "xfactor must be positive");
+    VCBCT_HIP(hipSetDevice(p->device));
+    hipStream_t st = (hipStream_t)stream;
+    const size_t nv = vox_count(p->g);
+    if (!p->d_grad_vox && dev_malloc((void**)&p->d_grad_vox, nv * 32) != hipSuccess)
+        return fail(VCBCT_ENOMEM, "device allocation of the gradient volume failed");
+    const size_t nrays = (size_t)p->g.nAng * p->g.H * p->g.W;
+    if (!p->d_coef && dev_malloc((void**)&p->d_coef, nrays * p->g.nmat * sizeof(float)) != hipSuccess)
+        return fail(VCBCT_ENOMEM, "device allocation of the adjoint coefficients failed");
+    rc = launch_zero(p->d_grad_vox, nv * 4 * p->g.nmat, st);
+    if (rc) return rc;
+    rc = launch_adjoint_saved(p->g, p->d_tables, dev_L, spec_of(p), xfactor, dev_gp, p->d_coef, p->d_grad_vox, st);
+    if (rc) return rc;
+    return launch_untranspose_f32(p->d_grad_vox, dev_grad, p->g, st);
+}
+
 extern "C" int vcbct_projector_segment_counts(vcbct_projector* p, float* dev_counts, void* stream) {
     if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
     if (!dev_counts) return fail(VCBCT_EINVAL, "segment_counts: output is NULL");

@@ -262,6 +262,12 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
         for (int m = 0; m < g.nmat; ++m) stg(aux, o * g.nmat + m, (float)acc.L[m]);
         return;
     }
+    if (MODE == VCBCT_MODE_LOGPROJ && aux) {  // the six line integrals, kept for the backward pass ([mat][ray])
+        const size_t nrays = (size_t)g.nAng * g.H * g.W;
+#pragma unroll
+        for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+            if (m < g.nmat) stg(aux, (size_t)m * nrays + o, acc.L[m]);
+    }
     bool zero = true;
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) zero = zero && (acc.L[m] == 0.0);
@@ -538,6 +544,37 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, c
     }
 }
 
+// The coefficients of the upstream-gradient mode from line integrals saved by the forward log projection
+// (the same traversal, so the same FP32 sums): one thread per ray, the epilogue of k_adjoint_coef.
+__global__ __launch_bounds__(256) void k_adjoint_coef_saved(ProjGeom g, const float* __restrict__ Ls,
+                                                            const double* __restrict__ mu,
+                                                            const double* __restrict__ pdf,
+                                                            const float* __restrict__ muf, int NE, double inv_xf,
+                                                            const float* __restrict__ gp, float* __restrict__ coef) {
+    const size_t nrays = (size_t)g.nAng * g.H * g.W;
+    for (size_t o = (size_t)blockIdx.x * 256 + threadIdx.x; o < nrays; o += (size_t)gridDim.x * 256) {
+        float L[VCBCT_MAX_MAT];
+#pragma unroll
+        for (int m = 0; m < VCBCT_MAX_MAT; ++m) L[m] = m < g.nmat ? ldg(Ls, (size_t)m * nrays + o) : 0.0f;
+        double se = 0.0, mub[VCBCT_MAX_MAT] = {0, 0, 0, 0, 0, 0};
+        const float k2 = (float)(-inv_xf * 1.4426950408889634);
+        for (int e = 0; e < NE; ++e) {
+            float tr = 0.0f;
+#pragma unroll
+            for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+                if (m < g.nmat) tr = fmaf(ldg(muf, m * NE + e), L[m], tr);
+            const double pe = ldg(pdf, e);
+            const double we = pe * (double)exp2f(tr * k2);
+            se += we;
+#pragma unroll
+            for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+                if (m < g.nmat) mub[m] += we * ldg(mu, m * NE + e);
+        }
+        const double w = (double)ldg(gp, o);
+        for (int m = 0; m < g.nmat; ++m) stg(coef, (size_t)m * nrays + o, (float)(w * (mub[m] / se) * inv_xf));
+    }
+}
+
 __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_scatter(ProjGeom g, const double* __restrict__ tab, int quads_per_xcd,
                                                          const float* __restrict__ coef, float* __restrict__ gvox) {
     __shared__ WaveLds step_tabs[4];
@@ -649,6 +686,21 @@ int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, co
     else
         hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_UPSTREAM>, dim3(nb), dim3(256), 0, st, g, tables,
                            (const float4*)vox, sp.mu, sp.pdf, sp.muf, sp.NE, inv_xf, qpx, target_or_gp, coef, loss);
+    VCBCT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, grad_vox);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_adjoint_saved(const ProjGeom& g, const double* tables, const float* Ls, const Spectrum& sp,
+                         double xfactor, const float* gp, float* coef, float* grad_vox, hipStream_t st) {
+    int qpx;
+    int nb = n_blocks(g, g.nAng, qpx);
+    if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
+    const size_t nrays = (size_t)g.nAng * g.H * g.W;
+    const unsigned nbc = (unsigned)std::min<size_t>((nrays + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_adjoint_coef_saved, dim3(nbc), dim3(256), 0, st, g, Ls, sp.mu, sp.pdf, sp.muf, sp.NE,
+                       1.0 / xfactor, gp, coef);
     VCBCT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, grad_vox);
     VCBCT_HIP(hipGetLastError());

@@ -114,6 +114,8 @@ int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int
                    double xfactor, int a0, int a1, float* out, int mode, float* aux, hipStream_t st);
 int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double xfactor,
                    const float* target_or_gp, int mode, float* coef, float* grad_vox, double* loss, hipStream_t st);
+int launch_adjoint_saved(const ProjGeom& g, const double* tables, const float* Ls, const Spectrum& sp,
+                         double xfactor, const float* gp, float* coef, float* grad_vox, hipStream_t st);
 
 // a / b for a divisor b known in advance, with rb = 1 / b correctly rounded: the product a * rb is within an
 // ulp of a / b, and one correction with the exact remainder a - q b (an FMA) rounds it correctly (Markstein's

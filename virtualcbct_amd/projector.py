@@ -177,6 +177,29 @@ class Projector:
                                             C.c_void_p(loss.data_ptr()), float(xfactor), _stream(stream)))
         return grad, loss
 
+    def logproj_save(self, xfactor, out=None, stream=None):
+        """(log projection, its six line integrals per ray [nmat][n_angles*H*W]) for logproj_backward_saved."""
+        torch = _torch()
+        if out is None:
+            out = torch.empty(self.shape_proj, dtype=torch.float32, device=f"cuda:{self.device}")
+        n = out.numel()
+        L = torch.empty((self.g.num_material, n), dtype=torch.float32, device=f"cuda:{self.device}")
+        check(lib().vcbct_projector_logproj_save(self.h, C.c_void_p(out.data_ptr()), C.c_void_p(L.data_ptr()),
+                                                 float(xfactor), _stream(stream)))
+        return out, L
+
+    def logproj_backward_saved(self, L, grad_p, xfactor, stream=None):
+        """The backward pass from the line integrals kept by logproj_save (the volume is not read)."""
+        torch = _torch()
+        gp = grad_p.contiguous().to(torch.float32)
+        if tuple(L.shape) != (self.g.num_material, gp.numel()) or L.dtype != torch.float32 or not L.is_contiguous():
+            raise ValueError("L must be the [nmat][rays] FP32 tensor returned by logproj_save")
+        grad = torch.empty(self.shape_maps, dtype=torch.float32, device=f"cuda:{self.device}")
+        check(lib().vcbct_projector_logproj_backward_saved(self.h, C.c_void_p(L.data_ptr()),
+                                                           C.c_void_p(gp.data_ptr()), C.c_void_p(grad.data_ptr()),
+                                                           float(xfactor), _stream(stream)))
+        return grad
+
     def logproj_backward(self, grad_p, xfactor, stream=None):
         torch = _torch()
         gp = grad_p.contiguous().to(torch.float32)
