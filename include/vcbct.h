@@ -42,7 +42,7 @@ const char* vcbct_version(void);
 /* ------------------------------------------------------------------------------------------------
  * Forward projector.
  *
- * The argument block of initializeGPU (P/reprojectionOnGPU.h:76-82 and P/reprojectionOnGPU.cu:720-757),
+ * The argument block of initializeGPU (P/reprojectionOnGPU.h:2-8 and P/reprojectionOnGPU.cu:720-757),
  * field for field.
  */
 typedef struct vcbct_geometry {
@@ -97,10 +97,16 @@ int vcbct_projector_line_integrals(vcbct_projector* p, float* dev_L, void* strea
  * FP32: the work unit of the roofline model (12 bytes of voxel data per segment). */
 int vcbct_projector_segment_counts(vcbct_projector* p, float* dev_counts, void* stream);
 
-/* Host drop-in for reprojectionOnGPU: uploads host_vol/atten/pdf, runs the forward projection and
- * copies the result to host_out [n_angles][H][W]. */
+/* Host drop-in for reprojectionOnGPU (P/reprojectionOnGPU.cu:838-1100, declared at
+ * P/reprojectionOnGPU.h:12-19): uploads host_vol/atten/pdf, runs the forward projection and copies the
+ * result to host_out [n_angles][H][W]. */
 int vcbct_reprojection(vcbct_projector* p, const uint16_t* host_vol, const float* host_atten,
                        const float* host_pdf, int NE, double xfactor, float* host_out);
+/* The same for the first n_out_angles angles only, host_out [n_out_angles][H][W]: the reference's launch
+ * covers angles [0, endProjNumber) (P/reprojectionOnGPU.cu:976) and writes exactly those.  n_out_angles
+ * must lie in [1, n_angles] (VCBCT_EINVAL otherwise). */
+int vcbct_reprojection_angles(vcbct_projector* p, const uint16_t* host_vol, const float* host_atten,
+                              const float* host_pdf, int NE, double xfactor, int n_out_angles, float* host_out);
 
 /* ------------------------------------------------------------------------------------------------
  * Organ-label phantom -> six elemental maps, the step before the projector
@@ -189,11 +195,11 @@ int vcbct_projector_logproj_backward_saved(vcbct_projector* p, const float* dev_
 /* ------------------------------------------------------------------------------------------------
  * FDK.
  *
- * Replaces prepareConstMap (F/filterSinogram.cpp:65-120): the W x W filter matrix h(m-n), FP64,
+ * Replaces prepareConstMap (F/filterSinogram.cpp:10-65): the W x W filter matrix h(m-n), FP64,
  * host output [W][W]. */
 int vcbct_prepare_const_map(int proj_width, double det_pitch_iso_cm, double* host_const_map);
 
-/* Replaces filterSinogram (F/filterSinogram.cpp:122-186): q = -log(p), convolution with the reference's
+/* Replaces filterSinogram (F/filterSinogram.cpp:67-131): q = -log(p), convolution with the reference's
  * truncated kernel (centre tap coeff, off-centre h(m-k), |m-k| <= W-1) done as a batched FFT
  * convolution, then the cosine weight Cm at the output pixel.  Device [n][H][W] in and out.
  * xshift/yshift are host arrays of n_proj entries. */

@@ -16,8 +16,8 @@
  *   orc_line_integrals       the same traversal, six material line integrals per ray (energy collapsed)
  *   orc_forward_collapsed    one traversal per ray, then the spectral sum (new form of the same result)
  *   orc_logproj / orc_adjoint  polychromatic -log(I/I0) and its exact gradient (no reference counterpart)
- *   orc_const_map            F/filterSinogram.cpp:65-120 (prepareConstMap)
- *   orc_filter_sinogram      F/filterSinogram.cpp:122-186 (filterSinogram; FP32 accumulator like the
+ *   orc_const_map            F/filterSinogram.cpp:10-65 (prepareConstMap)
+ *   orc_filter_sinogram      F/filterSinogram.cpp:67-131 (filterSinogram; FP32 accumulator like the
  *                            reference, or FP64 when acc64 != 0)
  *   orc_mask                 F/misc.cpp:9-37            (prepareMaskData)
  *   orc_reconstruct_image    F/reconstructImage.cpp:10-184 (reconstructImage)
@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-/* The argument block of initializeGPU (P/reprojectionOnGPU.h:76-82). */
+/* The argument block of initializeGPU (P/reprojectionOnGPU.h:2-8). */
 typedef struct orc_geom {
     int N;              /* reconSize (square, x and y) */
     double s;           /* reconScale, voxel pitch in x/y [cm] */

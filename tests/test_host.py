@@ -190,3 +190,45 @@ def test_error_paths_return_codes(tmp_path):
     with pytest.raises(VcbctError) as e:
         io.attenuation_coefficient(str(tmp_path / "s.csv"), str(d))
     assert e.value.code == _lib.VCBCT_EIO
+
+
+def _build_shim_driver(tmp_path):
+    """The compat shim's source (compat_shim.cpp) linked with a host stand-in of the four C-ABI calls it makes
+    (tests/shim_host/stub_vcbct.cpp) and a driver that calls it as P/main.cpp:111-125 does."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = [os.path.join(ROOT, "virtualcbct_amd", "csrc", "compat_shim.cpp"),
+           os.path.join(ROOT, "tests", "shim_host", "stub_vcbct.cpp"),
+           os.path.join(ROOT, "tests", "shim_host", "drive_shim.cpp")]
+    exe = str(tmp_path / "drive_shim")
+    base = ["g++", "-std=c++17", "-O1", "-g", "-I", os.path.join(ROOT, "include"), *src, "-o", exe]
+    # address sanitizer on the host build when the toolchain has it: an overrun of the output buffer fails loudly
+    r = subprocess.run(base[:1] + ["-fsanitize=address", "-fno-omit-frame-pointer"] + base[1:], capture_output=True)
+    if r.returncode != 0:
+        subprocess.run(base, check=True)
+    return exe
+
+
+def test_compat_shim_reprojection_angle_range(tmp_path):
+    """reprojectionOnGPU writes angles [0, endProjNumber) (P/reprojectionOnGPU.cu:976), sized by endProjNumber;
+    an endProjNumber beyond initializeGPU's totalProjNumber, below 1, or an image size other than
+    initializeGPU's exits with status 1 and a message, never writing past the buffer."""
+    import subprocess
+    exe = _build_shim_driver(tmp_path)
+    H, W = 3, 5
+    a = np.arange(4)[:, None, None] * 1e6 + np.arange(H)[None, :, None] * 1e3 + np.arange(W)[None, None, :]
+    for total, end in ((4, 4), (4, 2), (4, 1)):
+        out = tmp_path / f"proj_{total}_{end}"
+        r = subprocess.run([exe, str(total), str(end), str(out)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        got = np.fromfile(str(out) + ".raw", np.float32)
+        assert got.size == end * H * W
+        assert np.array_equal(got.reshape(end, H, W), a[:end].astype(np.float32))
+    for total, end, extra, msg in ((4, 6, [], "6 angles requested"), (4, 0, [], "must be positive"),
+                                   (4, 4, ["7", "3"], "differs from initializeGPU")):
+        out = tmp_path / f"bad_{end}"
+        r = subprocess.run([exe, str(total), str(end), str(out), *extra], capture_output=True, text=True)
+        assert r.returncode == 1 and msg in r.stderr, (r.returncode, r.stderr)
+        assert not os.path.exists(str(out) + ".raw")

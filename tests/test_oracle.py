@@ -260,8 +260,8 @@ def test_z0_plane_not_split_and_slice0_extension():
 
 
 def test_adjoint_identity_and_gradient():
-    # <A x, y> = <x, A^T y> for the linear line-integral operator (through the adjoint kernel's math with a
-    # linearised loss), and the loss gradient against central differences.
+    # The loss and its gradient against central differences of orc_logproj, and the additivity of the
+    # line integrals.
     rng = np.random.default_rng(5)
     N, Nz = 12, 4
     g = _geom(N, Nz, 1.0, 1.0, 8, 10, [-170.0001, -40.00001, 59.99999], xs=0.02)
@@ -284,8 +284,8 @@ def test_adjoint_identity_and_gradient():
         lm = np.sum((O.logproj(g, xm.astype(np.float32), atten, pdf, 1.0) - target) ** 2)
         fd = (lp - lm) / (2 * h)
         assert grad[z, m, yy, xx] == pytest.approx(fd, rel=2e-2, abs=1e-6)
-    # linear part: L = A x;  <A x, w> == <x, A^T w> with A^T w from the gradient of 0.5*sum(w*L)... checked by
-    # building A column-by-column on a few voxels
+    # the line integrals are additive over (slice, material) blocks of the volume: L(x) = sum_b L(x_b)
+    # (the transpose identity through orc_adjoint itself is test_adjoint_transpose_identity_monochromatic)
     Lx, _, _ = O.line_integrals(g, x)
     w = rng.normal(0, 1, Lx.shape)
     lhs = np.sum(Lx * w)
@@ -320,3 +320,114 @@ def test_reference_build_agrees_when_present():
     a = O.filter_sinogram(proj, xs, ys)
     b, _ = O.ref_filter_sinogram(proj, xs, ys)
     assert np.array_equal(a, b)
+
+
+# ------------------------------------------------------------------------------------------- log projection
+def test_logproj_monochromatic_is_the_line_integral_sum():
+    # NE = 1: p = -log(pdf e^{-x} / pdf) = x = sum_m mu_m L_m / xf exactly, for every ray (L from the traversal
+    # pinned above against the reference's CPU twin).  A sign error, a dropped 1/xf or a wrong material index
+    # in the spectral sum fails this.
+    d = gold("reproj_small.npz")
+    g = _small_geom(d)
+    vol = d["vol"].astype(np.float32)
+    L, _, _ = O.line_integrals(g, vol)
+    for ke in (0, 3):
+        at = np.ascontiguousarray(d["atten"][:, ke:ke + 1])
+        for pdf0, xf in ((0.37, 1000.0), (2.0, 250.0)):
+            p = O.logproj(g, vol, at, np.array([pdf0], np.float32), xf)
+            x = np.einsum("ahwm,m->ahw", L, at[:, 0].astype(np.float64)) / xf  # L in the projection layout
+            np.testing.assert_allclose(p, x, rtol=1e-13, atol=1e-15)
+            assert p.max() > 0.1  # the fixture's rays do cross material
+
+
+def test_logproj_zero_volume_is_zero():
+    d = gold("reproj_small.npz")
+    g = _small_geom(d)
+    p = O.logproj(g, np.zeros((8, 6, 32, 32), np.float32), d["atten"], d["pdf"], 1000.0)
+    assert np.all(p == 0.0)
+
+
+def test_logproj_against_the_pinned_intensity():
+    # p = -log(I / (leng sum pdf)) with I from the energy-collapsed forward projection, itself equal to the
+    # reference's per-energy recompute (test above) and so to its CPU twin.  I is an FP32 sum of FP32 terms, so
+    # the two agree to its rounding (about NE x 6e-8 relative in I, the same absolute in p).
+    d = gold("reproj_small.npz")
+    g = _small_geom(d)
+    I = O.forward_collapsed(g, d["vol"], d["atten"], d["pdf"], 1000.0).astype(np.float64)
+    _, _, leng = O.line_integrals(g, d["vol"])
+    p = O.logproj(g, d["vol"].astype(np.float32), d["atten"], d["pdf"], 1000.0)
+    expect = -np.log(I / (leng * float(np.sum(d["pdf"].astype(np.float64)))))
+    np.testing.assert_allclose(p, expect, rtol=0, atol=2e-6)
+    assert p.max() > 0.1
+
+
+def test_logproj_between_jensen_bounds():
+    # -log E_w[e^{-x}] lies between min_e x_e and E_w[x] (convexity of e^{-x}), strictly for a ray whose
+    # exponents differ between bins: a flipped sign, a missing normalisation by sum pdf, or weights applied
+    # twice breaks one of the two inequalities.
+    d = gold("reproj_small.npz")
+    g = _small_geom(d)
+    vol = d["vol"].astype(np.float32)
+    L, _, _ = O.line_integrals(g, vol)
+    at, pdf, xf = d["atten"].astype(np.float64), d["pdf"].astype(np.float64), 1000.0
+    p = O.logproj(g, vol, d["atten"], d["pdf"], xf)
+    x = np.einsum("ahwm,me->ahwe", L, at) / xf
+    lo = x.min(axis=-1)
+    hi = (x * pdf).sum(axis=-1) / pdf.sum()
+    hit = hi > 1e-3
+    assert hit.sum() > 100
+    assert np.all(p[hit] > lo[hit]) and np.all(p[hit] < hi[hit])
+    assert np.all(p[~hit] <= hi[~hit] + 1e-15)
+
+
+def test_logproj_opaque_ray_closed_form():
+    # Rays that no FP32 exponential survives (x_e of 110 to 700 in every bin) stay finite in the FP64 oracle:
+    # p = x_min - log(sum_e w_e e^{-(x_e - x_min)} / sum w), evaluated here in closed form from the line
+    # integrals.  The GPU's opaque-ray path is compared against exactly these values.
+    g = _geom(16, 16, 1.0, 1.0, 24, 24, [-170.0001, 33.3])
+    g.nmat = 2
+    vol = np.zeros((16, 2, 16, 16), np.float32)
+    vol[:, 0] = 8.0
+    vol[:, 1] = 12.0
+    L, _, _ = O.line_integrals(g, vol)
+    at = np.array([[0.5, 0.9, 1.4], [0.6, 1.1, 0.8]], np.float32)
+    pdf = np.array([0.3, 0.5, 0.2], np.float32)
+    p = O.logproj(g, vol, at, pdf, 1.0)
+    x = np.einsum("ahwm,me->ahwe", L, at.astype(np.float64))
+    hit = x.min(axis=-1) > 110
+    assert hit.sum() >= 100 and x.max() < 700
+    xm = x.min(axis=-1, keepdims=True)
+    w = pdf.astype(np.float64)
+    expect = xm[..., 0] - np.log((w * np.exp(-(x - xm))).sum(-1) / w.sum())
+    assert np.all(np.isfinite(p))
+    np.testing.assert_allclose(p[hit], expect[hit], rtol=1e-13)
+
+
+def test_adjoint_transpose_identity_monochromatic():
+    # With one energy bin the log projection is linear in the maps, p(x) = A x (A: the traversal's chords
+    # times mu / xf), and orc_adjoint's gradient of sum (p - t)^2 at t = p(x) - w / 2 is A^T w.  So
+    # <A d, w> = <d, orc_adjoint(...)> for random d and w, to rounding: this pins the scatter (which voxel,
+    # which material, which weight) independently of finite differences.
+    rng = np.random.default_rng(11)
+    N, Nz = 12, 4
+    g = _geom(N, Nz, 1.0, 1.0, 8, 10, [-170.0001, -40.00001, 59.99999], xs=0.02)
+    g.nmat = 3
+    at = np.array([[0.25], [0.4], [0.15]], np.float32)
+    pdf = np.array([1.0], np.float32)
+    x = rng.uniform(0.5, 1.5, (Nz, 3, N, N)).astype(np.float32)
+    dlt = rng.normal(0, 1, (Nz, 3, N, N)).astype(np.float32)
+    px = O.logproj(g, x, at, pdf, 2.0)
+    w = rng.normal(0, 1, px.shape)
+    ATw, _ = O.adjoint(g, x, at, pdf, 2.0, px - w / 2)
+    # A d by linearity: p(x + d) - p(x) (both exact sums of the same chords)
+    Ad = O.logproj(g, (x.astype(np.float64) + dlt).astype(np.float32), at, pdf, 2.0) - px
+    lhs = float(np.sum(Ad * w))
+    rhs = float(np.sum(dlt.astype(np.float64) * ATw))
+    assert abs(lhs) > 1.0
+    assert rhs == pytest.approx(lhs, rel=1e-6)  # x + d rounded to FP32 once
+    # and a single voxel: the gradient there is the sum over its rays of w * mu_m * chord / xf
+    z, m, yy, xx = 2, 1, 6, 5
+    e = np.zeros_like(x)
+    e[z, m, yy, xx] = 1.0
+    Ae = O.logproj(g, e, at, pdf, 2.0)
+    assert ATw[z, m, yy, xx] == pytest.approx(float(np.sum(Ae * w)), rel=1e-12, abs=1e-14)

@@ -3,7 +3,8 @@
 `poly_logproj(maps, projector, xfactor)` returns the polychromatic log projection
     p = -log( sum_e pdf_e exp(-sum_m mu_me L_m / xfactor) / sum_e pdf_e )
 of six FP32 elemental density maps [mat][z][y][x] (a torch tensor on the device).  The forward keeps the six
-per-ray material line integrals (24 B per ray; the per-energy terms are never stored), and the backward runs
+per-ray material line integrals (24 B per ray; the per-energy terms are never stored, the backward recomputes
+them from the line integrals and the projector's spectrum, which must not change in between), and the backward runs
 the hand-written adjoint kernels: from them, the per-ray coefficients dL/dp * mubar_m / xfactor (mubar_m the
 transmission-weighted spectral mean of mu_m), scattered along a second traversal of the rays.  There is no reference counterpart; the reference's scatter U-Net (Unet_scatter.py)
 consumes projections of this kind.
@@ -25,14 +26,18 @@ class PolyLogProjection(torch.autograd.Function):
         p, L = projector.logproj_save(xfactor)
         ctx.projector = projector
         ctx.xfactor = xfactor
+        ctx.spectrum_generation = projector.spectrum_generation
         ctx.save_for_backward(L)
         return p
 
     @staticmethod
     def backward(ctx, grad_p):
         (L,) = ctx.saved_tensors
-        # the scatter reads only the geometry and the spectrum (fixed per projector), not the maps, so the
-        # projector may have been used with other maps since forward
+        # the backward reads the geometry and the projector's current spectrum, not the maps: the projector may
+        # have projected other maps since forward, but its spectrum must still be the one forward used
+        if ctx.projector.spectrum_generation != ctx.spectrum_generation:
+            raise RuntimeError("poly_logproj: the projector's spectrum was replaced between forward and backward; "
+                               "use one projector per spectrum")
         grad = ctx.projector.logproj_backward_saved(L, grad_p.contiguous(), ctx.xfactor)
         return grad, None, None
 

@@ -250,24 +250,37 @@ extern "C" int vcbct_projector_line_integrals(vcbct_projector* p, float* dev_L, 
                           sp, 1.0, 0, p->g.nAng, nullptr, VCBCT_MODE_LINEINT, dev_L, (hipStream_t)stream);
 }
 
-extern "C" int vcbct_reprojection(vcbct_projector* p, const uint16_t* host_vol, const float* atten, const float* pdf,
-                                  int NE, double xfactor, float* host_out) {
+extern "C" int vcbct_reprojection_angles(vcbct_projector* p, const uint16_t* host_vol, const float* atten,
+                                         const float* pdf, int NE, double xfactor, int n_out_angles,
+                                         float* host_out) {
     if (!p || !host_vol || !host_out) return fail(VCBCT_EINVAL, "reprojection: NULL argument");
+    // the reference's grid covers angles [0, endProjNumber) of the tables initializeGPU uploaded
+    // (P/reprojectionOnGPU.cu:976); more than those tables hold has no defined result
+    if (n_out_angles < 1 || n_out_angles > p->g.nAng)
+        return fail(VCBCT_EINVAL, "reprojection: %d angles requested, the projector was created for %d",
+                    n_out_angles, p->g.nAng);
     int rc = vcbct_projector_set_volume_u16(p, host_vol, 0, nullptr);
     if (rc) return rc;
     rc = vcbct_projector_set_spectrum(p, atten, pdf, NE);
     if (rc) return rc;
-    const size_t n = (size_t)p->g.nAng * p->g.H * p->g.W;
+    const size_t n = (size_t)n_out_angles * p->g.H * p->g.W;
     float* d_out = nullptr;
     if (dev_malloc((void**)&d_out, n * sizeof(float)) != hipSuccess)
         return fail(VCBCT_ENOMEM, "device allocation of the projections (%zu bytes) failed", n * sizeof(float));
-    rc = vcbct_projector_forward(p, d_out, xfactor, 0, -1, nullptr);
+    // angle a lands at a * H * W: the first n_out_angles projections fill d_out exactly
+    rc = vcbct_projector_forward(p, d_out, xfactor, 0, n_out_angles, nullptr);
     if (rc == 0) {
         hipError_t e = hipMemcpy(host_out, d_out, n * sizeof(float), hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = fail(VCBCT_EHIP, "projection download failed: %s", hipGetErrorString(e));
     }
     (void)hipFree(d_out);
     return rc;
+}
+
+extern "C" int vcbct_reprojection(vcbct_projector* p, const uint16_t* host_vol, const float* atten, const float* pdf,
+                                  int NE, double xfactor, float* host_out) {
+    if (!p) return fail(VCBCT_EINVAL, "reprojection: NULL argument");
+    return vcbct_reprojection_angles(p, host_vol, atten, pdf, NE, xfactor, p->g.nAng, host_out);
 }
 
 extern "C" int vcbct_projector_logproj(vcbct_projector* p, float* dev_p, double xfactor, void* stream) {

@@ -1,6 +1,6 @@
 // The reference's C++-linkage entry points, re-exported over the C ABI so that the reference's own host
 // programs link unchanged against libvcbct_compat.so:
-//   initializeGPU / reprojectionOnGPU / terminateGPU   (P/reprojectionOnGPU.h:76-93)
+//   initializeGPU / reprojectionOnGPU / terminateGPU   (P/reprojectionOnGPU.h:2-19)
 //   prepareConstMap / filterSinogram                   (F/filterSinogram.h:2-6)
 //   reconstructImage                                   (F/reconstructImage.h:2-5)
 // Behaviour follows the reference (same output file, same array ownership), except that failures print
@@ -14,6 +14,7 @@
 
 namespace {
 vcbct_projector* g_proj = nullptr;
+int g_width = 0, g_height = 0;  // projection size fixed by initializeGPU
 
 [[noreturn]] void die(const char* where) {
     fprintf(stderr, "%s: %s\n", where, vcbct_last_error());
@@ -46,6 +47,8 @@ void initializeGPU(int reconSize, double reconScale, int reconSlices, double rec
     if (g_proj) vcbct_projector_destroy(g_proj);
     if (vcbct_projector_create(&g_proj, &g, cosTable, sinTable, XcenterShift, YcenterShift, dv ? atoi(dv) : 0))
         die("initializeGPU");
+    g_width = projImgWidth;
+    g_height = projImgHeight;
 }
 
 void terminateGPU() {
@@ -69,11 +72,24 @@ void reprojectionOnGPU(int ite, int reconSize, int reconSlices, unsigned short* 
         fprintf(stderr, "reprojectionOnGPU: initializeGPU was not called\n");
         exit(1);
     }
-    // the reference computes angles [0, endProjNumber) (grid z, P/reprojectionOnGPU.cu:976) and writes them all
-    const size_t n = (size_t)endProjNumber * projImgWidth * projImgHeight;
+    // The reference computes angles [0, endProjNumber) (grid z, P/reprojectionOnGPU.cu:976; startProjNumber is
+    // not used) and writes them all.  The buffer holds exactly those angles, and the library computes exactly
+    // those; endProjNumber beyond initializeGPU's totalProjNumber (or below 1) is an error.  The image size
+    // is the one initializeGPU fixed; a different one here would not match the projector's buffers.
+    if (endProjNumber < 1) {
+        fprintf(stderr, "reprojectionOnGPU: endProjNumber %d must be positive\n", endProjNumber);
+        exit(1);
+    }
+    const size_t n = (size_t)endProjNumber * g_width * g_height;
+    if (projImgWidth != g_width || projImgHeight != g_height) {
+        fprintf(stderr, "reprojectionOnGPU: projection size %dx%d differs from initializeGPU's %dx%d\n",
+                projImgWidth, projImgHeight, g_width, g_height);
+        exit(1);
+    }
     std::vector<float> out(n);
-    // geometry (all angles) was fixed by initializeGPU; endProjNumber equals its angle count in P/main.cpp
-    if (vcbct_reprojection(g_proj, reconImageshort, Attenuation, pdf_pe, NE, xfactor, out.data())) die("reprojectionOnGPU");
+    if (vcbct_reprojection_angles(g_proj, reconImageshort, Attenuation, pdf_pe, NE, xfactor, endProjNumber,
+                                  out.data()))
+        die("reprojectionOnGPU");
     char filename[1024];
     snprintf(filename, sizeof filename, "%s.raw", outputname);
     FILE* fp = fopen(filename, "wb");

@@ -1,6 +1,6 @@
 // FDK on gfx950: ramp filter as a batched FFT convolution and voxel-driven cone-beam backprojection.
 //
-// Replaces filterSinogram (F/filterSinogram.cpp:122-186, a direct O(W^2) spatial convolution with
+// Replaces filterSinogram (F/filterSinogram.cpp:67-131, a direct O(W^2) spatial convolution with
 // -log() inside the inner loop) and reconstructImage (F/reconstructImage.cpp:10-184, OpenMP over z).
 #include <hip/hip_runtime.h>
 
@@ -20,7 +20,7 @@ namespace vcbct {
 static const double kPI = 3.141592653589793238462643383279;
 
 // ------------------------------------------------------------------------------------------------
-// Filter kernel h(d), F/filterSinogram.cpp:65-120 (off-centre taps) and :133-135 (centre coefficient).
+// Filter kernel h(d), F/filterSinogram.cpp:10-65 (off-centre taps) and :78-80 (centre coefficient).
 // FP64, the reference's operation order.
 static double ramp_offcentre(int md, double pitch_iso, int W) {
     const double CUT = 1.60, GAM = 1.00, ALP = 0.50;
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, f
         for (int k = threadIdx.x; k < L; k += FT) X[k] = cmul(X[k], HL[k]);
         __syncthreads();
         X = fft_lds<true>(X, Y, T, L, logL);
-        // cosine weight at the output pixel (F/filterSinogram.cpp:141-154)
+        // cosine weight at the output pixel (F/filterSinogram.cpp:92-99, applied at :103 and :108)
         for (int s = 0; s < 2; ++s) {
             const long r = r1 + s;
             if (s == 1 && !has2) break;
@@ -535,7 +535,7 @@ static std::vector<double> bp_table(int n, int W, const double* angles, const do
 }
 
 static std::vector<float> filter_table(int n, int W, const double* xshift, const double* yshift) {
-    // per-projection centre offsets, F/filterSinogram.cpp:141-142 (both with the projection WIDTH)
+    // per-projection centre offsets, F/filterSinogram.cpp:86-87 (both with the projection WIDTH)
     std::vector<float> tab(2 * (size_t)n);
     for (int th = 0; th < n; ++th) {
         tab[th] = (float)(-W / 2.0 - xshift[th] / (0.05 * 800 / W) - 1);

@@ -202,6 +202,94 @@ struct AccumL<float4> {
     }
 };
 
+// ------------------------------------------------------------------------------------------------
+// Spectral sums of opaque rays.  The fast paths form each energy term with an FP32 exponential, which
+// underflows once mu.L / xf exceeds ~87 (103 with denormals) in a bin; a ray that is that opaque in every bin
+// would give I = 0, p = -log(0) = +inf and a coefficient 0/0.  Such rays are recognised by their FP32 sum
+// (below 1e-30 of the spectrum's weight: every term then sits within a few decades of underflow) and their
+// terms are re-formed in FP64 from the FP64 tables, relative to the smallest exponent x_min (over the bins
+// with weight):  sum_e pdf_e e^{-x_e} = e^{-x_min} sum_e pdf_e e^{-(x_e - x_min)}.  Rare, so the FP64
+// exponentials cost nothing measurable.
+
+// All six material loops below run over the compile-time VCBCT_MAX_MAT with a guard on nmat, so the per-ray
+// arrays keep constant indices and stay in registers.
+
+// x_e = sum_m mu_me L_m / xf in FP64.
+__device__ __forceinline__ double spectral_x(const float (&L)[VCBCT_MAX_MAT], int nmat, const double* __restrict__ mu,
+                                             int NE, int e, double inv_xf) {
+    double x = 0.0;
+#pragma unroll
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+        if (m < nmat) x += ldg(mu, m * NE + e) * (double)L[m];
+    return x * inv_xf;
+}
+
+// x_min over the bins with pdf_e > 0 (0 if there are none).
+__device__ __forceinline__ double spectral_xmin(const float (&L)[VCBCT_MAX_MAT], int nmat,
+                                                const double* __restrict__ mu, const double* __restrict__ pdf, int NE,
+                                                double inv_xf) {
+    double xmin = 0.0;
+    bool first = true;
+    for (int e = 0; e < NE; ++e) {
+        if (!(ldg(pdf, e) > 0.0)) continue;
+        const double x = spectral_x(L, nmat, mu, NE, e, inv_xf);
+        if (first || x < xmin) xmin = x;
+        first = false;
+    }
+    return xmin;
+}
+
+// sum_e pdf_e e^{-(x_e - x_min)}, and with MUB also sum_e pdf_e e^{-(x_e - x_min)} mu_me into mub (FP64).
+template <bool MUB>
+__device__ __forceinline__ double spectral_shifted(const float (&L)[VCBCT_MAX_MAT], int nmat,
+                                                   const double* __restrict__ mu, const double* __restrict__ pdf,
+                                                   int NE, double inv_xf, double xmin, double (&mub)[VCBCT_MAX_MAT]) {
+    double se = 0.0;
+    for (int e = 0; e < NE; ++e) {
+        const double we = ldg(pdf, e) * exp(-(spectral_x(L, nmat, mu, NE, e, inv_xf) - xmin));
+        se += we;
+        if (MUB) {
+#pragma unroll
+            for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+                if (m < nmat) mub[m] += we * ldg(mu, m * NE + e);
+        }
+    }
+    return se;
+}
+
+// The intensity of an opaque ray with the reference's own terms, float(pdf_e * exp(-transp_e / xf) * leng)
+// summed in FP32 in e order (P/reprojectionOnGPU.cu:432, 1038-1044): most are zero or denormal in FP32.
+__device__ __forceinline__ float intensity_exact(const float (&L)[VCBCT_MAX_MAT], int nmat,
+                                                 const double* __restrict__ mu, const double* __restrict__ pdf, int NE,
+                                                 double inv_xf, double leng) {
+    float s = 0.0f;
+    for (int e = 0; e < NE; ++e) s += (float)(ldg(pdf, e) * exp(-spectral_x(L, nmat, mu, NE, e, inv_xf)) * leng);
+    return s;
+}
+
+// -log(sum_e pdf_e e^{-x_e} / sw) of an opaque ray.
+__device__ __forceinline__ double logproj_opaque(const float (&L)[VCBCT_MAX_MAT], int nmat,
+                                                 const double* __restrict__ mu, const double* __restrict__ pdf, int NE,
+                                                 double inv_xf, double sw) {
+    const double xmin = spectral_xmin(L, nmat, mu, pdf, NE, inv_xf);
+    double unused[VCBCT_MAX_MAT];
+    return xmin - log(spectral_shifted<false>(L, nmat, mu, pdf, NE, inv_xf, xmin, unused) / sw);
+}
+
+// The transmission-weighted spectral mean of mu of an opaque ray (mub[m] = mubar_m, already divided by the
+// sum) and its log projection.
+__device__ __forceinline__ double mubar_opaque(const float (&L)[VCBCT_MAX_MAT], int nmat, const double* __restrict__ mu,
+                                               const double* __restrict__ pdf, int NE, double inv_xf, double sw,
+                                               double (&mub)[VCBCT_MAX_MAT]) {
+    const double xmin = spectral_xmin(L, nmat, mu, pdf, NE, inv_xf);
+#pragma unroll
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m) mub[m] = 0.0;
+    const double se = spectral_shifted<true>(L, nmat, mu, pdf, NE, inv_xf, xmin, mub);
+#pragma unroll
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m) mub[m] /= se;
+    return xmin - log(se / sw);
+}
+
 // Counts the charged segments of a ray (the algorithmic work unit of the roofline model).
 struct CountSeg {
     int n = 0;
@@ -290,6 +378,8 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
                     if (m < g.nmat) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
                 s += ldg(pdff, e) * exp2f(tr * k2) * lf;
             }
+            // opaque ray: the FP32 exponentials underflow; the reference's own FP64 terms instead
+            if (s < 1e-30f) s = intensity_exact(acc.L, g.nmat, mu, pdf, NE, inv_xf, leng);
         }
         stg(out, o, s);
     } else {  // VCBCT_MODE_LOGPROJ: p = -log(sum_e pdf_e e^{-mu.L/xf} / sum_e pdf_e); FP32 terms, FP64 sums
@@ -304,7 +394,9 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
             sw += pe;
             se += pe * (double)exp2f(tr * k2);
         }
-        stg(out, o, (float)(-log(se / sw)));
+        // opaque ray (the FP32 exponentials underflow): FP64 terms relative to the smallest exponent
+        const double p = se < 1e-30 * sw ? logproj_opaque(acc.L, g.nmat, mu, pdf, NE, inv_xf, sw) : -log(se / sw);
+        stg(out, o, (float)p);
     }
 }
 
@@ -524,16 +616,25 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, c
         }
         const size_t nrays = (size_t)g.nAng * g.H * g.W;
         const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - j) * g.W + jj;
+        double p;
+        if (se < 1e-30 * sw) {  // opaque ray: FP64 terms relative to the smallest exponent
+            p = mubar_opaque(acc.L, g.nmat, mu, pdf, NE, inv_xf, sw, mub);
+        } else {
+            p = -log(se / sw);
+#pragma unroll
+            for (int m = 0; m < VCBCT_MAX_MAT; ++m) mub[m] /= se;
+        }
         double w;
         if (MODE == VCBCT_ADJ_LOSS) {
-            double p = -log(se / sw);
             double res = p - (double)ldg(tg, o);
             lossc = res * res;
             w = 2.0 * res;
         } else {
             w = (double)ldg(tg, o);
         }
-        for (int m = 0; m < g.nmat; ++m) stg(coef, (size_t)m * nrays + o, (float)(w * (mub[m] / se) * inv_xf));
+#pragma unroll
+        for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+            if (m < g.nmat) stg(coef, (size_t)m * nrays + o, (float)(w * mub[m] * inv_xf));
     }
     if (MODE == VCBCT_ADJ_LOSS && loss) {
         // wave reduction of the loss, one atomic per wave
@@ -556,7 +657,7 @@ __global__ __launch_bounds__(256) void k_adjoint_coef_saved(ProjGeom g, const fl
         float L[VCBCT_MAX_MAT];
 #pragma unroll
         for (int m = 0; m < VCBCT_MAX_MAT; ++m) L[m] = m < g.nmat ? ldg(Ls, (size_t)m * nrays + o) : 0.0f;
-        double se = 0.0, mub[VCBCT_MAX_MAT] = {0, 0, 0, 0, 0, 0};
+        double sw = 0.0, se = 0.0, mub[VCBCT_MAX_MAT] = {0, 0, 0, 0, 0, 0};
         const float k2 = (float)(-inv_xf * 1.4426950408889634);
         for (int e = 0; e < NE; ++e) {
             float tr = 0.0f;
@@ -565,13 +666,22 @@ __global__ __launch_bounds__(256) void k_adjoint_coef_saved(ProjGeom g, const fl
                 if (m < g.nmat) tr = fmaf(ldg(muf, m * NE + e), L[m], tr);
             const double pe = ldg(pdf, e);
             const double we = pe * (double)exp2f(tr * k2);
+            sw += pe;
             se += we;
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
                 if (m < g.nmat) mub[m] += we * ldg(mu, m * NE + e);
         }
+        if (se < 1e-30 * sw) {  // opaque ray: FP64 terms relative to the smallest exponent
+            (void)mubar_opaque(L, g.nmat, mu, pdf, NE, inv_xf, sw, mub);
+        } else {
+#pragma unroll
+            for (int m = 0; m < VCBCT_MAX_MAT; ++m) mub[m] /= se;
+        }
         const double w = (double)ldg(gp, o);
-        for (int m = 0; m < g.nmat; ++m) stg(coef, (size_t)m * nrays + o, (float)(w * (mub[m] / se) * inv_xf));
+#pragma unroll
+        for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+            if (m < g.nmat) stg(coef, (size_t)m * nrays + o, (float)(w * mub[m] * inv_xf));
     }
 }
 

@@ -25,7 +25,7 @@ def _stream(stream):
 
 
 def prepareConstMap(projWidth, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752):
-    """prepareConstMap (F/filterSinogram.cpp:65-120): the W x W FP64 filter matrix h(m - n)."""
+    """prepareConstMap (F/filterSinogram.cpp:10-65): the W x W FP64 filter matrix h(m - n)."""
     out = np.empty((projWidth, projWidth), np.float64)
     check(lib().vcbct_prepare_const_map(int(projWidth), float(DETECTOR_PITCH_CM_AT_ISO_CENT), ptr(out, P_D)))
     return out
@@ -49,7 +49,7 @@ def filter_sinogram_device(proj, xshift, yshift, DETECTOR_PITCH_CM_AT_ISO_CENT=0
 
 def filterSinogram(projHeight, projWidth, sngrmHeight, projVolume, xshift, yshift,
                    DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, device=0):
-    """filterSinogram (F/filterSinogram.cpp:122-186) on host arrays: returns fltdVoxData [th][n][m] FP32."""
+    """filterSinogram (F/filterSinogram.cpp:67-131) on host arrays: returns fltdVoxData [th][n][m] FP32."""
     torch = _torch()
     pv = np.ascontiguousarray(projVolume, np.float32).reshape(sngrmHeight, projHeight, projWidth)
     d = torch.from_numpy(pv).to(f"cuda:{device}")

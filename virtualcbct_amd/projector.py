@@ -1,5 +1,5 @@
 """Forward projector: the reference's initializeGPU / reprojectionOnGPU / terminateGPU interface
-(P/reprojectionOnGPU.h:76-93) over the HIP kernels of libvcbct.so.
+(P/reprojectionOnGPU.h:2-19) over the HIP kernels of libvcbct.so.
 
 Device tensors are torch tensors on the ROCm device (PyTorch is the allocator and stream provider);
 the computation is the hand-written HIP code behind the C ABI.
@@ -30,7 +30,7 @@ def _stream(stream):
 class Projector:
     """Device state of initializeGPU (P/reprojectionOnGPU.cu:720-825).
 
-    Arguments follow initializeGPU's (P/reprojectionOnGPU.h:76-82); `device` replaces the reference's
+    Arguments follow initializeGPU's (P/reprojectionOnGPU.h:2-8); `device` replaces the reference's
     hard-coded device 1.
     """
 
@@ -52,6 +52,7 @@ class Projector:
         check(lib().vcbct_projector_create(C.byref(h), C.byref(self.g), *[ptr(t, P_D) for t in tabs], self.device))
         self.h = h
         self.NE = 0
+        self.spectrum_generation = 0
         self._vol_keep = None
 
     # shape helpers
@@ -135,6 +136,8 @@ class Projector:
         at = np.ascontiguousarray(at[: self.g.num_material * ne])
         check(lib().vcbct_projector_set_spectrum(self.h, ptr(at, P_F), ptr(pdf, P_F), ne))
         self.NE = ne
+        # saved autograd state refers to the spectrum it was computed with; a backward pass checks this
+        self.spectrum_generation += 1
 
     # ------------------------------------------------------------------ compute
     def forward(self, xfactor, out=None, a0=0, a1=None, stream=None):
@@ -177,13 +180,26 @@ class Projector:
                                             C.c_void_p(loss.data_ptr()), float(xfactor), _stream(stream)))
         return grad, loss
 
+    @property
+    def n_rays(self):
+        return self.g.n_angles * self.g.proj_height * self.g.proj_width
+
+    def _check_dev(self, t, shape, name, dtype=None):
+        """t must be a contiguous tensor of `shape` (and dtype, FP32 by default) on this projector's device:
+        the kernels index it by the projector's geometry, not by the tensor's size."""
+        torch = _torch()
+        dtype = torch.float32 if dtype is None else dtype
+        if (t.dtype != dtype or not t.is_cuda or t.device.index != self.device or not t.is_contiguous()
+                or tuple(t.shape) != tuple(shape)):
+            raise ValueError(f"{name} must be a contiguous {dtype} tensor of shape {tuple(shape)} on cuda:{self.device}")
+
     def logproj_save(self, xfactor, out=None, stream=None):
         """(log projection, its six line integrals per ray [nmat][n_angles*H*W]) for logproj_backward_saved."""
         torch = _torch()
         if out is None:
             out = torch.empty(self.shape_proj, dtype=torch.float32, device=f"cuda:{self.device}")
-        n = out.numel()
-        L = torch.empty((self.g.num_material, n), dtype=torch.float32, device=f"cuda:{self.device}")
+        self._check_dev(out, self.shape_proj, "out")
+        L = torch.empty((self.g.num_material, self.n_rays), dtype=torch.float32, device=f"cuda:{self.device}")
         check(lib().vcbct_projector_logproj_save(self.h, C.c_void_p(out.data_ptr()), C.c_void_p(L.data_ptr()),
                                                  float(xfactor), _stream(stream)))
         return out, L
@@ -191,9 +207,11 @@ class Projector:
     def logproj_backward_saved(self, L, grad_p, xfactor, stream=None):
         """The backward pass from the line integrals kept by logproj_save (the volume is not read)."""
         torch = _torch()
+        if not torch.is_tensor(grad_p) or grad_p.numel() != self.n_rays or not grad_p.is_cuda \
+                or grad_p.device.index != self.device or not grad_p.dtype.is_floating_point:
+            raise ValueError(f"grad_p must be a floating-point tensor of {self.n_rays} rays on cuda:{self.device}")
         gp = grad_p.contiguous().to(torch.float32)
-        if tuple(L.shape) != (self.g.num_material, gp.numel()) or L.dtype != torch.float32 or not L.is_contiguous():
-            raise ValueError("L must be the [nmat][rays] FP32 tensor returned by logproj_save")
+        self._check_dev(L, (self.g.num_material, self.n_rays), "L (the line integrals returned by logproj_save)")
         grad = torch.empty(self.shape_maps, dtype=torch.float32, device=f"cuda:{self.device}")
         check(lib().vcbct_projector_logproj_backward_saved(self.h, C.c_void_p(L.data_ptr()),
                                                            C.c_void_p(gp.data_ptr()), C.c_void_p(grad.data_ptr()),
@@ -210,7 +228,7 @@ class Projector:
 
 
 # ---------------------------------------------------------------------------------------------------
-# The reference's module-level interface (P/reprojectionOnGPU.h:76-93), one projector per process.
+# The reference's module-level interface (P/reprojectionOnGPU.h:2-19), one projector per process.
 
 _current: Projector | None = None
 
@@ -248,11 +266,11 @@ def reprojectionOnGPU(reconImageshort, xfactor, Attenuation, pdf_pe, NE, outputn
     vol = np.ascontiguousarray(reconImageshort, np.uint16)
     at = np.ascontiguousarray(np.asarray(Attenuation, np.float32).ravel()[: p.g.num_material * NE])
     pdf = np.ascontiguousarray(np.asarray(pdf_pe, np.float32).ravel()[:NE])
-    out = np.empty(p.shape_proj, np.float32)
-    check(lib().vcbct_reprojection(p.h, ptr(vol, P_U16), ptr(at, P_F), ptr(pdf, P_F), int(NE), float(xfactor),
-                                   ptr(out, P_F)))
-    if endProjNumber is not None:
-        out = out[:endProjNumber]
+    # the reference computes and writes angles [0, endProjNumber) (P/reprojectionOnGPU.cu:976)
+    n_out = p.g.n_angles if endProjNumber is None else int(endProjNumber)
+    out = np.empty((n_out,) + p.shape_proj[1:], np.float32)
+    check(lib().vcbct_reprojection_angles(p.h, ptr(vol, P_U16), ptr(at, P_F), ptr(pdf, P_F), int(NE),
+                                          float(xfactor), n_out, ptr(out, P_F)))
     if outputname:
         out.tofile(f"{outputname}.raw")
     return out
