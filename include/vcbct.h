@@ -215,6 +215,19 @@ int vcbct_reconstruct_image(int recon_size, int recon_slices, int zstart_cm, dou
                             const float* dev_filtered, int xfactor, const double* host_angles_rad,
                             const double* host_xshift, double det_pitch_iso_cm, void* stream);
 
+/* vcbct_reconstruct_image without any allocation (no reference counterpart; for callers that cannot
+ * allocate device memory: code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The angles and x shifts are DEVICE arrays of n_proj doubles here; the per-angle table the backprojector
+ * reads (4 doubles per projection) is built by a kernel into a file-scope device array of the library
+ * (512 KiB per GPU, held until the process ends), so n_proj is limited to 16384 (VCBCT_EINVAL beyond).
+ * Same arithmetic and result as vcbct_reconstruct_image (up to an ulp of the device cos/sin in the table).
+ * Asynchronous on `stream`.  Every call on one GPU uses that GPU's single table: calls that overlap on one
+ * GPU (other streams or threads) race and must be serialised by the caller. */
+int vcbct_reconstruct_image_noalloc(int recon_size, int recon_slices, int zstart_cm, double recon_step,
+                                    double recon_scale, float* dev_recon, int proj_height, int proj_width, int n_proj,
+                                    const float* dev_filtered, int xfactor, const double* dev_angles_rad,
+                                    const double* dev_xshift, double det_pitch_iso_cm, void* stream);
+
 /* Host-array forms of the two FDK calls for drop-in callers (F/main.cpp:133-137, 170-175): inputs are the
  * contiguous new3DArray blocks; reconstruct_image_host accumulates (+=) into the caller's FP64 volume. */
 int vcbct_filter_sinogram_host(int proj_height, int proj_width, int n_proj, const float* host_proj,

@@ -854,6 +854,83 @@ extern "C" int vcbct_reconstruct_image(int R, int Z, int zstart, double step, do
     return rc;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Allocation-free backprojection (no reference counterpart): vcbct_reconstruct_image with its per-call
+// angle table (cos, sin, dbeta, xshift in pixels; 4 doubles per projection) built by a kernel into a
+// file-scope device array instead of a hipMalloc'd buffer, for callers that cannot allocate.  The array
+// lives in every GPU the library runs on until the process ends; its device address is looked up once
+// per device (host code never uses the symbol itself: the name stands for a host-side shadow).
+
+namespace vcbct {
+constexpr int NOALLOC_MAX_PROJ = 16384;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) double scratch[4 * NOALLOC_MAX_PROJ];  // 512 KiB per device
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[NOALLOC_MAX_DEVICES];  // device address of `scratch` per device, 0 until looked up
+
+// The current device's copy of `scratch` (hipGetSymbolAddress on its first use on that device).
+static int scratch_on_current_device(double** out) {
+    int dev = -1;
+    VCBCT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(VCBCT_EINVAL, "noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(VCBCT_EHIP, "noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_dev[dev] = (double*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+// The per-angle table of bp_table, one thread per projection, from device copies of the angles and shifts
+// (same expressions; FP64 division and the neighbour rule of F/reconstructImage.cpp:44-71).
+__global__ __launch_bounds__(256) void k_bp_table(const double* __restrict__ angles, const double* __restrict__ xshift,
+                                                  int n, int W, double* __restrict__ tab) {
+    const int th = blockIdx.x * 256 + threadIdx.x;
+    if (th >= n) return;
+    const double kp = 3.141592653589793238462643383279;
+    double dbeta;
+    if (th == 0) {
+        dbeta = fabs(angles[1] - angles[0]) / 2;
+    } else if (th == n - 1) {
+        dbeta = fabs(angles[th] - angles[th - 1]) / 2;
+    } else {
+        const double thdb0 = angles[th - 1], thdb2 = angles[th + 1];
+        double thdb1 = angles[th + 1] - 2 * kp;
+        if (thdb2 < 0) thdb1 = angles[th + 1] + 2 * kp;
+        dbeta = fabs(thdb2 - thdb0) / 2;
+        if (dbeta > kp / 2) dbeta = fabs(thdb1 - thdb0) / 2;
+    }
+    tab[th] = cos(angles[th]);
+    tab[n + th] = sin(angles[th]);
+    tab[2 * n + th] = dbeta;
+    tab[3 * n + th] = xshift[th] / (0.05 * 800 / W);
+}
+}  // namespace vcbct
+
+extern "C" int vcbct_reconstruct_image_noalloc(int R, int Z, int zstart, double step, double scale, float* dev_recon,
+                                               int H, int W, int n, const float* dev_sino, int xfactor,
+                                               const double* dev_angles, const double* dev_xshift, double pitch_iso,
+                                               void* stream) {
+    if (R < 1 || Z < 1 || H < 2 || W < 2 || n < 2 || !dev_recon || !dev_sino || !dev_angles || !dev_xshift)
+        return fail(VCBCT_EINVAL, "reconstruct_image_noalloc: bad arguments");
+    if (n > NOALLOC_MAX_PROJ)
+        return fail(VCBCT_EINVAL, "reconstruct_image_noalloc: %d projections exceed the maximum of %d", n,
+                    NOALLOC_MAX_PROJ);
+    double* tab = nullptr;
+    int rc = scratch_on_current_device(&tab);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_bp_table, dim3((n + 255) / 256), dim3(256), 0, st, dev_angles, dev_xshift, n, W, tab);
+    VCBCT_HIP(hipGetLastError());
+    // four slices per thread (the kernels give the same sums for every slice count)
+    return launch_bp(4, R, Z, zstart, step, scale, dev_recon, H, W, n, dev_sino, tab, xfactor, pitch_iso, st);
+}
+
 extern "C" int vcbct_filter_sinogram_host(int H, int W, int n, const float* hp, float* ho, const double* xs,
                                           const double* ys, double pitch_iso) {
     if (!hp || !ho) return fail(VCBCT_EINVAL, "filter_sinogram_host: NULL argument");
