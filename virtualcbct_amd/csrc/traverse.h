@@ -363,20 +363,38 @@ __host__ __device__ __forceinline__ void traverse_steps(const ProjGeom& g, const
     }
 }
 
-// The wave-uniform part of plane step k (the xy path is shared by the lanes of a wave): plane position,
-// minor-axis crossing, the step's major-axis cell term and whether the reference's loop still runs (the
-// previous plane inside the box).  Same expressions, in the same order, as the per-step loop above.
+// The wave-uniform part of plane step k (the xy path is shared by the lanes of a wave), computed for 64 steps
+// at a time, one step per lane, and read back by every lane in step order.  A step's segments are the one
+// or two pieces of its xy chord split at the minor-axis crossing (the same on every lane), one of which a
+// lane's own z crossing may split again.  Everything about a piece that depends only on its xy chord is
+// decided here, once per step: the plane positions, the minor crossing, the piece's voxel column (its minor
+// index from the piece's midpoint, and the step's major index), whether that column is in range, and how the
+// inscribed-circle mask treats the piece.  The piece of a z split lies inside its xy piece, so it has the
+// same column (a piece's minor cell is that of any of its points; only a piece shorter than ~1e-12 cm could
+// round its midpoint across a minor boundary, and such a piece charges nothing measurable), and the same
+// circle decision unless the circle's boundary crosses the xy piece: those rare steps carry STEP_CIRC and
+// every lane tests its own midpoints there.  Positions are kept relative to the source along the major axis
+// (d = u - su), the form in which a midpoint's z is formed: zm = d_mid * dz + sz.
 struct StepRec {
-    double uk, duk, uv;  // plane k, plane k - su, minor crossing (== uk when none)
-    int majterm;         // imaj * majmul, or -1 when no segment of the step can be charged
-    int flags;           // STEP_HV: minor-axis crossing in this step
+    double dk;     // plane k, uk - su
+    double dv;     // minor crossing, uv - su (== dk when there is none)
+    int cb0, cb1;  // voxel-column base (minor term + major term) of the first and second xy piece
+    int flags;     // STEP_* below
+    int majterm;   // imaj * majmul (for a segment that spans the minor crossing)
 };
-enum { STEP_HV = 1 };
+enum {
+    STEP_SKIP = 1,    // no segment of the step can be charged (column out of range, or wholly outside the circle)
+    STEP_SPLITV = 2,  // the minor crossing splits the chord into two xy pieces
+    STEP_OK0 = 4,     // first xy piece chargeable (column in range, not wholly outside the circle)
+    STEP_OK1 = 8,     // second xy piece chargeable
+    STEP_CIRC = 16,   // the circle's boundary crosses the chord: test each midpoint
+};
 
-// A lane's held z-split piece (see traverse_table).
+// A lane's held z-split piece (see traverse_table): [a, b] relative to the source, its column base and its
+// xy piece's STEP_OK / STEP_CIRC bits.
 struct HeldPiece {
-    double a, b;  // the piece [a, b]
-    int maj;      // its step's major-axis term
+    double a, b;
+    int cb, flags;
 };
 
 // Per-wave scratch of the table walk (LDS on the GPU): the step table and one held piece per lane.
@@ -384,6 +402,15 @@ struct WaveLds {
     StepRec steps[64];
     HeldPiece held[64];
 };
+
+// Squared distance bounds of the xy chord from (u0, v0) to (u1, v1) to the volume's axis: the largest is at an
+// end (r^2 is convex along a line), the smallest is at least that of the chord's bounding box.
+__host__ __device__ __forceinline__ void chord_r2(double u0, double v0, double u1, double v1, double& lo, double& hi) {
+    const double au = (u0 > 0.0) == (u1 > 0.0) ? fmin(fabs(u0), fabs(u1)) : 0.0;
+    const double av = (v0 > 0.0) == (v1 > 0.0) ? fmin(fabs(v0), fabs(v1)) : 0.0;
+    lo = au * au + av * av;
+    hi = fmax(u0 * u0 + v0 * v0, u1 * u1 + v1 * v1);
+}
 
 __host__ __device__ __forceinline__ bool make_step(const ProjGeom& g, const RayCtx& r, const ChargeC& c, int k,
                                                    StepRec& t) {
@@ -402,19 +429,37 @@ __host__ __device__ __forceinline__ bool make_step(const ProjGeom& g, const RayC
         const double vb = (vma * g.s) - ((vk <= 0 && r.dirv < 0) || (vk >= 0 && r.dirv > 0) ? 0.0 : r.dirv * g.s);
         uv = (vb - r.sv) * r.duv + r.su;
     }
+    // a crossing at the end plane splits nothing (the reference's crossing distance is 0 there)
+    const bool splitv = hv && fabs(uk - uv) != 0.0;
     const int imaj = (int)((up + uk) * 0.5 * c.smaj + c.nmaj);
-    // The step's segments have their midpoints on the chord from (up, vp) to (uk, vk), inside the chord's
-    // bounding box (one voxel wide).  When that box lies wholly outside the inscribed circle (radius^2
-    // rmax2; margin 1e-9 cm^2, while the rounding of a midpoint is ~1e-13 cm^2) the mask rejects every
-    // segment of the step and it is skipped.
-    const double au = (up > 0.0) == (uk > 0.0) ? fmin(fabs(up), fabs(uk)) : 0.0;
-    const double av = (vp > 0.0) == (vk > 0.0) ? fmin(fabs(vp), fabs(vk)) : 0.0;
-    const bool outside = au * au + av * av > c.rmax2 + 1e-9;
-    t.uk = uk;
-    t.duk = duk;
-    t.uv = uv;
-    t.majterm = ((unsigned)imaj < (unsigned)g.N && !outside) ? imaj * c.majmul : -1;
-    t.flags = hv ? STEP_HV : 0;
+    const bool majok = (unsigned)imaj < (unsigned)g.N;
+    const int majterm = imaj * c.majmul;
+    // xy pieces [up, a1] and [uv, uk]; their columns from their midpoints, as a segment's (make_seg)
+    const double a1 = splitv ? uv : uk;
+    const double vv = (uv - r.su) * r.dv + r.sv;  // v at the minor crossing
+    const double va1 = splitv ? vv : vk;
+    const double m0 = (up + a1) * 0.5 - r.su;
+    const double m1 = (uv + uk) * 0.5 - r.su;
+    const int imin0 = (int)((m0 * r.dv + r.sv) * c.smin + c.nmaj);
+    const int imin1 = (int)((m1 * r.dv + r.sv) * c.smin + c.nmaj);
+    // circle mask over each piece: wholly inside (margin 1e-9 cm^2, while a midpoint's rounding is ~1e-13 cm^2),
+    // wholly outside, or crossed (then tested per midpoint)
+    double lo0, hi0, lo1, hi1;
+    chord_r2(up, vp, a1, va1, lo0, hi0);
+    chord_r2(uv, vv, uk, vk, lo1, hi1);
+    const bool out0 = lo0 > c.rmax2 + 1e-9, out1 = !splitv || lo1 > c.rmax2 + 1e-9;
+    const bool circ = (!out0 && hi0 >= c.rmax2 - 1e-9) || (!out1 && hi1 >= c.rmax2 - 1e-9);
+    const bool ok0 = majok && (unsigned)imin0 < (unsigned)g.N && !out0;
+    const bool ok1 = splitv && majok && (unsigned)imin1 < (unsigned)g.N && !out1;
+    t.dk = duk;
+    t.dv = uv - r.su;
+    t.cb0 = (int)((unsigned)imin0 * (unsigned)c.mstride + (unsigned)majterm);
+    t.cb1 = (int)((unsigned)imin1 * (unsigned)c.mstride + (unsigned)majterm);
+    t.majterm = majterm;
+    // a step none of whose pieces can be charged only moves the z state on; a tie of the minor and z crossing
+    // distances makes the whole chord one segment, so such a step is kept whenever its column is in range
+    t.flags = (!majok || (out0 && out1) ? STEP_SKIP : 0) | (splitv ? STEP_SPLITV : 0) | (ok0 ? STEP_OK0 : 0) |
+              (ok1 ? STEP_OK1 : 0) | (circ ? STEP_CIRC : 0);
     return inprev;
 }
 
@@ -427,40 +472,83 @@ __host__ __device__ __forceinline__ bool any_lane(bool b) {
 #endif
 }
 
-// The aligned-grid walk with the wave-uniform step data computed 64 steps at a time, one step per lane, into
-// a per-wave table (`tab`, 64 records; LDS on the GPU), and read back by every lane in step order.  Only
-// the per-lane z part (z crossing, crossing order, segments) stays in the sequential loop.  Requires the
-// lanes to share their xy path bit for bit (caller checks).
-// A step's segments are those of the wave-uniform minor-axis split (one or two, the same on every lane),
-// one of which a lane's z crossing may split again.  The z crossings are sparse per lane (a few per cent of
-// the lanes in a step) but, over 64 rows, present in almost every step: so the second piece of a z split is
-// not handed over in the step (which would run a mostly idle extra slot for the whole wave) but held by the
-// lane (in LDS), and the wave hands over all held pieces together when a lane that holds one meets its
-// next z crossing, and at the end.  The segments are the same; only the order of their sums changes.
-template <bool MUL24, class V>
+// Per-lane constants of the table walk: the z slope and its inverse, the segment length per unit of d, and
+// the z offset of the midpoint rule.
+struct LaneZ {
+    double hdz;   // dz / 2: zm = (a + b) * hdz + sz for the piece [a, b]
+    double K;     // segment length per |du|
+};
+
+// One piece [a, b] (relative to the source) of column base cb: its slice from the midpoint and its length.
+__host__ __device__ __forceinline__ Seg make_piece(const ProjGeom& g, const RayCtx& r, const ChargeC& c,
+                                                   const LaneZ& lz, double a, double b, int cb, bool on) {
+#pragma clang fp contract(fast)
+    Seg sg;
+    const double zm = (a + b) * lz.hdz + r.sz;
+    const int zmc = (int)(zm * c.ninv_t + c.nzm);
+    sg.ok = on && (unsigned)zmc < (unsigned)g.Nz;
+    sg.cell = (int)((unsigned)cb + (unsigned)zmc);
+    sg.rr = fabs(b - a) * lz.K;
+    return sg;
+}
+
+// The column base of a segment that spans the minor crossing (a tie of the crossing distances), from its own
+// midpoint; -1 when the column is out of range.
+__host__ __device__ __forceinline__ int span_base(const ProjGeom& g, const RayCtx& r, const ChargeC& c, double a,
+                                                  double b, int majterm) {
+#pragma clang fp contract(fast)
+    const double dm = (a + b) * 0.5;
+    const int imin = (int)((dm * r.dv + r.sv) * c.smin + c.nmaj);
+    return (unsigned)imin < (unsigned)g.N ? (int)((unsigned)imin * (unsigned)c.mstride + (unsigned)majterm) : -1;
+}
+
+// The inscribed-circle mask at the midpoint of [a, b] (relative to the source), as make_seg tests it.
+__host__ __device__ __forceinline__ bool in_circle(const RayCtx& r, const ChargeC& c, double a, double b) {
+#pragma clang fp contract(fast)
+    const double dm = (a + b) * 0.5;
+    const double um = dm + r.su;
+    const double vm = dm * r.dv + r.sv;
+    return um * um + vm * vm < c.rmax2;
+}
+
+// The aligned-grid walk with the step records of 64 steps at a time in a per-wave table (`lds`; LDS on the
+// GPU).  Only the z part stays per lane: the z crossing of each step, which piece it splits, and each
+// piece's slice and length.  Requires the lanes to share their xy path bit for bit (caller checks).
+// The z crossings are sparse per lane (a few per cent of the lanes in a step) but, over 64 rows, present in
+// almost every step: so the second piece of a z split is not handed over in the step (which would run a
+// mostly idle extra slot for the whole wave) but held by the lane (in LDS), and the wave hands over all held
+// pieces together when a lane that holds one meets its next z crossing, and at the end.  The segments are
+// the same; only the order of their sums changes.  The step body has one hand-over to the visitor, with both
+// xy pieces (the second charges nothing on steps without a minor crossing), and no branch that depends on a
+// single lane's data: the rare cases (a piece near the circle's boundary, a tie of the crossing distances)
+// are wave-uniform branches around their extra arithmetic.
+template <class V>
 __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
                                                         int kmax, V& vis, WaveLds* lds) {
 #pragma clang fp contract(fast)
     const double lim_u = g.lim_u;
-    const double inv_t = g.inv_t;
     const bool on = active && r.hit;
     const ChargeC c = charge_consts(g, r);
+    LaneZ lz;
+    lz.hdz = 0.5 * r.dz;
+    lz.K = r.K;
 
     double u0 = r.uin;
     double v0 = (u0 - r.su) * r.dv + r.sv;
     bool inbox0 = (u0 <= lim_u && u0 >= -lim_u && v0 <= lim_u && v0 >= -lim_u);
     if (!inbox0) return;
     int k = kmin;
-    double b0;
+    double b0;  // the previous plane, relative to the source
     if (k <= 1) {
         const Seg none{0, 0.0, false};
-        vis.step(make_seg<false, MUL24>(g, r, c, r.ui, u0, on, 0, false), none, none);
+        vis.step(make_seg<false, false>(g, r, c, r.ui, u0, on, 0, false), none, none);
         k = 1;
-        b0 = u0;
+        b0 = u0 - r.su;
     } else {
-        b0 = plane_u(r, k - 1, g.s);
+        b0 = plane_u(r, k - 1, g.s) - r.su;
     }
-    int zmaprev = (int)(((b0 - r.su) * r.dz + r.sz) * inv_t);
+    const double inv_t = g.inv_t;
+    int zmaprev = (int)(((b0 + b0) * lz.hdz + r.sz) * inv_t);
     const double dt = r.dirz * g.t;
     const Seg none{0, 0.0, false};
     StepRec* tab = lds->steps;
@@ -470,6 +558,15 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
     HeldPiece* hp = &lds->held[0];
 #endif
     bool held = false;
+
+    // hands the held pieces over (a wave-wide slot; lanes without one charge nothing); the midpoint test of
+    // the circle is exact for every piece, so it is applied to all of them
+    auto flush = [&]() {
+        const HeldPiece q = *hp;
+        const bool qon = held && (q.flags & STEP_OK0) && ((q.flags & STEP_CIRC) == 0 || in_circle(r, c, q.a, q.b));
+        vis.step(make_piece(g, r, c, lz, q.a, q.b, q.cb, qon), none, none);
+        held = false;
+    };
 
     for (;;) {
         // steps k .. k+63: record and validity (k' <= kmax and the reference's loop still running)
@@ -492,68 +589,70 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
         }
 #endif
         for (int i = 0; i < nsteps; ++i) {
-            const StepRec t = tab[i];
+            const double dk = tab[i].dk;
 #ifdef __HIP_DEVICE_COMPILE__
             // the record is the same for every lane: its integer fields in scalar registers make the
             // branches on them scalar
-            const int majterm = __builtin_amdgcn_readfirstlane(t.majterm);
-            const int flags = __builtin_amdgcn_readfirstlane(t.flags);
+            const int flags = __builtin_amdgcn_readfirstlane(tab[i].flags);
 #else
-            const int majterm = t.majterm, flags = t.flags;
+            const int flags = tab[i].flags;
 #endif
-            const double uk = t.uk;
-            const double zk = t.duk * r.dz + r.sz;
+            const double zk = (dk + dk) * lz.hdz + r.sz;  // dk * dz + sz exactly (doubling and halving are exact)
             const int zma = (int)(zk * inv_t);
-            if (majterm < 0) {  // nothing charged in this step: only the z state moves on
-                b0 = uk;
+            if (flags & STEP_SKIP) {  // nothing charged in this step: only the z state moves on
+                b0 = dk;
                 zmaprev = zma;
                 continue;
             }
+#ifdef __HIP_DEVICE_COMPILE__
+            int cb0 = __builtin_amdgcn_readfirstlane(tab[i].cb0);
+            const int cb1 = __builtin_amdgcn_readfirstlane(tab[i].cb1);
+#else
+            int cb0 = tab[i].cb0;
+            const int cb1 = tab[i].cb1;
+#endif
+            const double dv = tab[i].dv;
             const bool hz = zma != zmaprev;
-            if (any_lane(hz && held)) {  // a lane that holds a piece meets a z crossing: hand over every held
-                const HeldPiece q = *hp;  // piece first (at the start of the step, where few values are live)
-                vis.step(make_seg<true, MUL24>(g, r, c, q.a, q.b, held, q.maj, true), none, none);
-                held = false;
-            }
-            const bool hv = (flags & STEP_HV) != 0;
-            const double uv = t.uv;
-            double uz = uk;
-            if (hz) {
-                const double zb = (zma * g.t) - ((zk < 0 && r.dirz < 0) || (zk > 0 && r.dirz > 0) ? 0.0 : dt);
-                uz = (zb - r.sz) * r.duz + r.su;
-            }
+            if (any_lane(hz && held)) flush();  // a lane that holds a piece meets a z crossing: hand over first
+            // the z crossing, relative to the source (formed on every lane, used where hz)
+            const double zb = (zma * g.t) - ((zk < 0 && r.dirz < 0) || (zk > 0 && r.dirz > 0) ? 0.0 : dt);
+            const double dzc = hz ? (zb - r.sz) * r.duz : dk;
             // Crossings ordered by distance from the end plane (the farther one comes first; equal distances
-            // drop both, as no branch of the reference matches), a crossing at the end plane splits nothing.
-            const double chv = fabs(uk - uv), chz = fabs(uk - uz);
-            const bool splitv = hv && chv != 0;  // wave-uniform: uk and uv are
-            const bool splitz = hz && chz != 0;
-            double a1 = splitv ? uv : uk;        // first piece [b0, a1]
-            double c1 = uk;                      // second piece [uv, c1] (splitv only)
-            bool onb = on && splitv;
-            double h0 = 0.0, h1 = 0.0;
-            bool hold = false;
-            if (splitv && splitz && chz == chv) {  // both crossings dropped: one segment
-                a1 = uk;
-                onb = false;
-            } else if (splitz) {
-                if (!splitv || chz > chv) {  // z crossing inside the first piece
-                    h0 = uz;
-                    h1 = a1;
-                    a1 = uz;
-                } else {  // inside the second
-                    h0 = uz;
-                    h1 = uk;
-                    c1 = uz;
-                }
-                hold = on;
-            }
-            if (hold) {
-                *hp = HeldPiece{h0, h1, majterm};
+            // drop both, as no branch of the reference matches); a crossing at the end plane splits nothing.
+            const bool splitv = (flags & STEP_SPLITV) != 0;  // wave-uniform
+            const double chz = fabs(dk - dzc), chv = fabs(dk - dv);
+            const bool splitz = hz && chz != 0.0;
+            const bool tie = splitv && splitz && chz == chv;
+            const bool z0 = splitz && !tie && (!splitv || chz > chv);  // z crossing inside the first piece
+            const bool z1 = splitz && !tie && !z0;                     // inside the second
+            const double a1 = splitv ? dv : dk;                        // first piece [b0, a1]
+            if (on && (z0 || z1)) {  // the piece beyond the z crossing is held
+                *hp = HeldPiece{dzc, z0 ? a1 : dk, z0 ? cb0 : cb1, z0 ? (flags & (STEP_OK0 | STEP_CIRC))
+                                                                      : ((flags & STEP_OK1) ? STEP_OK0 : 0) |
+                                                                            (flags & STEP_CIRC)};
                 held = true;
             }
-            vis.step(make_seg<true, MUL24>(g, r, c, b0, a1, on, majterm, true),
-                     make_seg<true, MUL24>(g, r, c, uv, c1, onb, majterm, true), none);
-            b0 = uk;
+            double e0 = z0 ? dzc : a1;
+            const double e1 = z1 ? dzc : dk;
+            bool on0 = on && (flags & STEP_OK0);
+            const bool on1 = on && !tie && (flags & STEP_OK1);
+            bool circ = (flags & STEP_CIRC) != 0;
+            if (any_lane(on && tie)) {  // lanes with a tie charge the whole chord as one segment, from its midpoint
+                const int cbt = span_base(g, r, c, b0, dk, tab[i].majterm);
+                if (tie) {
+                    e0 = dk;
+                    cb0 = cbt;
+                    on0 = on && cbt >= 0;
+                }
+                circ = true;
+            }
+            bool on1c = on1;
+            if (circ) {  // a piece near the circle's boundary: the midpoint tests
+                on0 = on0 && in_circle(r, c, b0, e0);
+                on1c = on1c && in_circle(r, c, dv, e1);
+            }
+            vis.step(make_piece(g, r, c, lz, b0, e0, cb0, on0), make_piece(g, r, c, lz, dv, e1, cb1, on1c), none);
+            b0 = dk;
             zmaprev = zma;
         }
 #ifdef __HIP_DEVICE_COMPILE__
@@ -562,10 +661,7 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
         if (nsteps < 64) break;
         k += 64;
     }
-    if (any_lane(held)) {
-        const HeldPiece q = *hp;
-        vis.step(make_seg<true, MUL24>(g, r, c, q.a, q.b, held, q.maj, true), none, none);
-    }
+    if (any_lane(held)) flush();
 }
 
 // Walks the compat traversal of one ray (lanes of a wave in lockstep over plane steps).
@@ -575,8 +671,7 @@ __host__ __device__ __forceinline__ void traverse_lane(const ProjGeom& g, const 
                                                        int kmax, V& vis, WaveLds* lds = nullptr) {
     if (kmin > kmax) return;  // no lane of this wave can reach the volume's z band
     if (g.aligned && lds) {  // the caller vouches that the lanes share their xy path
-        if (g.mul24) traverse_table<true>(g, r, active, kmin, kmax, vis, lds);
-        else traverse_table<false>(g, r, active, kmin, kmax, vis, lds);
+        traverse_table(g, r, active, kmin, kmax, vis, lds);
     } else if (g.aligned) {
         if (g.mul24) traverse_steps<true, true>(g, r, active, kmin, kmax, vis);
         else traverse_steps<true, false>(g, r, active, kmin, kmax, vis);
