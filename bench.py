@@ -242,6 +242,7 @@ def bench_adjoint(line, inp, dev, stream, e):
                                    "densities x (1 + 0.05 N(0,1))",
                        "roofline_frac": adj_bytes / (np.median(ta) * 1e-3) / 1e9 / HBM_PEAK_GBS,
                        "logproj_ms": float(np.median(tl)),
+                       "adjoint_channels": P4.adjoint_channels,
                        "adjoint_over_logproj": float(np.median(ta) / np.median(tl)),
                        "segments": segs4}
     log(f"adjoint: {np.median(ta):.3f} ms/gradient")

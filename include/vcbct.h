@@ -189,6 +189,13 @@ int vcbct_projector_logproj_backward(vcbct_projector* p, const float* dev_gp, fl
  * kept instead of a second traversal; the per-energy terms are never stored).  The backward does not read the
  * volume. */
 int vcbct_projector_logproj_save(vcbct_projector* p, float* dev_p, float* dev_L, double xfactor, void* stream);
+/* Number of channels the adjoint scatters (no reference counterpart): num_material, or r < num_material when
+ * the spectrum's attenuation table has a rank-r form mu = B (B^T mu) to 1e-6 of its largest entry (B orthonormal,
+ * from the table's eigenvectors).  Attenuation between ~10 and ~150 keV is close to a photoelectric and a
+ * Compton basis function, and tables built from just those two terms have r = 2: the gradient is then scattered
+ * in r channels (per-ray coefficients with the table B^T mu) and mixed back to the maps per voxel, which cuts
+ * the scatter's atomics by num_material / r.  Requires a spectrum (VCBCT_ESTATE otherwise). */
+int vcbct_projector_adjoint_channels(const vcbct_projector* p);
 int vcbct_projector_logproj_backward_saved(vcbct_projector* p, const float* dev_L, const float* dev_gp,
                                            float* dev_grad, double xfactor, void* stream);
 

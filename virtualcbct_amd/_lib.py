@@ -86,6 +86,7 @@ SIGNATURES = {
     "vcbct_projector_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_projector_logproj_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_projector_logproj_save": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
+    "vcbct_projector_adjoint_channels": (c_int, [c_void_p]),
     "vcbct_projector_logproj_backward_saved": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_prepare_const_map": (c_int, [c_int, c_double, P_D]),
     "vcbct_filter_sinogram": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, P_D, P_D, c_double, c_void_p]),
@@ -93,7 +94,7 @@ SIGNATURES = {
                                         c_void_p, c_int, P_D, P_D, c_double, c_void_p]),
     "vcbct_reconstruct_image_noalloc": (c_int, [c_int, c_int, c_int, c_double, c_double, c_void_p, c_int, c_int,
                                                 c_int, c_void_p, c_int, c_void_p, c_void_p, c_double, c_void_p]),
-    "vcbct_filter_sinogram_host":(c_int, [c_int, c_int, c_int, P_F, P_F, P_D, P_D, c_double]),
+    "vcbct_filter_sinogram_host": (c_int, [c_int, c_int, c_int, P_F, P_F, P_D, P_D, c_double]),
     "vcbct_reconstruct_image_host": (c_int, [c_int, c_int, c_int, c_double, c_double, P_D, c_int, c_int, c_int, P_F,
                                              c_int, P_D, P_D, c_double]),
     "vcbct_fdk_plan_create": (c_int, [C.POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int, c_int, c_double,

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <atomic>
 #include <vector>
 
@@ -37,6 +38,77 @@ hipError_t dev_malloc(void** p, size_t bytes) {
         return hipErrorOutOfMemory;
     }
     return hipMalloc(p, bytes);
+}
+
+int spectral_basis(const double* mu, int nmat, int NE, double tol, double B[VCBCT_MAX_MAT][VCBCT_MAX_MAT]) {
+    const int n = nmat;
+    // Gram matrix A = mu mu^T and its eigenvectors by cyclic Jacobi rotations (n <= 6)
+    double A[VCBCT_MAX_MAT][VCBCT_MAX_MAT] = {}, V[VCBCT_MAX_MAT][VCBCT_MAX_MAT] = {};
+    double amax = 0.0;
+    for (int i = 0; i < n; ++i) {
+        V[i][i] = 1.0;
+        for (int j = 0; j < n; ++j) {
+            double a = 0.0;
+            for (int e = 0; e < NE; ++e) a += mu[(size_t)i * NE + e] * mu[(size_t)j * NE + e];
+            A[i][j] = a;
+        }
+        for (int e = 0; e < NE; ++e) amax = std::max(amax, std::fabs(mu[(size_t)i * NE + e]));
+    }
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < n; ++i) {
+            diag += A[i][i] * A[i][i];
+            for (int j = i + 1; j < n; ++j) off += A[i][j] * A[i][j];
+        }
+        if (off <= 1e-32 * diag) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                if (A[p][q] == 0.0) continue;
+                const double th = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < n; ++k) {  // A <- A J, then J^T A
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - sn * akq;
+                    A[k][q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - sn * aqk;
+                    A[q][k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[k][p], vkq = V[k][q];
+                    V[k][p] = c * vkp - sn * vkq;
+                    V[k][q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    int order[VCBCT_MAX_MAT];
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::sort(order, order + n, [&](int a, int b) { return A[a][a] > A[b][b]; });
+    for (int r = 1; r <= n; ++r) {
+        double err = 0.0;
+        for (int e = 0; e < NE; ++e) {
+            double cfs[VCBCT_MAX_MAT];
+            for (int k = 0; k < r; ++k) {
+                double s = 0.0;
+                for (int m = 0; m < n; ++m) s += V[m][order[k]] * mu[(size_t)m * NE + e];
+                cfs[k] = s;
+            }
+            for (int m = 0; m < n; ++m) {
+                double s = 0.0;
+                for (int k = 0; k < r; ++k) s += V[m][order[k]] * cfs[k];
+                err = std::max(err, std::fabs(s - mu[(size_t)m * NE + e]));
+            }
+        }
+        if (err <= tol * amax || r == n) {
+            for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+                for (int k = 0; k < VCBCT_MAX_MAT; ++k) B[m][k] = (m < n && k < r) ? V[m][order[k]] : 0.0;
+            return r;
+        }
+    }
+    return n;
 }
 
 static int check_geometry(const vcbct_geometry* g) {
@@ -127,6 +199,7 @@ extern "C" void vcbct_projector_destroy(vcbct_projector* p) {
     (void)hipFree(p->d_pdf);
     (void)hipFree(p->d_muf);
     (void)hipFree(p->d_pdff);
+    (void)hipFree(p->d_ch);
     free(p);
 }
 
@@ -202,8 +275,38 @@ extern "C" int vcbct_projector_set_spectrum(vcbct_projector* p, const float* att
     VCBCT_HIP(hipMemcpy(p->d_pdf, pd.data(), pd.size() * sizeof(double), hipMemcpyHostToDevice));
     VCBCT_HIP(hipMemcpy(p->d_muf, atten, mu.size() * sizeof(float), hipMemcpyHostToDevice));
     VCBCT_HIP(hipMemcpy(p->d_pdff, pdf, pd.size() * sizeof(float), hipMemcpyHostToDevice));
+    // The adjoint's channels.  When the table has a lower-rank form (mu = B (B^T mu) to 1e-6 of its largest
+    // entry: attenuation in this energy range is close to a photoelectric and a Compton basis function, and
+    // exactly so for tables built from those two terms) the gradient is scattered in r < nmat channels with the
+    // coefficient table B^T mu and mixed back to the maps per voxel; otherwise the channels are the maps.
+    double B[VCBCT_MAX_MAT][VCBCT_MAX_MAT];
+    const int r = spectral_basis(mu.data(), nm, NE, 1e-6, B);
+    p->mix.nch = r < nm ? r : nm;
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+        for (int k = 0; k < VCBCT_MAX_MAT; ++k)
+            p->mix.b[m][k] = r < nm ? (float)B[m][k] : (m == k && m < nm ? 1.0f : 0.0f);
+    (void)hipFree(p->d_ch);
+    p->d_ch = nullptr;
+    if (r < nm) {
+        std::vector<double> ch((size_t)r * NE);
+        for (int k = 0; k < r; ++k)
+            for (int e = 0; e < NE; ++e) {
+                double s = 0.0;
+                for (int m = 0; m < nm; ++m) s += B[m][k] * mu[(size_t)m * NE + e];
+                ch[(size_t)k * NE + e] = s;
+            }
+        if (dev_malloc((void**)&p->d_ch, ch.size() * sizeof(double)) != hipSuccess)
+            return fail(VCBCT_ENOMEM, "device allocation of the adjoint channel table failed");
+        VCBCT_HIP(hipMemcpy(p->d_ch, ch.data(), ch.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     p->NE = NE;
     return 0;
+}
+
+extern "C" int vcbct_projector_adjoint_channels(const vcbct_projector* p) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    if (p->NE < 1) return fail(VCBCT_ESTATE, "no spectrum set (vcbct_projector_set_spectrum)");
+    return p->mix.nch;
 }
 
 static int need(const vcbct_projector* p, bool u16, bool f32, bool spec) {
@@ -221,6 +324,8 @@ static Spectrum spec_of(const vcbct_projector* p) {
     s.muf = p->d_muf;
     s.pdff = p->d_pdff;
     s.NE = p->NE;
+    s.ch = p->d_ch ? p->d_ch : p->d_mu;
+    s.nch = p->mix.nch;
     return s;
 }
 
@@ -245,7 +350,7 @@ extern "C" int vcbct_projector_line_integrals(vcbct_projector* p, float* dev_L, 
     if (rc) return rc;
     if (!dev_L) return fail(VCBCT_EINVAL, "line_integrals: output is NULL");
     VCBCT_HIP(hipSetDevice(p->device));
-    Spectrum sp{nullptr, nullptr, nullptr, nullptr, 0};
+    Spectrum sp{nullptr, nullptr, nullptr, nullptr, 0, nullptr, 0};
     return launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
                           sp, 1.0, 0, p->g.nAng, nullptr, VCBCT_MODE_LINEINT, dev_L, (hipStream_t)stream);
 }
@@ -311,13 +416,13 @@ static int run_adjoint(vcbct_projector* p, const float* tg, float* dev_grad, dou
         return fail(VCBCT_ENOMEM, "device allocation of the adjoint coefficients failed");
     // scatter target [mat][y][x][z] (FP32 planes); zero with a fill kernel, not hipMemsetAsync, so the call
     // stays safe under stream capture
-    rc = launch_zero(p->d_grad_vox, nv * 4 * p->g.nmat, st);
+    rc = launch_zero(p->d_grad_vox, nv * 4 * p->mix.nch, st);
     if (rc) return rc;
     if (dev_loss && (rc = launch_zero(dev_loss, sizeof(double), st))) return rc;
     rc = launch_adjoint(p->g, p->d_tables, p->d_vox_f32, spec_of(p), xfactor, tg, mode, p->d_coef, p->d_grad_vox,
                         dev_loss, st);
     if (rc) return rc;
-    return launch_untranspose_f32(p->d_grad_vox, dev_grad, p->g, st);
+    return launch_untranspose_mix(p->d_grad_vox, dev_grad, p->g, p->mix, st);
 }
 
 extern "C" int vcbct_projector_adjoint(vcbct_projector* p, const float* dev_target, float* dev_grad, double* dev_loss,
@@ -360,18 +465,18 @@ extern "C" int vcbct_projector_logproj_backward_saved(vcbct_projector* p, const 
     const size_t nrays = (size_t)p->g.nAng * p->g.H * p->g.W;
     if (!p->d_coef && dev_malloc((void**)&p->d_coef, nrays * p->g.nmat * sizeof(float)) != hipSuccess)
         return fail(VCBCT_ENOMEM, "device allocation of the adjoint coefficients failed");
-    rc = launch_zero(p->d_grad_vox, nv * 4 * p->g.nmat, st);
+    rc = launch_zero(p->d_grad_vox, nv * 4 * p->mix.nch, st);
     if (rc) return rc;
     rc = launch_adjoint_saved(p->g, p->d_tables, dev_L, spec_of(p), xfactor, dev_gp, p->d_coef, p->d_grad_vox, st);
     if (rc) return rc;
-    return launch_untranspose_f32(p->d_grad_vox, dev_grad, p->g, st);
+    return launch_untranspose_mix(p->d_grad_vox, dev_grad, p->g, p->mix, st);
 }
 
 extern "C" int vcbct_projector_segment_counts(vcbct_projector* p, float* dev_counts, void* stream) {
     if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
     if (!dev_counts) return fail(VCBCT_EINVAL, "segment_counts: output is NULL");
     VCBCT_HIP(hipSetDevice(p->device));
-    Spectrum sp{nullptr, nullptr, nullptr, nullptr, 0};
+    Spectrum sp{nullptr, nullptr, nullptr, nullptr, 0, nullptr, 0};
     return launch_forward(p->g, p->d_tables, nullptr, 0, sp, 1.0, 0, p->g.nAng, nullptr, VCBCT_MODE_SEGCOUNT,
                           dev_counts, (hipStream_t)stream);
 }

@@ -115,6 +115,40 @@ __global__ __launch_bounds__(256) void k_untranspose_f32(const float* __restrict
     }
 }
 
+// The adjoint's channel planes [nch][y][x][z] -> the maps [nmat][z][y][x], grad_m = sum_k b[m][k] G_k: a block
+// stages a 32 (x) x 32 (z) tile of every channel of one y in LDS (z-runs in), then writes each map's tile
+// (x-runs out).  Without a spectral basis the mix is the identity (nch = nmat).
+__global__ __launch_bounds__(256) void k_untranspose_mix(const float* __restrict__ src, float* __restrict__ dst, int N,
+                                                         int Nz, int nmat, ChannelMix mix) {
+    __shared__ float tile[VCBCT_MAX_MAT][TT][TT + 1];
+    const int x0 = blockIdx.x * TT, z0 = blockIdx.y * TT, y = blockIdx.z;
+    const size_t n2 = (size_t)N * N, n3 = n2 * Nz;
+    const int nch = mix.nch;
+    for (int k = 0; k < nch; ++k) {
+        const float* s = src + (size_t)k * n3 + (size_t)y * N * Nz;
+        for (int i = threadIdx.x; i < TT * TT; i += 256) {
+            const int xx = i / TT, zz = i % TT;
+            const int x = x0 + xx, z = z0 + zz;
+            tile[k][zz][xx] = (x < N && z < Nz) ? s[(size_t)x * Nz + z] : 0.f;
+        }
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
+    for (int zz = ty; zz < TT; zz += 8) {
+        const int z = z0 + zz, x = x0 + tx;
+        if (z >= Nz || x >= N) continue;
+        float g[VCBCT_MAX_MAT];
+#pragma unroll
+        for (int k = 0; k < VCBCT_MAX_MAT; ++k) g[k] = k < nch ? tile[k][zz][tx] : 0.f;
+        for (int m = 0; m < nmat; ++m) {
+            float v = 0.f;
+#pragma unroll
+            for (int k = 0; k < VCBCT_MAX_MAT; ++k) v = fmaf(mix.b[m][k], g[k], v);
+            dst[(size_t)m * n3 + (size_t)z * n2 + (size_t)y * N + x] = v;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Voxel access
 
@@ -239,11 +273,13 @@ __device__ __forceinline__ double spectral_xmin(const float (&L)[VCBCT_MAX_MAT],
     return xmin;
 }
 
-// sum_e pdf_e e^{-(x_e - x_min)}, and with MUB also sum_e pdf_e e^{-(x_e - x_min)} mu_me into mub (FP64).
+// sum_e pdf_e e^{-(x_e - x_min)}, and with MUB also sum_e pdf_e e^{-(x_e - x_min)} ch_ke into mub (FP64; ch the
+// adjoint's channel table, nch channels).
 template <bool MUB>
 __device__ __forceinline__ double spectral_shifted(const float (&L)[VCBCT_MAX_MAT], int nmat,
                                                    const double* __restrict__ mu, const double* __restrict__ pdf,
-                                                   int NE, double inv_xf, double xmin, double (&mub)[VCBCT_MAX_MAT]) {
+                                                   int NE, double inv_xf, double xmin, double (&mub)[VCBCT_MAX_MAT],
+                                                   const double* __restrict__ ch = nullptr, int nch = 0) {
     double se = 0.0;
     for (int e = 0; e < NE; ++e) {
         const double we = ldg(pdf, e) * exp(-(spectral_x(L, nmat, mu, NE, e, inv_xf) - xmin));
@@ -251,7 +287,7 @@ __device__ __forceinline__ double spectral_shifted(const float (&L)[VCBCT_MAX_MA
         if (MUB) {
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                if (m < nmat) mub[m] += we * ldg(mu, m * NE + e);
+                if (m < nch) mub[m] += we * ldg(ch, m * NE + e);
         }
     }
     return se;
@@ -276,15 +312,15 @@ __device__ __forceinline__ double logproj_opaque(const float (&L)[VCBCT_MAX_MAT]
     return xmin - log(spectral_shifted<false>(L, nmat, mu, pdf, NE, inv_xf, xmin, unused) / sw);
 }
 
-// The transmission-weighted spectral mean of mu of an opaque ray (mub[m] = mubar_m, already divided by the
+// The transmission-weighted spectral mean of the channel table of an opaque ray (mub[k], already divided by the
 // sum) and its log projection.
 __device__ __forceinline__ double mubar_opaque(const float (&L)[VCBCT_MAX_MAT], int nmat, const double* __restrict__ mu,
                                                const double* __restrict__ pdf, int NE, double inv_xf, double sw,
-                                               double (&mub)[VCBCT_MAX_MAT]) {
+                                               double (&mub)[VCBCT_MAX_MAT], const double* __restrict__ ch, int nch) {
     const double xmin = spectral_xmin(L, nmat, mu, pdf, NE, inv_xf);
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) mub[m] = 0.0;
-    const double se = spectral_shifted<true>(L, nmat, mu, pdf, NE, inv_xf, xmin, mub);
+    const double se = spectral_shifted<true>(L, nmat, mu, pdf, NE, inv_xf, xmin, mub, ch, nch);
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) mub[m] /= se;
     return xmin - log(se / sw);
@@ -575,7 +611,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, const double* __restrict__ tab,
                                                       const float4* __restrict__ vox, const double* __restrict__ mu,
                                                       const double* __restrict__ pdf, const float* __restrict__ muf,
-                                                      int NE, double inv_xf,
+                                                      const double* __restrict__ ch, int nch, int NE, double inv_xf,
                                                       int quads_per_xcd, const float* __restrict__ tg,
                                                       float* __restrict__ coef, double* __restrict__ loss) {
     __shared__ WaveLds step_tabs[4];
@@ -612,13 +648,13 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, c
             se += we;
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                if (m < g.nmat) mub[m] += we * ldg(mu, m * NE + e);
+                if (m < nch) mub[m] += we * ldg(ch, m * NE + e);
         }
         const size_t nrays = (size_t)g.nAng * g.H * g.W;
         const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - j) * g.W + jj;
         double p;
         if (se < 1e-30 * sw) {  // opaque ray: FP64 terms relative to the smallest exponent
-            p = mubar_opaque(acc.L, g.nmat, mu, pdf, NE, inv_xf, sw, mub);
+            p = mubar_opaque(acc.L, g.nmat, mu, pdf, NE, inv_xf, sw, mub, ch, nch);
         } else {
             p = -log(se / sw);
 #pragma unroll
@@ -634,7 +670,7 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, c
         }
 #pragma unroll
         for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-            if (m < g.nmat) stg(coef, (size_t)m * nrays + o, (float)(w * mub[m] * inv_xf));
+            if (m < nch) stg(coef, (size_t)m * nrays + o, (float)(w * mub[m] * inv_xf));
     }
     if (MODE == VCBCT_ADJ_LOSS && loss) {
         // wave reduction of the loss, one atomic per wave
@@ -650,8 +686,10 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, c
 __global__ __launch_bounds__(256) void k_adjoint_coef_saved(ProjGeom g, const float* __restrict__ Ls,
                                                             const double* __restrict__ mu,
                                                             const double* __restrict__ pdf,
-                                                            const float* __restrict__ muf, int NE, double inv_xf,
-                                                            const float* __restrict__ gp, float* __restrict__ coef) {
+                                                            const float* __restrict__ muf,
+                                                            const double* __restrict__ ch, int nch, int NE,
+                                                            double inv_xf, const float* __restrict__ gp,
+                                                            float* __restrict__ coef) {
     const size_t nrays = (size_t)g.nAng * g.H * g.W;
     for (size_t o = (size_t)blockIdx.x * 256 + threadIdx.x; o < nrays; o += (size_t)gridDim.x * 256) {
         float L[VCBCT_MAX_MAT];
@@ -670,10 +708,10 @@ __global__ __launch_bounds__(256) void k_adjoint_coef_saved(ProjGeom g, const fl
             se += we;
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                if (m < g.nmat) mub[m] += we * ldg(mu, m * NE + e);
+                if (m < nch) mub[m] += we * ldg(ch, m * NE + e);
         }
         if (se < 1e-30 * sw) {  // opaque ray: FP64 terms relative to the smallest exponent
-            (void)mubar_opaque(L, g.nmat, mu, pdf, NE, inv_xf, sw, mub);
+            (void)mubar_opaque(L, g.nmat, mu, pdf, NE, inv_xf, sw, mub, ch, nch);
         } else {
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m) mub[m] /= se;
@@ -681,12 +719,13 @@ __global__ __launch_bounds__(256) void k_adjoint_coef_saved(ProjGeom g, const fl
         const double w = (double)ldg(gp, o);
 #pragma unroll
         for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-            if (m < g.nmat) stg(coef, (size_t)m * nrays + o, (float)(w * mub[m] * inv_xf));
+            if (m < nch) stg(coef, (size_t)m * nrays + o, (float)(w * mub[m] * inv_xf));
     }
 }
 
 __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_scatter(ProjGeom g, const double* __restrict__ tab, int quads_per_xcd,
-                                                         const float* __restrict__ coef, float* __restrict__ gvox) {
+                                                         const float* __restrict__ coef, int nch,
+                                                         float* __restrict__ gvox) {
     __shared__ WaveLds step_tabs[4];
     WaveLds* steps = &step_tabs[threadIdx.x >> 6];
     const Tile t = tile_of(g.nAng, g.W, quads_per_xcd);
@@ -698,14 +737,14 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_scatter(ProjGeom g
     const bool active = j < g.H;
     ScatterG sc;
     sc.gvox = gvox;
-    sc.nmat = g.nmat;
+    sc.nmat = nch;  // the scattered channels (the maps, or their spectral basis)
     sc.plane = (size_t)g.N * g.N * g.Nz;
     bool any = false;
     const size_t nrays = (size_t)g.nAng * g.H * g.W;
     const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - (active ? j : 0)) * g.W + jj;
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) {
-        sc.c[m] = (active && m < g.nmat) ? ldg(coef, (size_t)m * nrays + o) : 0.f;
+        sc.c[m] = (active && m < nch) ? ldg(coef, (size_t)m * nrays + o) : 0.f;
         any = any || sc.c[m] != 0.f;
     }
     if (!__any(any)) return;  // nothing to scatter for this wave (e.g. zero upstream gradient)
@@ -751,6 +790,12 @@ int launch_transpose_f32(const float* src, float* dst, const ProjGeom& g, hipStr
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
+int launch_untranspose_mix(const float* src, float* dst, const ProjGeom& g, const ChannelMix& mix, hipStream_t st) {
+    dim3 grid((g.N + TT - 1) / TT, (g.Nz + TT - 1) / TT, g.N);
+    hipLaunchKernelGGL(k_untranspose_mix, grid, dim3(256), 0, st, src, dst, g.N, g.Nz, g.nmat, mix);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
 int launch_untranspose_f32(const float* src, float* dst, const ProjGeom& g, hipStream_t st) {
     dim3 grid((g.N + TT - 1) / TT, (g.Nz + TT - 1) / TT, g.N * g.nmat);
     hipLaunchKernelGGL(k_untranspose_f32, grid, dim3(256), 0, st, src, dst, g.N, g.Nz, g.nmat);
@@ -792,12 +837,13 @@ int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, co
     const double inv_xf = 1.0 / xfactor;
     if (mode == VCBCT_ADJ_LOSS)
         hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_LOSS>, dim3(nb), dim3(256), 0, st, g, tables, (const float4*)vox,
-                           sp.mu, sp.pdf, sp.muf, sp.NE, inv_xf, qpx, target_or_gp, coef, loss);
+                           sp.mu, sp.pdf, sp.muf, sp.ch, sp.nch, sp.NE, inv_xf, qpx, target_or_gp, coef, loss);
     else
         hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_UPSTREAM>, dim3(nb), dim3(256), 0, st, g, tables,
-                           (const float4*)vox, sp.mu, sp.pdf, sp.muf, sp.NE, inv_xf, qpx, target_or_gp, coef, loss);
+                           (const float4*)vox, sp.mu, sp.pdf, sp.muf, sp.ch, sp.nch, sp.NE, inv_xf, qpx, target_or_gp,
+                           coef, loss);
     VCBCT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, grad_vox);
+    hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, sp.nch, grad_vox);
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
@@ -809,10 +855,10 @@ int launch_adjoint_saved(const ProjGeom& g, const double* tables, const float* L
     if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
     const size_t nrays = (size_t)g.nAng * g.H * g.W;
     const unsigned nbc = (unsigned)std::min<size_t>((nrays + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_adjoint_coef_saved, dim3(nbc), dim3(256), 0, st, g, Ls, sp.mu, sp.pdf, sp.muf, sp.NE,
-                       1.0 / xfactor, gp, coef);
+    hipLaunchKernelGGL(k_adjoint_coef_saved, dim3(nbc), dim3(256), 0, st, g, Ls, sp.mu, sp.pdf, sp.muf, sp.ch, sp.nch,
+                       sp.NE, 1.0 / xfactor, gp, coef);
     VCBCT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, grad_vox);
+    hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, sp.nch, grad_vox);
     VCBCT_HIP(hipGetLastError());
     return 0;
 }

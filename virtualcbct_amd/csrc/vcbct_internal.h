@@ -81,7 +81,23 @@ struct Spectrum {
     const float* muf;   // the same tables as given (FP32): the intensity projection's spectral sum
     const float* pdff;
     int NE;
+    // The adjoint's channels: the gradient with respect to the nmat maps is formed as sum_k B_mk G_k from nch
+    // scattered channels G_k, whose per-ray coefficients use the channel table ch [nch][NE] in place of mu.
+    // ch = mu, nch = nmat, B = identity when the table has no lower-rank form (see spectral_basis).
+    const double* ch;   // [nch][NE]
+    int nch;
 };
+
+// The mix from the adjoint's channels to the maps: grad_m = sum_k b[m][k] G_k (identity without a basis).
+struct ChannelMix {
+    float b[VCBCT_MAX_MAT][VCBCT_MAX_MAT];
+    int nch;
+};
+
+// Spectral basis of an attenuation table mu [nmat][NE] (FP64): the smallest r and orthonormal columns
+// B [nmat][r] with max |mu - B (B^T mu)| <= tol * max |mu|, from the eigenvectors of mu mu^T (Jacobi).
+// Writes B (row-major [nmat][VCBCT_MAX_MAT], first r columns used) and returns r (nmat when no smaller r does).
+int spectral_basis(const double* mu, int nmat, int NE, double tol, double B[VCBCT_MAX_MAT][VCBCT_MAX_MAT]);
 
 struct vcbct_projector_impl;
 
@@ -110,6 +126,8 @@ int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks
 int launch_transpose_u16(const uint16_t* src, uint16_t* dst, const ProjGeom& g, hipStream_t st);
 int launch_transpose_f32(const float* src, float* dst, const ProjGeom& g, hipStream_t st);
 int launch_untranspose_f32(const float* src, float* dst, const ProjGeom& g, hipStream_t st);
+// The adjoint's channel planes [nch][y][x][z] -> the caller's maps [nmat][z][y][x] through the mix.
+int launch_untranspose_mix(const float* src, float* dst, const ProjGeom& g, const ChannelMix& mix, hipStream_t st);
 int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int vox_is_f32, const Spectrum& sp,
                    double xfactor, int a0, int a1, float* out, int mode, float* aux, hipStream_t st);
 int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double xfactor,
@@ -173,6 +191,8 @@ struct vcbct_projector {
     double* d_pdf;         // [NE]
     float* d_muf;          // [nmat][NE] as given
     float* d_pdff;         // [NE] as given
+    double* d_ch;          // the adjoint's channel table [nch][NE] (the basis form of mu; unused when nch == nmat)
+    vcbct::ChannelMix mix; // channels -> maps
     int NE;
     int have_u16, have_f32;
 };

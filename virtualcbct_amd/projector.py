@@ -139,6 +139,14 @@ class Projector:
         # saved autograd state refers to the spectrum it was computed with; a backward pass checks this
         self.spectrum_generation += 1
 
+    @property
+    def adjoint_channels(self):
+        """Channels the adjoint scatters: num_material, or the rank of the attenuation table's spectral basis."""
+        rc = lib().vcbct_projector_adjoint_channels(self.h)
+        if rc < 0:
+            check(rc)
+        return rc
+
     # ------------------------------------------------------------------ compute
     def forward(self, xfactor, out=None, a0=0, a1=None, stream=None):
         """Polychromatic intensity projections [n_angles][H][W] (torch float32 on the device)."""
