@@ -15,6 +15,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import tempfile
 import time
@@ -46,24 +47,57 @@ def build_c2(td):
                 ys_fdk=ys_fdk)
 
 
-def read_traffic():
-    """HBM bytes per launch of the forward kernel from the committed rocprofv3 PMC summary, if any."""
-    p = os.path.join(ROOT, "profiles", "forward_c2_traffic.txt")
+def read_profile_value(name, key):
+    """A value recorded as `key: value` in a committed rocprofv3 summary under profiles/ (None if absent)."""
+    p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     for line in open(p):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
 
 
+def read_traffic():
+    """HBM bytes per launch of the forward kernel from the committed rocprofv3 PMC summary, if any (FETCH_SIZE and
+    WRITE_SIZE of the kernel, each scaled by the same counter's reading for a plain copy of known size)."""
+    return read_profile_value("forward_c2_traffic.txt", "traffic_bytes_per_launch")
+
+
+def host_cpu():
+    """The host's CPU as the CPU baselines report it: the logical CPU count (nproc), the CPUs this process may run
+    on, OMP_NUM_THREADS, the lscpu model name, and the thread count used (the CPUs this process may run on,
+    capped by OMP_NUM_THREADS where that is set, as on a GPU host shared between GPUs)."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    if model is None and os.path.exists("/proc/cpuinfo"):
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(affinity, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else affinity
+    return {"nproc": os.cpu_count(), "affinity_cpus": affinity, "OMP_NUM_THREADS": omp, "model": model,
+            "threads": threads}
+
+
 def cpu_baseline(inp, seconds_target=15.0):
     """The reference's CPU projector (P/reprojection.cpp, built from its sources into oracle/_ref) on a bounded
-    sample of (angle, bin) pairs of the same workload, OpenMP over pairs (the function itself is single-threaded
-    per call).  Falls back to the FP64 oracle when the reference build is absent."""
+    sample of (angle, bin) pairs of the same workload, OpenMP over pairs on every CPU this process may use (the
+    function itself is single-threaded per call).  Falls back to the FP64 oracle when the reference build is
+    absent."""
     from oracle import oracle as O
     w = inp["w"]
-    nthreads = min(16, os.cpu_count() or 1)
+    cpu = host_cpu()
+    nthreads = cpu["threads"]
     g = O.Geometry(N=w["N"], s=w["s"], Nz=w["Nz"], t=w["t"], H=w["H"], W=w["W"], cos_t=inp["cs"], sin_t=inp["sn"],
                    xc=inp["xc"], yc=inp["yc"])
     rng = np.random.default_rng(0)
@@ -78,7 +112,7 @@ def cpu_baseline(inp, seconds_target=15.0):
         rate = n * w["W"] * w["H"] / sec
         return {"value": rate, "unit": "rays*bins/s", "cores": nthreads, "kind": "reference",
                 "sample": f"{n} (angle, bin) pairs of the 360x91 workload, each a full 400x400 detector "
-                          f"(P/reprojection.cpp CPU twin, one call per pair), {sec:.1f} s"}
+                          f"(P/reprojection.cpp CPU twin, one call per pair), {sec:.1f} s", "host": cpu}
     # oracle: one traversal per ray then 91 bins; time a slab of rays of one angle
     sub = O.Geometry(N=w["N"], s=w["s"], Nz=w["Nz"], t=w["t"], H=w["H"], W=w["W"], cos_t=inp["cs"][:1],
                      sin_t=inp["sn"][:1], xc=inp["xc"][:1], yc=inp["yc"][:1])
@@ -86,7 +120,7 @@ def cpu_baseline(inp, seconds_target=15.0):
     O.forward_collapsed(sub, inp["vol"], inp["atten"], inp["pdf"], 1000.0)
     sec = time.perf_counter() - t0
     return {"value": w["W"] * w["H"] * 91 / sec, "unit": "rays*bins/s", "cores": nthreads, "kind": "port",
-            "sample": f"1 angle x 160000 rays x 91 bins with the FP64 oracle, {sec:.1f} s"}
+            "sample": f"1 angle x 160000 rays x 91 bins with the FP64 oracle, {sec:.1f} s", "host": cpu}
 
 
 def bench_fdk(line, inp, dev, stream, out):
@@ -143,6 +177,24 @@ def bench_fdk(line, inp, dev, stream, out):
         thb.append(e[1].elapsed_time(e[2]))
     plan_hf.close()
     del filt_hf
+    # the same scan through a plan with the opt-in band-limited ramp (window alpha = 1, Ram-Lak), reported
+    # separately: the reference's kernel (alpha = 0.5) is the parity path
+    plan_rl = fdk.FdkPlan(w["H"], w["W"], w["n_ang"], w["R"], w["Z"], w["zstart"], w["zstep"], w["rscale"], 1000,
+                          inp["a_fdk"], inp["xs_fdk"], inp["ys_fdk"], window_alpha=1.0)
+    plan_rl.filter(proj, filt)
+    torch.cuda.synchronize()
+    trl, trb = [], []
+    for _ in range(nrep):
+        rec.zero_()
+        e[0].record(stream)
+        plan_rl.filter(proj, filt)
+        e[1].record(stream)
+        plan_rl.backproject(filt, rec)
+        e[2].record(stream)
+        torch.cuda.synchronize()
+        trl.append(e[0].elapsed_time(e[1]))
+        trb.append(e[1].elapsed_time(e[2]))
+    plan_rl.close()
     # the reference's CPU backprojection (F/reconstructImage.cpp, built from its sources; OpenMP over slices)
     # on a bounded sample of the same reconstruction: whole slices from the centre up, all 360 views, a
     # multiple of the thread count (one slice per thread at a time)
@@ -150,7 +202,7 @@ def bench_fdk(line, inp, dev, stream, out):
     from oracle import oracle as O
     if O.ref_available():
         fh = filt.cpu().numpy()
-        nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        nthreads = host_cpu()["threads"]
         _, sec1 = O.ref_reconstruct_image(w["R"], 1, 0, w["zstep"], w["rscale"], fh, 1000, inp["a_fdk"],
                                           inp["xs_fdk"])
         nz = int(min(w["Z"], nthreads * max(1, round(6.0 / max(sec1, 1e-3)))))
@@ -167,16 +219,38 @@ def bench_fdk(line, inp, dev, stream, out):
                    "filter_ms": float(np.median(tf)), "offset_weight_ms": float(np.median(tw)),
                    "half_fan_plan": {"filter_ms": float(np.median(thf)), "bp_ms": float(np.median(thb)),
                                      "width": plan_hf.width},
+                   "ram_lak_plan": {"window_alpha": 1.0, "filter_ms": float(np.median(trl)),
+                                    "bp_ms": float(np.median(trb))},
                    "cpu_reference_backprojection": cpu_ref,
                    "filter_rows_per_s": w["n_ang"] * w["H"] / (np.median(tf) * 1e-3),
                    "updates": updates, "shape": [w["Z"], w["R"], w["R"]],
-                   "bp_roofline_frac": (16.0 * updates / (np.median(tb) * 1e-3) / 1e9) / HBM_PEAK_GBS,
+                   # the backprojector is bound by VALU issue (its taps hit in L2), so its bound is stated in
+                   # instructions: VALU wave-instructions per launch from the committed counter profile, as
+                   # lane-instructions per voxel update, and the VALU busy fraction they imply at this run's time
+                   "bp_bound": fdk_bound(updates, float(np.median(tb))),
                    "autotune": [l for l in autotune_log().strip().split("\n") if l.startswith("bp ")]}
     log(f"fdk: filter {np.median(tf):.3f} ms, backprojection {np.median(tb):.3f} ms, "
         f"{line['fdk']['gups']:.1f} GUPS, offset weighting {np.median(tw):.3f} ms; half-fan plan: filter "
         f"{np.median(thf):.3f} ms, backprojection {np.median(thb):.3f} ms")
     plan.close()
     del filt, rec
+
+
+def fdk_bound(updates, bp_ms):
+    """The backprojector's bound from its counters (profiles/fdk_c2_counters.md, the plain plan's launch):
+    VALU-issue time = wave-instructions x 4 cycles / (SIMDs x clock)."""
+    valu = read_profile_value("fdk_c2_counters.md", "bp_valu_wave_instructions_per_launch")
+    simds, clock_hz = 256 * 4, 2.4e9  # MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock
+    out = {"bound": "valu", "source": "profiles/fdk_c2_counters.md (rocprofv3 SQ_INSTS_VALU)",
+           "hbm_equivalent_frac": 16.0 * updates / (bp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "hbm_equivalent_note": "16 B per update (4 FP32 taps) over the 8 TB/s peak; above 1 because the taps "
+                                  "hit in L2, so HBM is not the bound"}
+    if valu:
+        valu_ms = valu * 4 / simds / clock_hz * 1e3
+        out.update({"valu_wave_instructions_per_launch": valu,
+                    "lane_instructions_per_update": valu * 64 / updates,
+                    "valu_issue_ms": valu_ms, "valu_busy_frac": valu_ms / bp_ms})
+    return out
 
 
 def bench_detector(line, dev, stream, e, out):
@@ -407,7 +481,7 @@ def bench_phantom(line, td, dev, stream, e):
     t0 = time.perf_counter()
     O.labels_to_maps(lab_h, w, org)
     sec = time.perf_counter() - t0
-    res["cpu_port"] = {"voxels_per_s": nvox / sec, "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+    res["cpu_port"] = {"voxels_per_s": nvox / sec, "cores": host_cpu()["threads"],
                        "sample": f"the whole 543x271x100 volume, {len(org[0])} organ rows, {sec:.2f} s"}
     line["phantom"] = res
     log(f"phantom: maps {t['maps']:.3f} ms ({res['maps_gb_s']:.0f} GB/s), projector volume "
@@ -459,7 +533,8 @@ def bench_iterative(line, dev, stream, e):
         t0 = time.perf_counter()
         O.ref_lengthproj(g, proj_h[sel], ang[sel])
         sec = time.perf_counter() - t0
-        res["cpu_reference"] = {"voxel_angles_per_s": N * N * nz_c * na_c / sec, "cores": min(nz_c, os.cpu_count()),
+        res["cpu_reference"] = {"voxel_angles_per_s": N * N * nz_c * na_c / sec,
+                                "cores": min(nz_c, host_cpu()["threads"]),
                                 "sample": f"P/lengthproj.cpp on {N}x{N}x{nz_c} x {na_c} angles, {sec:.2f} s"}
     line["iterative"] = res
     log(f"iterative: lengthproj {lp_ms:.2f} ms ({res['lengthproj_voxel_angles_per_s']:.3e} voxel-angles/s), "
@@ -566,11 +641,16 @@ def main():
     achieved = alg_bytes / k_avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
+                "traffic_source": "profiles/forward_c2_traffic.txt: rocprofv3 FETCH_SIZE and WRITE_SIZE of this "
+                                  "kernel, scaled by their readings for a plain 1 GiB copy (not measured in this run)",
                 "kernel": "k_forward<0, uint4>", "segments_per_launch": segs,
                 "algorithmic_bytes_per_launch": alg_bytes}
     line = {"metric": METRIC, "value": value, "unit": "rays*bins/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "precision": "FP32 line integrals (u16 x FP32 segment length), FP32 spectral terms and sum; FP64 ray "
+                         "set-up, plane positions, crossings and segment midpoints (they choose voxels); FP64 "
+                         "terms for rays whose FP32 exponentials all underflow",
             "config": {"workload": "25 cm water cylinder, 512x512x128 voxels (0.5 mm x 0.5 mm x 1 mm), 6 uint16 "
                                    "elemental maps, 360 angles, 400x400 detector, 115 mm half-fan offset, "
                                    "91 energy bins 10-100 keV, FP32 projections"},

@@ -277,6 +277,12 @@ int vcbct_fdk_plan_create_half_fan(vcbct_fdk_plan** out, int proj_height, int pr
                                    double taper_px, int device);
 /* Columns of the panel the plan filters and backprojects (proj_width, or the padded width of a half-fan plan). */
 int vcbct_fdk_plan_width(const vcbct_fdk_plan* p);
+/* The filter's window parameter (no reference counterpart: the reference fixes ALPHA = 0.5 in
+ * F/physParams.h:39-41, used by prepareConstMap, F/filterSinogram.cpp:10-65, and its centre coefficient,
+ * :78-80).  alpha in [0, 1]; 0.5 (the default) is the reference's kernel, 1 the band-limited ramp without a
+ * window (Ram-Lak, cutoff 1.6 x Nyquist as the reference's).  Rebuilds the plan's frequency response (set-up
+ * call: allocates and synchronises once per new (width, pitch, alpha)). */
+int vcbct_fdk_plan_set_window(vcbct_fdk_plan* p, double alpha);
 
 /* Panel-offset class of an FDK angle list (F/projectionDataMVkV.cpp:277-281): "S" when the mean panel x
  * shift of the used projections is below 6, "L" above 15, else "M".  out: at least 2 chars. */

@@ -569,9 +569,12 @@ int orc_trace_ray(const orc_geom* gp, const double* cos_t, const double* sin_t, 
     return (int)segs.size();
 }
 
-void orc_const_map(int W, double pitch_iso, double* cmap) {
-    /* F/filterSinogram.cpp:65-120 with MAX_PIC_NUM 800, CUTOFF_FREQ 1.6, GAMMA 1, ALPHA 0.5 */
-    const double CUT = 1.60, GAM = 1.00, ALP = 0.50;
+void orc_const_map(int W, double pitch_iso, double* cmap) { orc_const_map_w(W, pitch_iso, 0.5, cmap); }
+
+void orc_const_map_w(int W, double pitch_iso, double ALP, double* cmap) {
+    /* F/filterSinogram.cpp:10-65 with MAX_PIC_NUM 800, CUTOFF_FREQ 1.6, GAMMA 1, and the window parameter ALPHA
+       (0.5 in F/physParams.h:39-41; 1 is the build's opt-in band-limited ramp) */
+    const double CUT = 1.60, GAM = 1.00;
     double theConst = 1.0 / (8.0 * kPI * kPI * pitch_iso * 800 / W);
     double gamma_x_cutoffFreq = GAM * CUT;
     double PI_x_cutoffFreq = kPI * CUT;
@@ -602,15 +605,16 @@ void orc_const_map(int W, double pitch_iso, double* cmap) {
         }
 }
 
-static double filter_coeff(int W, double pitch_iso) {
-    const double CUT = 1.60, GAM = 1.00, ALP = 0.50;
+static double filter_coeff(int W, double pitch_iso, double ALP = 0.50) {
+    /* F/filterSinogram.cpp:78-80 */
+    const double CUT = 1.60, GAM = 1.00;
     double omega = kPI / GAM;
     return CUT * CUT / 4.0 / (pitch_iso * 800 / W) *
            (ALP * 0.5 + (1.0 - ALP) * (1.0 / omega * sin(omega) + 1.0 / omega / omega * (cos(omega) - 1.0)));
 }
 
 static inline double filter_cm(int W, int n, int m, double xs, double ys, double pitch_iso) {
-    /* F/filterSinogram.cpp:141-154: both centres use the projection WIDTH */
+    /* F/filterSinogram.cpp:86-99: both centres use the projection WIDTH */
     double XcenterShift = -W / 2.0 - xs / (0.05 * 800 / W) - 1;
     double YcenterShift = -W / 2.0 - ys / (0.05 * 800 / W) - 1;
     double pos2 = fabs((YcenterShift + n)) * (pitch_iso * 800 / W);
@@ -620,7 +624,7 @@ static inline double filter_cm(int W, int n, int m, double xs, double ys, double
 
 void orc_filter_sinogram(int H, int W, int n_ang, const float* proj, float* out, const double* cmap,
                          const double* xshift, const double* yshift, double pitch_iso) {
-    /* F/filterSinogram.cpp:122-173: FP64 terms accumulated into an FP32 lvalue, k ascending */
+    /* F/filterSinogram.cpp:84-118: FP64 terms accumulated into an FP32 lvalue, k ascending */
     double coeff = filter_coeff(W, pitch_iso);
 #pragma omp parallel for
     for (int th = 0; th < n_ang; th++)
@@ -637,7 +641,12 @@ void orc_filter_sinogram(int H, int W, int n_ang, const float* proj, float* out,
 
 void orc_filter_sinogram64(int H, int W, int n_ang, const float* proj, double* out, const double* cmap,
                            const double* xshift, const double* yshift, double pitch_iso) {
-    double coeff = filter_coeff(W, pitch_iso);
+    orc_filter_sinogram64_w(H, W, n_ang, proj, out, cmap, xshift, yshift, pitch_iso, 0.5);
+}
+
+void orc_filter_sinogram64_w(int H, int W, int n_ang, const float* proj, double* out, const double* cmap,
+                             const double* xshift, const double* yshift, double pitch_iso, double alpha) {
+    double coeff = filter_coeff(W, pitch_iso, alpha);
 #pragma omp parallel for
     for (int th = 0; th < n_ang; th++)
         for (int n = 0; n < H; n++) {

@@ -92,8 +92,12 @@ int orc_adjoint(const orc_geom* g, const double* cos_t, const double* sin_t, con
 int orc_trace_ray(const orc_geom* g, const double* cos_t, const double* sin_t, const double* xc, const double* yc,
                   int a, int j, int jj, double* out, int cap);
 void orc_const_map(int W, double pitch_iso, double* cmap /* [W][W] */);
+/* the same with the window parameter alpha (0.5: the reference's; 1: the band-limited ramp) */
+void orc_const_map_w(int W, double pitch_iso, double alpha, double* cmap);
 void orc_filter_sinogram(int H, int W, int n_ang, const float* proj, float* out, const double* cmap,
                          const double* xshift, const double* yshift, double pitch_iso);
+void orc_filter_sinogram64_w(int H, int W, int n_ang, const float* proj, double* out, const double* cmap,
+                             const double* xshift, const double* yshift, double pitch_iso, double alpha);
 void orc_filter_sinogram64(int H, int W, int n_ang, const float* proj, double* out, const double* cmap,
                            const double* xshift, const double* yshift, double pitch_iso);
 void orc_mask(int R, char* mask);

@@ -34,7 +34,7 @@ class orc_geom(C.Structure):
 
 @dataclass
 class Geometry:
-    """The initializeGPU argument block (P/reprojectionOnGPU.h:76-82) plus the trig tables."""
+    """The initializeGPU argument block (P/reprojectionOnGPU.h:2-8) plus the trig tables."""
     N: int
     s: float
     Nz: int
@@ -172,19 +172,27 @@ def adjoint(g: Geometry, vol_f32, atten, pdf, xfactor, target):
     return grad, loss.value
 
 
-def const_map(W, pitch_iso=0.032509752):
+def const_map(W, pitch_iso=0.032509752, alpha=0.5):
+    """prepareConstMap's matrix; alpha: the window parameter (0.5 the reference's, 1 the band-limited ramp)."""
     out = np.zeros((W, W))
-    lib().orc_const_map(W, C.c_double(pitch_iso), _p(out, _D))
+    if alpha == 0.5:
+        lib().orc_const_map(W, C.c_double(pitch_iso), _p(out, _D))
+    else:
+        lib().orc_const_map_w(W, C.c_double(pitch_iso), C.c_double(alpha), _p(out, _D))
     return out
 
 
-def filter_sinogram(proj, xshift, yshift, pitch_iso=0.032509752, acc64=False):
+def filter_sinogram(proj, xshift, yshift, pitch_iso=0.032509752, acc64=False, alpha=0.5):
     n, H, W = proj.shape
     p = np.ascontiguousarray(proj, np.float32)
-    cm = const_map(W, pitch_iso)
+    cm = const_map(W, pitch_iso, alpha)
     xs = np.ascontiguousarray(xshift, np.float64)
     ys = np.ascontiguousarray(yshift, np.float64)
-    if acc64:
+    if alpha != 0.5:  # the window option exists in the FP64 form only
+        out = np.zeros((n, H, W))
+        lib().orc_filter_sinogram64_w(H, W, n, _p(p, _F), _p(out, _D), _p(cm, _D), _p(xs, _D), _p(ys, _D),
+                                      C.c_double(pitch_iso), C.c_double(alpha))
+    elif acc64:
         out = np.zeros((n, H, W))
         lib().orc_filter_sinogram64(H, W, n, _p(p, _F), _p(out, _D), _p(cm, _D), _p(xs, _D), _p(ys, _D),
                                     C.c_double(pitch_iso))

@@ -431,3 +431,39 @@ def test_adjoint_transpose_identity_monochromatic():
     e[z, m, yy, xx] = 1.0
     Ae = O.logproj(g, e, at, pdf, 2.0)
     assert ATw[z, m, yy, xx] == pytest.approx(float(np.sum(Ae * w)), rel=1e-12, abs=1e-14)
+
+
+def test_ramlak_window_is_the_band_limited_ramp():
+    """The window parameter's alpha = 1 end (the build's opt-in Ram-Lak; the reference fixes alpha = 0.5) is the
+    band-limited ramp: off-centre taps h(d) = (p/2) * int_{-F}^{F} |f| cos(2 pi f d p) df with F = 1.6 / (2 p) and
+    p the pixel pitch at the isocentre, by numerical quadrature, and the centre coefficient (p/2) F^2,
+    read through the filter of an impulse.  And the kernel is linear in alpha, h_0.5 = (h_0 + h_1) / 2, with
+    h_0.5 the reference's (pinned bitwise above)."""
+    W, pitch = 80, 0.032509752
+    p = pitch * 800 / W
+    F = 1.6 / (2 * p)
+    h1 = O.const_map(W, pitch, alpha=1.0)
+    # Gauss-Legendre on [0, F] (400 nodes: exact to rounding for the at most 25 oscillations of the integrand)
+    t, wt = np.polynomial.legendre.leggauss(400)
+    f = 0.5 * F * (t + 1.0)
+    for d in (1, 2, 3, 7, 15, 31):
+        x = d * p
+        want = 0.5 * p * 2 * (0.5 * F) * np.sum(wt * f * np.cos(2 * np.pi * f * x))
+        assert h1[40, 40 - d] == pytest.approx(want, rel=1e-9, abs=1e-12 * abs(h1).max())
+        assert h1[40, 40 + d] == h1[40, 40 - d]
+    h0 = O.const_map(W, pitch, alpha=0.0)
+    hh = O.const_map(W, pitch, alpha=0.5)
+    np.testing.assert_allclose(hh, (h0 + h1) / 2, rtol=1e-12, atol=1e-15 * np.abs(hh).max())
+    # centre coefficient through an impulse on a centred row (Cm = 1 at the panel centre's neighbourhood)
+    proj = np.ones((1, 1, W), np.float32)
+    m0 = W // 2 - 1
+    proj[0, 0, m0] = np.float32(np.exp(-1.0))
+    out1 = O.filter_sinogram(proj, np.zeros(1), np.zeros(1), pitch, alpha=1.0)
+    out5 = O.filter_sinogram(proj, np.zeros(1), np.zeros(1), pitch, acc64=True)
+    q = -np.log(np.float32(np.exp(-1.0)))
+    pos1 = abs(-W / 2.0 - 1 + m0) * p
+    pos2 = abs(-W / 2.0 - 1) * p
+    cm = 100.0 / np.sqrt(100.0 ** 2 + pos1 ** 2 + pos2 ** 2)
+    assert out1[0, 0, m0] == pytest.approx(0.5 * p * F ** 2 * q * cm, rel=1e-6)
+    coeff05 = 1.6 ** 2 / 4 / p * (0.25 + 0.5 * (0.0 + (np.cos(np.pi) - 1) / np.pi ** 2))
+    assert out5[0, 0, m0] == pytest.approx(coeff05 * q * cm, rel=1e-6)

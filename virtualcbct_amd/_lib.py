@@ -107,6 +107,7 @@ SIGNATURES = {
     "vcbct_fdk_plan_create_half_fan": (c_int, [C.POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int, c_int, c_double,
                                                c_double, c_int, P_D, P_D, P_D, c_double, c_double, c_int]),
     "vcbct_fdk_plan_width": (c_int, [c_void_p]),
+    "vcbct_fdk_plan_set_window": (c_int, [c_void_p, c_double]),
     "vcbct_fdk_plan_bp_staged": (c_int, [c_void_p]),
     "vcbct_fdk_offset_weight_host": (c_int, [c_int, c_int, c_int, c_void_p, P_D, P_D, c_double, c_int, c_void_p, P_D,
                                              P_D]),

@@ -21,9 +21,12 @@ static const double kPI = 3.141592653589793238462643383279;
 
 // ------------------------------------------------------------------------------------------------
 // Filter kernel h(d), F/filterSinogram.cpp:10-65 (off-centre taps) and :78-80 (centre coefficient).
-// FP64, the reference's operation order.
-static double ramp_offcentre(int md, double pitch_iso, int W) {
-    const double CUT = 1.60, GAM = 1.00, ALP = 0.50;
+// FP64, the reference's operation order.  ALP is the window parameter: 0.5 is the reference's (ALPHA,
+// F/physParams.h:39-41); 1 drops the window terms, leaving the band-limited ramp (Ram-Lak), an opt-in of the
+// FDK plan (vcbct_fdk_plan_set_window).
+constexpr double kRefAlpha = 0.50;
+static double ramp_offcentre(int md, double pitch_iso, int W, double ALP = kRefAlpha) {
+    const double CUT = 1.60, GAM = 1.00;
     double theConst = 1.0 / (8.0 * kPI * kPI * pitch_iso * 800 / W);
     double gamma_x_cutoffFreq = GAM * CUT;
     double PI_x_cutoffFreq = kPI * CUT;
@@ -42,8 +45,8 @@ static double ramp_offcentre(int md, double pitch_iso, int W) {
             xi * xi * one_minus_alpha * (ct * cos_omega - st * sin_omega - 1.0) +
             eta * eta * one_minus_alpha * (ct * cos_omega + st * sin_omega - 1.0));
 }
-static double ramp_centre(int W, double pitch_iso) {
-    const double CUT = 1.60, GAM = 1.00, ALP = 0.50;
+static double ramp_centre(int W, double pitch_iso, double ALP = kRefAlpha) {
+    const double CUT = 1.60, GAM = 1.00;
     double omega = kPI / GAM;
     return CUT * CUT / 4.0 / (pitch_iso * 800 / W) *
            (ALP * 0.5 + (1.0 - ALP) * (1.0 / omega * sin(omega) + 1.0 / omega / omega * (cos(omega) - 1.0)));
@@ -77,17 +80,31 @@ struct FilterPlan {
 };
 
 static std::mutex g_plan_mu;
-static std::unordered_map<unsigned long long, FilterPlan> g_plans;  // key: device, W, pitch bits
+struct FilterKey {
+    int dev, W;
+    double pitch, alpha;
+    bool operator==(const FilterKey& o) const {
+        return dev == o.dev && W == o.W && pitch == o.pitch && alpha == o.alpha;
+    }
+};
+struct FilterKeyHash {
+    size_t operator()(const FilterKey& k) const {
+        unsigned long long pb, ab;
+        memcpy(&pb, &k.pitch, 8);
+        memcpy(&ab, &k.alpha, 8);
+        return (size_t)(((unsigned long long)k.dev << 56) ^ ((unsigned long long)k.W << 40) ^
+                        (pb * 0x9E3779B97F4A7C15ull) ^ (ab * 0xC2B2AE3D27D4EB4Full));
+    }
+};
+static std::unordered_map<FilterKey, FilterPlan, FilterKeyHash> g_plans;  // per device, width, pitch and window
 
-static int get_plan(int W, double pitch_iso, FilterPlan** out) {
+static int get_plan(int W, double pitch_iso, FilterPlan** out, double alpha = kRefAlpha) {
     int dev = 0;
     VCBCT_HIP(hipGetDevice(&dev));
-    unsigned long long pb;
-    memcpy(&pb, &pitch_iso, 8);
-    unsigned long long key = ((unsigned long long)dev << 56) ^ ((unsigned long long)W << 40) ^ (pb * 0x9E3779B97F4A7C15ull);
+    const FilterKey key{dev, W, pitch_iso, alpha};
     std::lock_guard<std::mutex> lk(g_plan_mu);
     auto it = g_plans.find(key);
-    if (it != g_plans.end() && it->second.W == W) {
+    if (it != g_plans.end()) {
         *out = &it->second;
         return 0;
     }
@@ -98,10 +115,10 @@ static int get_plan(int W, double pitch_iso, FilterPlan** out) {
     while (P.L < 2 * W - 1) { P.L <<= 1; P.logL++; }
     if (P.L > 4096) return fail(VCBCT_EINVAL, "filter: projection width %d too large (max 2048)", W);
     std::vector<std::complex<double>> h(P.L, 0.0);
-    h[0] = ramp_centre(W, pitch_iso);
+    h[0] = ramp_centre(W, pitch_iso, alpha);
     for (int d = 1; d <= W - 1; ++d) {
-        h[d] = ramp_offcentre(d, pitch_iso, W);           // out[m] gets h(d) q[m-d]
-        h[P.L - d] = ramp_offcentre(-d, pitch_iso, W);
+        h[d] = ramp_offcentre(d, pitch_iso, W, alpha);           // out[m] gets h(d) q[m-d]
+        h[P.L - d] = ramp_offcentre(-d, pitch_iso, W, alpha);
     }
     fft64(h);
     std::vector<float2> Hf(P.L), tw(P.L);
@@ -764,6 +781,17 @@ extern "C" int vcbct_fdk_plan_create_half_fan(vcbct_fdk_plan** out, int H, int W
 }
 
 extern "C" int vcbct_fdk_plan_width(const vcbct_fdk_plan* p) { return p ? p->W : -1; }
+
+extern "C" int vcbct_fdk_plan_set_window(vcbct_fdk_plan* p, double alpha) {
+    if (!p) return fail(VCBCT_EINVAL, "fdk_plan_set_window: plan is NULL");
+    if (!(alpha >= 0.0 && alpha <= 1.0)) return fail(VCBCT_EINVAL, "fdk_plan_set_window: alpha %g not in [0, 1]", alpha);
+    VCBCT_HIP(hipSetDevice(p->device));
+    FilterPlan* fp = nullptr;
+    int rc = get_plan(p->W, p->pitch, &fp, alpha);
+    if (rc) return rc;
+    p->fp = fp;
+    return 0;
+}
 
 extern "C" void vcbct_fdk_plan_destroy(vcbct_fdk_plan* p) {
     if (!p) return;

@@ -179,9 +179,11 @@ class FdkPlan:
 
     def __init__(self, projHeight, projWidth, nProj, reconSize, reconSlices, zstart, reconStep, reconScale, xfactor,
                  anglesRad, xshift, yshift, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, device=0, half_fan=False,
-                 taper_px=0.0):
+                 taper_px=0.0, window_alpha=0.5):
         """half_fan=True: an offset-panel scan; the plan weights and pads the projections inside its filter call
-        (see offset_weight_device) and filters / backprojects the padded panel (width `self.width`)."""
+        (see offset_weight_device) and filters / backprojects the padded panel (width `self.width`).
+        window_alpha: the filter's window parameter, 0.5 the reference's kernel (ALPHA, F/physParams.h:39-41),
+        1.0 the band-limited ramp without a window (Ram-Lak); not a reference option."""
         a, xs, ys = [np.ascontiguousarray(v, np.float64) for v in (anglesRad, xshift, yshift)]
         if min(a.size, xs.size, ys.size) < nProj:
             raise ValueError("angles/xshift/yshift need one entry per projection")
@@ -194,6 +196,9 @@ class FdkPlan:
         else:
             check(lib().vcbct_fdk_plan_create(C.byref(h), *args, int(device)))
         self.h = h
+        if window_alpha != 0.5:
+            check(lib().vcbct_fdk_plan_set_window(h, float(window_alpha)))
+        self.window_alpha = float(window_alpha)
         self.width = lib().vcbct_fdk_plan_width(h)
         self.shape_proj = (int(nProj), int(projHeight), int(projWidth))
         self.shape_filtered = (int(nProj), int(projHeight), int(self.width))
