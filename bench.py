@@ -391,14 +391,17 @@ def bench_batched(line, inp, dev, stream, e, NE):
     del outs3
 
 
-def bench_decompose(line, td, dev, stream, e):
+def bench_decompose(line, td, dev, stream, e, iters=4):
     """Dual-energy decomposition: four problems (512^2 x {64, 96, 112, 128} FP32 maps, 360 angles, 400^2) under
-    80 and 120 kV spectra, optimised together by Adam on the adjoint kernel.  Timed: Adam iterations (each:
-    8 log projections + 8 adjoints + the optimiser step)."""
+    80 and 120 kV spectra, optimised together by Adam.  The fused, graph-replayed path (per problem one
+    traversal and one scatter for both spectra, the Adam update on the device) runs `iters` iterations after one
+    eager warm-up and one capture; the per-problem autograd path (one log projection and one backward pass per
+    problem and spectrum, torch Adam) is timed on two iterations beside it."""
+    import time
     import torch
     from virtualcbct_amd import io, phantoms
     from virtualcbct_amd.projector import Projector
-    from virtualcbct_amd.decompose import decompose
+    from virtualcbct_amd.decompose import FusedDecomposition, decompose_autograd
     spectra = {kv: phantoms.synthetic_spectrum(td, kv) for kv in (80, 120)}
     ang, xs, ys = phantoms.flexmap_angles(360)
     c, s, x, y = io.trig_table(400, 400, ang, xs, ys)
@@ -417,19 +420,40 @@ def bench_decompose(line, td, dev, stream, e):
         problems.append({"projectors": projs, "targets": targets, "shape": tuple(truth.shape), "xfactor": 1.0})
         del truth
     init = [torch.full(p["shape"], 0.2, device=dev) for p in problems]
-    decompose(problems, iters=1, lr=0.02, init=init)  # warm-up (allocations, tuning)
+    # per-problem autograd path
+    decompose_autograd(problems, iters=1, lr=0.02, init=init)  # warm-up (allocations)
     torch.cuda.synchronize()
-    iters = 2
     e[0].record(stream)
-    _, hist = decompose(problems, iters=iters, lr=0.02, init=init)
+    decompose_autograd(problems, iters=2, lr=0.02, init=init)
     e[1].record(stream)
     torch.cuda.synchronize()
+    autograd_ms = e[0].elapsed_time(e[1]) / 2
+    # fused, graph-replayed path: the eager first iteration and the capture outside the timing
+    st = FusedDecomposition(problems, lr=0.02, init=init, graph=True)
+    st.step()
+    st.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e[0].record(stream)
+    for _ in range(iters):
+        st.step()
+    e[1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
     ms = e[0].elapsed_time(e[1]) / iters
+    hist = st.losses()
     line["decompose"] = {"workload": "4 problems 512^2 x {64,96,112,128} FP32 maps, 360 angles, 400^2, 80+120 kV "
-                                     "(91+111 bins), Adam",
-                         "iteration_ms": ms, "gradients_per_s": 8 * 1e3 / ms,
-                         "loss_first_iterations": [float(h) for h in hist]}
-    log(f"decompose: {ms:.1f} ms per Adam iteration (8 log projections + 8 adjoints)")
+                                     "(71+111 bins), Adam lr 0.02",
+                         "iteration_ms": ms, "iterations_timed": iters, "iterations_total": st.iteration,
+                         "wall_s_timed": wall, "problem_gradients_per_s": 4 * 1e3 / ms,
+                         "spectrum_gradients_per_s": 8 * 1e3 / ms,
+                         "per_problem_autograd_iteration_ms": autograd_ms,
+                         "speedup_vs_per_problem": autograd_ms / ms,
+                         "channels": st.channels, "graph_replayed": st.graph is not None,
+                         "loss_first": hist[0], "loss_last": hist[-1],
+                         "loss_every_10": [h for k, h in enumerate(hist) if k % 10 == 0]}
+    log(f"decompose: fused + graph {ms:.1f} ms per Adam iteration over {iters} iterations ({wall:.1f} s wall), "
+        f"per-problem autograd path {autograd_ms:.1f} ms; loss {hist[0]:.4e} -> {hist[-1]:.4e}")
     for p in problems:
         for P in p["projectors"]:
             P.close()
@@ -552,6 +576,8 @@ def main():
     ap.add_argument("--only", choices=["adjoint", "batched", "decompose", "fdk", "phantom", "iterative",
                                                  "detector"], default=None,
                     help="run one secondary section alone and print its JSON (used under the profiler)")
+    ap.add_argument("--decompose-iters", type=int, default=200,
+                    help="Adam iterations of `--only decompose` (the default bench line times 4)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (scale for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -603,7 +629,7 @@ def main():
         if args.only == "adjoint":
             bench_adjoint(line, inp, dev, stream, e)
         elif args.only == "decompose":
-            bench_decompose(line, td, dev, stream, e)
+            bench_decompose(line, td, dev, stream, e, iters=args.decompose_iters)
         elif args.only == "phantom":
             bench_phantom(line, td, dev, stream, e)
         elif args.only == "iterative":

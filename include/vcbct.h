@@ -189,6 +189,30 @@ int vcbct_projector_logproj_backward(vcbct_projector* p, const float* dev_gp, fl
  * kept instead of a second traversal; the per-energy terms are never stored).  The backward does not read the
  * volume. */
 int vcbct_projector_logproj_save(vcbct_projector* p, float* dev_p, float* dev_L, double xfactor, void* stream);
+/* Several spectra over the same rays, for a multi-spectrum (dual-energy) loss (no reference counterpart):
+ * nspec (1..4) attenuation tables and pdfs concatenated, host_atten [spec][mat][NE[spec]] and host_pdf
+ * [spec][NE[spec]], as vcbct_projector_set_spectrum takes one.  Their joint spectral basis (of the tables side
+ * by side) gives the channels the gradient is scattered in.  Independent of the single spectrum of
+ * vcbct_projector_set_spectrum. */
+int vcbct_projector_set_spectra(vcbct_projector* p, int nspec, const float* host_atten, const float* host_pdf,
+                                const int* NE);
+/* Channels of the multi-spectrum loss's scatter (num_material, or the joint basis's rank). */
+int vcbct_projector_multi_channels(const vcbct_projector* p);
+/* loss = sum_s sum_rays (p_s - t_s)^2, p_s the log projection of the FP32 maps (vcbct_projector_set_volume_f32)
+ * under spectrum s, and dev_grad [mat][z][y][x] (overwritten) its gradient: ONE traversal per ray for all the
+ * spectra (their line integrals are the same), the per-ray coefficients summed over the spectra, ONE scatter.
+ * dev_targets [spec][n_angles][H][W] FP32; dev_loss (optional) one double.  Equal to the sum over the spectra of
+ * vcbct_projector_adjoint with each spectrum, at half the traversals.  Allocates on its first call only. */
+int vcbct_projector_multi_loss_grad(vcbct_projector* p, const float* dev_targets, float* dev_grad, double* dev_loss,
+                                    double xfactor, void* stream);
+/* One Adam step on n FP32 values (no reference counterpart; the decomposition's optimiser on the device):
+ * dev_step (one int on the device) is incremented first, then m <- b1 m + (1 - b1) g, v <- b2 v + (1 - b2) g^2,
+ * x <- x - lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps), and x <- max(x, lower) unless lower is -inf.
+ * Nothing allocated, nothing synchronised: with vcbct_projector_multi_loss_grad an iteration can be captured
+ * in a hipGraph and replayed (the step count lives on the device). */
+int vcbct_adam_step(size_t n, float* dev_x, const float* dev_grad, float* dev_m, float* dev_v, int* dev_step,
+                    double lr, double beta1, double beta2, double eps, double lower, void* stream);
+
 /* Number of channels the adjoint scatters (no reference counterpart): num_material, or r < num_material when
  * the spectrum's attenuation table has a rank-r form mu = B (B^T mu) to 1e-6 of its largest entry (B orthonormal,
  * from the table's eigenvectors).  Attenuation between ~10 and ~150 keV is close to a photoelectric and a

@@ -293,6 +293,16 @@ def test_decompose_c5_full_size(torch, tmp_path):
         assert e < 2e-2, (k, (mat, z, y, x), gk, fd)
     print(f"c5 first gradient vs central differences (h = {h}) on 16 voxels: worst relative error {worst:.3e}",
           flush=True)
+    # the fused step (one traversal and one scatter for both spectra) gives the per-spectrum autograd gradient
+    P0 = pr["projectors"][0]
+    P0.set_spectra([Q.spectrum for Q in pr["projectors"]])
+    P0.set_volume(init[0])
+    gf, lf = P0.multi_loss_grad(torch.stack(pr["targets"]).contiguous(), 1.0)
+    ef = float((gf - grad).abs().max() / grad.abs().max())
+    print(f"c5 fused two-spectrum gradient vs the per-spectrum autograd gradient: {ef:.3e} "
+          f"({P0.multi_channels} channels)", flush=True)
+    assert ef < 1e-5
+    assert float(lf.item()) == pytest.approx(loss_of(init[0]), rel=1e-6)
     _, hist = decompose(problems, iters=3, lr=0.02, init=init)
     print(f"c5 loss over 3 Adam iterations: {hist}", flush=True)
     assert hist[1] < hist[0] and hist[2] < hist[1]

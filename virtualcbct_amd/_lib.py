@@ -87,6 +87,11 @@ SIGNATURES = {
     "vcbct_projector_logproj_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_projector_logproj_save": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_projector_adjoint_channels": (c_int, [c_void_p]),
+    "vcbct_projector_set_spectra": (c_int, [c_void_p, c_int, P_F, P_F, P_I]),
+    "vcbct_projector_multi_channels": (c_int, [c_void_p]),
+    "vcbct_projector_multi_loss_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
+    "vcbct_adam_step": (c_int, [C.c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double,
+                                c_double, c_double, c_double, c_void_p]),
     "vcbct_projector_logproj_backward_saved": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_prepare_const_map": (c_int, [c_int, c_double, P_D]),
     "vcbct_filter_sinogram": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, P_D, P_D, c_double, c_void_p]),

@@ -133,6 +133,8 @@ static int check_geometry(const vcbct_geometry* g) {
 
 using namespace vcbct;
 
+static void free_multi(vcbct_projector* p);
+
 extern "C" const char* vcbct_last_error(void) { return g_err; }
 
 extern "C" const char* vcbct_version(void) { return "vcbct 0.1 (gfx950)"; }
@@ -200,6 +202,7 @@ extern "C" void vcbct_projector_destroy(vcbct_projector* p) {
     (void)hipFree(p->d_muf);
     (void)hipFree(p->d_pdff);
     (void)hipFree(p->d_ch);
+    free_multi(p);
     free(p);
 }
 
@@ -301,6 +304,88 @@ extern "C" int vcbct_projector_set_spectrum(vcbct_projector* p, const float* att
     }
     p->NE = NE;
     return 0;
+}
+
+static void free_multi(vcbct_projector* p) {
+    for (int s = 0; s < VCBCT_MAX_SPECTRA; ++s) {
+        (void)hipFree(p->ms[s].d_mu);
+        (void)hipFree(p->ms[s].d_pdf);
+        (void)hipFree(p->ms[s].d_muf);
+        (void)hipFree(p->ms[s].d_ch);
+        p->ms[s].d_mu = p->ms[s].d_pdf = p->ms[s].d_ch = nullptr;
+        p->ms[s].d_muf = nullptr;
+        p->ms[s].NE = 0;
+    }
+    p->nspec_multi = 0;
+}
+
+extern "C" int vcbct_projector_set_spectra(vcbct_projector* p, int nspec, const float* atten, const float* pdf,
+                                           const int* NE) {
+    if (!p || !atten || !pdf || !NE) return fail(VCBCT_EINVAL, "set_spectra: NULL argument");
+    if (nspec < 1 || nspec > VCBCT_MAX_SPECTRA)
+        return fail(VCBCT_EINVAL, "set_spectra: %d spectra, 1 to %d supported", nspec, VCBCT_MAX_SPECTRA);
+    int tot = 0;
+    for (int s = 0; s < nspec; ++s) {
+        if (NE[s] < 1 || NE[s] > VCBCT_MAX_NE) return fail(VCBCT_EINVAL, "set_spectra: NE[%d] = %d out of range", s, NE[s]);
+        tot += NE[s];
+    }
+    VCBCT_HIP(hipSetDevice(p->device));
+    free_multi(p);
+    const int nm = p->g.nmat;
+    // the joint table [nmat][sum NE] (spectrum s's bins at columns off_s ..), its basis, and per spectrum the
+    // FP64 copies and the channel table B^T mu_s
+    std::vector<double> joint((size_t)nm * tot);
+    std::vector<int> off(nspec + 1, 0);
+    for (int s = 0; s < nspec; ++s) off[s + 1] = off[s] + NE[s];
+    for (int s = 0; s < nspec; ++s)
+        for (int m = 0; m < nm; ++m)
+            for (int e = 0; e < NE[s]; ++e)
+                joint[(size_t)m * tot + off[s] + e] = atten[(size_t)nm * off[s] + (size_t)m * NE[s] + e];
+    double B[VCBCT_MAX_MAT][VCBCT_MAX_MAT];
+    const int r = spectral_basis(joint.data(), nm, tot, 1e-6, B);
+    const int nch = r < nm ? r : nm;
+    p->mix_multi.nch = nch;
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+        for (int k = 0; k < VCBCT_MAX_MAT; ++k)
+            p->mix_multi.b[m][k] = r < nm ? (float)B[m][k] : (m == k && m < nm ? 1.0f : 0.0f);
+    for (int s = 0; s < nspec; ++s) {
+        const int ne = NE[s];
+        const float* at = atten + (size_t)nm * off[s];
+        const float* pd = pdf + off[s];
+        std::vector<double> mu((size_t)nm * ne), pdd(ne), ch((size_t)nch * ne);
+        for (size_t i = 0; i < mu.size(); ++i) mu[i] = at[i];
+        for (int e = 0; e < ne; ++e) pdd[e] = pd[e];
+        for (int k = 0; k < nch; ++k)
+            for (int e = 0; e < ne; ++e) {
+                double v = 0.0;
+                if (r < nm)
+                    for (int m = 0; m < nm; ++m) v += B[m][k] * mu[(size_t)m * ne + e];
+                else
+                    v = mu[(size_t)k * ne + e];
+                ch[(size_t)k * ne + e] = v;
+            }
+        auto& t = p->ms[s];
+        if (dev_malloc((void**)&t.d_mu, mu.size() * sizeof(double)) != hipSuccess ||
+            dev_malloc((void**)&t.d_pdf, pdd.size() * sizeof(double)) != hipSuccess ||
+            dev_malloc((void**)&t.d_muf, mu.size() * sizeof(float)) != hipSuccess ||
+            dev_malloc((void**)&t.d_ch, ch.size() * sizeof(double)) != hipSuccess) {
+            free_multi(p);
+            return fail(VCBCT_ENOMEM, "device allocation of the spectra failed");
+        }
+        VCBCT_HIP(hipMemcpy(t.d_mu, mu.data(), mu.size() * sizeof(double), hipMemcpyHostToDevice));
+        VCBCT_HIP(hipMemcpy(t.d_pdf, pdd.data(), pdd.size() * sizeof(double), hipMemcpyHostToDevice));
+        VCBCT_HIP(hipMemcpy(t.d_muf, at, mu.size() * sizeof(float), hipMemcpyHostToDevice));
+        VCBCT_HIP(hipMemcpy(t.d_ch, ch.data(), ch.size() * sizeof(double), hipMemcpyHostToDevice));
+        t.NE = ne;
+    }
+    p->nspec_multi = nspec;
+    return 0;
+}
+
+extern "C" int vcbct_projector_multi_channels(const vcbct_projector* p) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    if (p->nspec_multi < 1) return fail(VCBCT_ESTATE, "no spectra set (vcbct_projector_set_spectra)");
+    return p->mix_multi.nch;
 }
 
 extern "C" int vcbct_projector_adjoint_channels(const vcbct_projector* p) {
@@ -435,6 +520,41 @@ extern "C" int vcbct_projector_logproj_backward(vcbct_projector* p, const float*
                                                 void* stream) {
     if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
     return run_adjoint(p, dev_gp, dev_grad, nullptr, xfactor, VCBCT_ADJ_UPSTREAM, stream);
+}
+
+extern "C" int vcbct_projector_multi_loss_grad(vcbct_projector* p, const float* dev_targets, float* dev_grad,
+                                               double* dev_loss, double xfactor, void* stream) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    int rc = need(p, false, true, false);
+    if (rc) return rc;
+    if (p->nspec_multi < 1) return fail(VCBCT_ESTATE, "no spectra set (vcbct_projector_set_spectra)");
+    if (!dev_targets || !dev_grad) return fail(VCBCT_EINVAL, "multi_loss_grad: NULL argument");
+    if (!(xfactor > 0)) return fail(VCBCT_EINVAL, "xfactor must be positive");
+    VCBCT_HIP(hipSetDevice(p->device));
+    hipStream_t st = (hipStream_t)stream;
+    const size_t nv = vox_count(p->g);
+    if (!p->d_grad_vox && dev_malloc((void**)&p->d_grad_vox, nv * 32) != hipSuccess)
+        return fail(VCBCT_ENOMEM, "device allocation of the gradient volume failed");
+    const size_t nrays = (size_t)p->g.nAng * p->g.H * p->g.W;
+    if (!p->d_coef && dev_malloc((void**)&p->d_coef, nrays * p->g.nmat * sizeof(float)) != hipSuccess)
+        return fail(VCBCT_ENOMEM, "device allocation of the adjoint coefficients failed");
+    SpecSet ss{};
+    for (int s = 0; s < p->nspec_multi; ++s) {
+        ss.mu[s] = p->ms[s].d_mu;
+        ss.pdf[s] = p->ms[s].d_pdf;
+        ss.muf[s] = p->ms[s].d_muf;
+        ss.ch[s] = p->ms[s].d_ch;
+        ss.NE[s] = p->ms[s].NE;
+    }
+    ss.nspec = p->nspec_multi;
+    ss.nch = p->mix_multi.nch;
+    rc = launch_zero(p->d_grad_vox, nv * 4 * ss.nch, st);
+    if (rc) return rc;
+    if (dev_loss && (rc = launch_zero(dev_loss, sizeof(double), st))) return rc;
+    rc = launch_adjoint_multi(p->g, p->d_tables, p->d_vox_f32, ss, xfactor, dev_targets, p->d_coef, p->d_grad_vox,
+                              dev_loss, st);
+    if (rc) return rc;
+    return launch_untranspose_mix(p->d_grad_vox, dev_grad, p->g, p->mix_multi, st);
 }
 
 extern "C" int vcbct_projector_logproj_save(vcbct_projector* p, float* dev_p, float* dev_L, double xfactor,

@@ -681,6 +681,86 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, c
     }
 }
 
+// The multi-spectrum form of k_adjoint_coef (loss mode): one traversal per ray gives the line integrals, and
+// for every spectrum s the log projection p_s, its residual against the target t_s ([s][ray]) and its
+// coefficients; the loss sums (p_s - t_s)^2 over spectra and rays, and the stored coefficient of a channel is
+// the sum over the spectra, c_k = sum_s 2 (p_s - t_s) chbar_sk / xf, so that one scatter serves all spectra.
+template <int NCH>  // the channel count (the arrays of the spectral loop stay in registers)
+__global__ __launch_bounds__(256) VCBCT_WAVES7 void k_adjoint_coef_multi(ProjGeom g, const double* __restrict__ tab,
+                                                                         const float4* __restrict__ vox, SpecSet ss,
+                                                                         double inv_xf, int quads_per_xcd,
+                                                                         const float* __restrict__ tg,
+                                                                         float* __restrict__ coef,
+                                                                         double* __restrict__ loss) {
+    __shared__ WaveLds step_tabs[4];
+    WaveLds* steps = &step_tabs[threadIdx.x >> 6];
+    const Tile t = tile_of(g.nAng, g.W, quads_per_xcd);
+    const int lane = threadIdx.x & 63;
+    const int j = t.rg * 64 + lane;
+    const int jj = t.col;
+    const int a = t.a;
+    if (jj >= g.W) return;
+    const bool active = j < g.H;
+    const double c = ldg(tab, a), sn = ldg(tab, g.nAng + a), xc = ldg(tab, 2 * g.nAng + a),
+                 yc = ldg(tab, 3 * g.nAng + a);
+    AccumL<float4> acc;
+    acc.vox = vox;
+    acc.Nz = g.Nz;
+#pragma unroll
+    for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0f;
+    traverse_wave(g, c, sn, xc, yc, j, jj, active, acc, steps);
+
+    double lossc = 0.0;
+    if (active) {
+        const size_t nrays = (size_t)g.nAng * g.H * g.W;
+        const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - j) * g.W + jj;
+        const float k2 = (float)(-inv_xf * 1.4426950408889634);
+        double cacc[NCH];
+#pragma unroll
+        for (int m = 0; m < NCH; ++m) cacc[m] = 0.0;
+        for (int s = 0; s < ss.nspec; ++s) {
+            const int NE = ss.NE[s];
+            const double* __restrict__ mu = ss.mu[s];
+            const double* __restrict__ pdf = ss.pdf[s];
+            const float* __restrict__ muf = ss.muf[s];
+            const double* __restrict__ ch = ss.ch[s];
+            double sw = 0.0, se = 0.0, mub[VCBCT_MAX_MAT] = {0, 0, 0, 0, 0, 0};
+            for (int e = 0; e < NE; ++e) {
+                float tr = 0.0f;
+#pragma unroll
+                for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+                    if (m < g.nmat) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
+                const double pe = ldg(pdf, e);
+                const double we = pe * (double)exp2f(tr * k2);
+                sw += pe;
+                se += we;
+#pragma unroll
+                for (int m = 0; m < NCH; ++m) mub[m] += we * ldg(ch, m * NE + e);
+            }
+            double p;
+            if (se < 1e-30 * sw) {  // opaque ray: FP64 terms relative to the smallest exponent
+                p = mubar_opaque(acc.L, g.nmat, mu, pdf, NE, inv_xf, sw, mub, ch, NCH);
+            } else {
+                p = -log(se / sw);
+#pragma unroll
+                for (int m = 0; m < NCH; ++m) mub[m] /= se;
+            }
+            const double res = p - (double)ldg(tg, (size_t)s * nrays + o);
+            lossc += res * res;
+#pragma unroll
+            for (int m = 0; m < NCH; ++m) cacc[m] += 2.0 * res * mub[m];
+        }
+#pragma unroll
+        for (int m = 0; m < NCH; ++m) stg(coef, (size_t)m * nrays + o, (float)(cacc[m] * inv_xf));
+    }
+    if (loss) {
+        double v = lossc;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) atomicAdd(loss, v);
+    }
+}
+
 // The coefficients of the upstream-gradient mode from line integrals saved by the forward log projection
 // (the same traversal, so the same FP32 sums): one thread per ray, the epilogue of k_adjoint_coef.
 __global__ __launch_bounds__(256) void k_adjoint_coef_saved(ProjGeom g, const float* __restrict__ Ls,
@@ -844,6 +924,31 @@ int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, co
                            coef, loss);
     VCBCT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, sp.nch, grad_vox);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_adjoint_multi(const ProjGeom& g, const double* tables, const float* vox, const SpecSet& ss,
+                         double xfactor, const float* targets, float* coef, float* grad_vox, double* loss,
+                         hipStream_t st) {
+    int qpx;
+    int nb = n_blocks(g, g.nAng, qpx);
+    if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
+#define VCBCT_LAUNCH_MULTI(C)                                                                                   \
+    hipLaunchKernelGGL(k_adjoint_coef_multi<C>, dim3(nb), dim3(256), 0, st, g, tables, (const float4*)vox, ss, \
+                       1.0 / xfactor, qpx, targets, coef, loss)
+    switch (ss.nch) {
+        case 1: VCBCT_LAUNCH_MULTI(1); break;
+        case 2: VCBCT_LAUNCH_MULTI(2); break;
+        case 3: VCBCT_LAUNCH_MULTI(3); break;
+        case 4: VCBCT_LAUNCH_MULTI(4); break;
+        case 5: VCBCT_LAUNCH_MULTI(5); break;
+        case 6: VCBCT_LAUNCH_MULTI(6); break;
+        default: return fail(VCBCT_EINVAL, "adjoint: %d channels", ss.nch);
+    }
+#undef VCBCT_LAUNCH_MULTI
+    VCBCT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, ss.nch, grad_vox);
     VCBCT_HIP(hipGetLastError());
     return 0;
 }

@@ -88,6 +88,18 @@ struct Spectrum {
     int nch;
 };
 
+// Several spectra over the same rays (dual-energy decomposition): each spectrum's tables, and the channels of
+// the joint basis of all of them (the basis of the concatenated attenuation tables).
+#define VCBCT_MAX_SPECTRA 4
+struct SpecSet {
+    const double* mu[VCBCT_MAX_SPECTRA];   // [nmat][NE_s]
+    const double* pdf[VCBCT_MAX_SPECTRA];  // [NE_s]
+    const float* muf[VCBCT_MAX_SPECTRA];   // [nmat][NE_s] as given
+    const double* ch[VCBCT_MAX_SPECTRA];   // [nch][NE_s], the channel table of spectrum s
+    int NE[VCBCT_MAX_SPECTRA];
+    int nspec, nch;
+};
+
 // The mix from the adjoint's channels to the maps: grad_m = sum_k b[m][k] G_k (identity without a basis).
 struct ChannelMix {
     float b[VCBCT_MAX_MAT][VCBCT_MAX_MAT];
@@ -134,6 +146,11 @@ int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, co
                    const float* target_or_gp, int mode, float* coef, float* grad_vox, double* loss, hipStream_t st);
 int launch_adjoint_saved(const ProjGeom& g, const double* tables, const float* Ls, const Spectrum& sp,
                          double xfactor, const float* gp, float* coef, float* grad_vox, hipStream_t st);
+// loss = sum_s sum_rays (p_s - t_s)^2 over the spectra of `ss` and its gradient: one traversal per ray for all
+// spectra (their line integrals are the same), the coefficients summed over the spectra, one scatter.
+int launch_adjoint_multi(const ProjGeom& g, const double* tables, const float* vox, const SpecSet& ss,
+                         double xfactor, const float* targets, float* coef, float* grad_vox, double* loss,
+                         hipStream_t st);
 
 // a / b for a divisor b known in advance, with rb = 1 / b correctly rounded: the product a * rb is within an
 // ulp of a / b, and one correction with the exact remainder a - q b (an FMA) rounds it correctly (Markstein's
@@ -195,4 +212,12 @@ struct vcbct_projector {
     vcbct::ChannelMix mix; // channels -> maps
     int NE;
     int have_u16, have_f32;
+    // spectra of the multi-spectrum loss (vcbct_projector_set_spectra), with their joint channel basis
+    struct {
+        double *d_mu, *d_pdf, *d_ch;
+        float* d_muf;
+        int NE;
+    } ms[VCBCT_MAX_SPECTRA];
+    int nspec_multi;
+    vcbct::ChannelMix mix_multi;
 };

@@ -136,8 +136,54 @@ class Projector:
         at = np.ascontiguousarray(at[: self.g.num_material * ne])
         check(lib().vcbct_projector_set_spectrum(self.h, ptr(at, P_F), ptr(pdf, P_F), ne))
         self.NE = ne
+        self.spectrum = (at.reshape(self.g.num_material, ne).copy(), pdf.copy())
         # saved autograd state refers to the spectrum it was computed with; a backward pass checks this
         self.spectrum_generation += 1
+
+    def set_spectra(self, spectra):
+        """Spectra of the multi-spectrum loss: a list of (Attenuation [mat][NE_s], pdf_pe [NE_s]) pairs (1 to 4),
+        independent of set_spectrum's single spectrum."""
+        nm = self.g.num_material
+        ats, pds, nes = [], [], []
+        for atten, pdf_pe in spectra:
+            pdf = np.ascontiguousarray(pdf_pe, np.float32).ravel()
+            at = np.ascontiguousarray(atten, np.float32).ravel()
+            if at.size < nm * pdf.size:
+                raise ValueError("attenuation table smaller than num_material x NE")
+            ats.append(at[: nm * pdf.size])
+            pds.append(pdf)
+            nes.append(pdf.size)
+        at = np.ascontiguousarray(np.concatenate(ats))
+        pd = np.ascontiguousarray(np.concatenate(pds))
+        ne = np.ascontiguousarray(nes, np.int32)
+        check(lib().vcbct_projector_set_spectra(self.h, len(nes), ptr(at, P_F), ptr(pd, P_F),
+                                                ne.ctypes.data_as(_lib.P_I)))
+        self.nspec = len(nes)
+
+    @property
+    def multi_channels(self):
+        rc = lib().vcbct_projector_multi_channels(self.h)
+        if rc < 0:
+            check(rc)
+        return rc
+
+    def multi_loss_grad(self, targets, xfactor, grad=None, loss=None, stream=None):
+        """loss = sum over the set_spectra spectra and the rays of (p_s - t_s)^2 for the FP32 maps set with
+        set_volume, and its gradient [mat][z][y][x]: one traversal and one scatter for all spectra.  targets:
+        contiguous FP32 [nspec][n_angles][H][W] on the device.  Returns (grad, loss) (loss: one FP64 value)."""
+        torch = _torch()
+        nspec = getattr(self, "nspec", 0)
+        self._check_dev(targets, (nspec,) + self.shape_proj, "targets")
+        if grad is None:
+            grad = torch.empty(self.shape_maps, dtype=torch.float32, device=f"cuda:{self.device}")
+        self._check_dev(grad, self.shape_maps, "grad")
+        if loss is None:
+            loss = torch.zeros(1, dtype=torch.float64, device=f"cuda:{self.device}")
+        self._check_dev(loss, (1,), "loss", torch.float64)
+        check(lib().vcbct_projector_multi_loss_grad(self.h, C.c_void_p(targets.data_ptr()),
+                                                    C.c_void_p(grad.data_ptr()), C.c_void_p(loss.data_ptr()),
+                                                    float(xfactor), _stream(stream)))
+        return grad, loss
 
     @property
     def adjoint_channels(self):
