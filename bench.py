@@ -690,8 +690,12 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
                 "traffic_source": "profiles/forward_c2_traffic.txt: rocprofv3 FETCH_SIZE and WRITE_SIZE of this "
                                   "kernel, scaled by their readings for a plain 1 GiB copy (not measured in this run)",
-                "kernel": "k_forward<0, uint4>", "segments_per_launch": segs,
+                "kernel": f"k_forward<0, uint4, {P.tile[0] * P.tile[1]}>", "segments_per_launch": segs,
                 "algorithmic_bytes_per_launch": alg_bytes}
+    from virtualcbct_amd.batch import autotune_log
+    tile = {"row_groups": P.tile[0], "cols": P.tile[1], "order": P.tile[2], "split": P.tile[3],
+            "autotune": [l for l in autotune_log().strip().split("\n") if l.startswith("fwd ")]}
+    log(f"forward tile (autotuned): {P.tile}")
     line = {"metric": METRIC, "value": value, "unit": "rays*bins/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -701,7 +705,7 @@ def main():
             "config": {"workload": "25 cm water cylinder, 512x512x128 voxels (0.5 mm x 0.5 mm x 1 mm), 6 uint16 "
                                    "elemental maps, 360 angles, 400x400 detector, 115 mm half-fan offset, "
                                    "91 energy bins 10-100 keV, FP32 projections"},
-            "roofline": roofline}
+            "roofline": roofline, "tile": tile}
 
     if not args.no_extras:
         bench_fdk(line, inp, dev, stream, out)

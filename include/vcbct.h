@@ -97,6 +97,25 @@ int vcbct_projector_line_integrals(vcbct_projector* p, float* dev_L, void* strea
  * FP32: the work unit of the roofline model (12 bytes of voxel data per segment). */
 int vcbct_projector_segment_counts(vcbct_projector* p, float* dev_counts, void* stream);
 
+/* Tile shape of the intensity and log-projection launches (vcbct_projector_forward, _logproj, _logproj_save):
+ * a workgroup is row_groups groups of 64 detector rows by cols adjacent columns of one angle, one wave each
+ * (row_groups * cols = 2, 4 or 8); order 0 walks the angles inside each row block, 1 the row blocks inside
+ * each angle; split 0 divides the columns among the 8 XCDs, 1 the angles.  row_groups = 0 (the default)
+ * hands the choice to the autotuner, which times the candidates on the first launch of each geometry outside
+ * stream capture (VCBCT_NO_AUTOTUNE=1: 1 x 4, order 0, split 0) and records them in vcbct_autotune_log.  The
+ * shape changes only the schedule: every ray's value is the same under all of them.  Replaces the reference's
+ * hand-set launch shapes (P/reprojectionOnGPU.cu:900-983). */
+int vcbct_projector_set_tile(vcbct_projector* p, int row_groups, int cols, int order, int split);
+/* The same for the traversal kernels of the adjoint (vcbct_projector_adjoint, _adjoint_upstream,
+ * _logproj_backward_saved, _multi_loss_grad: the coefficient kernel and the scatter), whose workgroups are
+ * 4 waves (row_groups * cols = 4).  row_groups = 0 (the default): autotuned per geometry, the scatter on the
+ * caller's own coefficients with the gradient cleared again after the timing launches.  The scatter's
+ * floating-point additions are unordered under every shape. */
+int vcbct_projector_set_adjoint_tile(vcbct_projector* p, int row_groups, int cols, int order, int split);
+/* The shape the last intensity / log-projection launch used: shape4 = {row_groups, cols, order, split}
+ * (zeros before the first). */
+int vcbct_projector_get_tile(const vcbct_projector* p, int* shape4);
+
 /* Host drop-in for reprojectionOnGPU (P/reprojectionOnGPU.cu:838-1100, declared at
  * P/reprojectionOnGPU.h:12-19): uploads host_vol/atten/pdf, runs the forward projection and copies the
  * result to host_out [n_angles][H][W]. */

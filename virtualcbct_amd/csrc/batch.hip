@@ -318,6 +318,7 @@ extern "C" int vcbct_autotune_log(char* buf, int cap) {
         s = g_btune_log;
     }
     s += tune_log();
+    s += forward_tune_log();
     snprintf(buf, (size_t)cap, "%s", s.c_str());
     return (int)s.size();
 }

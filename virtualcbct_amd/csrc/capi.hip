@@ -424,8 +424,12 @@ extern "C" int vcbct_projector_forward(vcbct_projector* p, float* dev_out, doubl
     if (a1 < 0) a1 = p->g.nAng;
     if (a0 < 0 || a0 > a1 || a1 > p->g.nAng) return fail(VCBCT_EINVAL, "angle range [%d, %d) invalid", a0, a1);
     VCBCT_HIP(hipSetDevice(p->device));
-    return launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
-                          spec_of(p), xfactor, a0, a1, dev_out, VCBCT_MODE_INTENSITY, nullptr, (hipStream_t)stream);
+    TileShape ts = p->tile;
+    rc = launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
+                        spec_of(p), xfactor, a0, a1, dev_out, VCBCT_MODE_INTENSITY, nullptr,
+                        (hipStream_t)stream, &ts);
+    if (!rc) p->tile_used = ts;
+    return rc;
 }
 
 extern "C" int vcbct_projector_line_integrals(vcbct_projector* p, float* dev_L, void* stream) {
@@ -481,8 +485,12 @@ extern "C" int vcbct_projector_logproj(vcbct_projector* p, float* dev_p, double 
     if (!dev_p) return fail(VCBCT_EINVAL, "logproj: output is NULL");
     if (!(xfactor > 0)) return fail(VCBCT_EINVAL, "xfactor must be positive");
     VCBCT_HIP(hipSetDevice(p->device));
-    return launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
-                          spec_of(p), xfactor, 0, p->g.nAng, dev_p, VCBCT_MODE_LOGPROJ, nullptr, (hipStream_t)stream);
+    TileShape ts = p->tile;
+    rc = launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
+                        spec_of(p), xfactor, 0, p->g.nAng, dev_p, VCBCT_MODE_LOGPROJ, nullptr,
+                        (hipStream_t)stream, &ts);
+    if (!rc) p->tile_used = ts;
+    return rc;
 }
 
 static int run_adjoint(vcbct_projector* p, const float* tg, float* dev_grad, double* dev_loss, double xfactor, int mode,
@@ -505,7 +513,7 @@ static int run_adjoint(vcbct_projector* p, const float* tg, float* dev_grad, dou
     if (rc) return rc;
     if (dev_loss && (rc = launch_zero(dev_loss, sizeof(double), st))) return rc;
     rc = launch_adjoint(p->g, p->d_tables, p->d_vox_f32, spec_of(p), xfactor, tg, mode, p->d_coef, p->d_grad_vox,
-                        dev_loss, st);
+                        dev_loss, st, p->adj_tile);
     if (rc) return rc;
     return launch_untranspose_mix(p->d_grad_vox, dev_grad, p->g, p->mix, st);
 }
@@ -552,7 +560,7 @@ extern "C" int vcbct_projector_multi_loss_grad(vcbct_projector* p, const float* 
     if (rc) return rc;
     if (dev_loss && (rc = launch_zero(dev_loss, sizeof(double), st))) return rc;
     rc = launch_adjoint_multi(p->g, p->d_tables, p->d_vox_f32, ss, xfactor, dev_targets, p->d_coef, p->d_grad_vox,
-                              dev_loss, st);
+                              dev_loss, st, p->adj_tile);
     if (rc) return rc;
     return launch_untranspose_mix(p->d_grad_vox, dev_grad, p->g, p->mix_multi, st);
 }
@@ -566,8 +574,49 @@ extern "C" int vcbct_projector_logproj_save(vcbct_projector* p, float* dev_p, fl
     if (!dev_p || !dev_L) return fail(VCBCT_EINVAL, "logproj_save: NULL argument");
     if (!(xfactor > 0)) return fail(VCBCT_EINVAL, "xfactor must be positive");
     VCBCT_HIP(hipSetDevice(p->device));
-    return launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
-                          spec_of(p), xfactor, 0, p->g.nAng, dev_p, VCBCT_MODE_LOGPROJ, dev_L, (hipStream_t)stream);
+    TileShape ts = p->tile;
+    rc = launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
+                        spec_of(p), xfactor, 0, p->g.nAng, dev_p, VCBCT_MODE_LOGPROJ, dev_L,
+                        (hipStream_t)stream, &ts);
+    if (!rc) p->tile_used = ts;
+    return rc;
+}
+
+extern "C" int vcbct_projector_set_tile(vcbct_projector* p, int row_groups, int cols, int order, int split) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    if (row_groups == 0) {
+        p->tile = TileShape{0, 0, 0, 0};
+        return 0;
+    }
+    const TileShape s{row_groups, cols, order, split};
+    if (!tile_shape_valid(s))
+        return fail(VCBCT_EINVAL, "tile shape %dx%d (order %d, split %d): row groups x columns must be 2, 4 or 8, "
+                    "order and split 0 or 1", row_groups, cols, order, split);
+    p->tile = s;
+    return 0;
+}
+
+extern "C" int vcbct_projector_set_adjoint_tile(vcbct_projector* p, int row_groups, int cols, int order, int split) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    if (row_groups == 0) {
+        p->adj_tile = TileShape{0, 0, 0, 0};
+        return 0;
+    }
+    const TileShape s{row_groups, cols, order, split};
+    if (!tile_shape_valid(s) || row_groups * cols != 4)
+        return fail(VCBCT_EINVAL, "adjoint tile shape %dx%d (order %d, split %d): row groups x columns must be 4, "
+                    "order and split 0 or 1", row_groups, cols, order, split);
+    p->adj_tile = s;
+    return 0;
+}
+
+extern "C" int vcbct_projector_get_tile(const vcbct_projector* p, int* shape4) {
+    if (!p || !shape4) return fail(VCBCT_EINVAL, "get_tile: NULL argument");
+    shape4[0] = p->tile_used.rgb;
+    shape4[1] = p->tile_used.cols;
+    shape4[2] = p->tile_used.order;
+    shape4[3] = p->tile_used.split;
+    return 0;
 }
 
 extern "C" int vcbct_projector_logproj_backward_saved(vcbct_projector* p, const float* dev_L, const float* dev_gp,
@@ -587,7 +636,8 @@ extern "C" int vcbct_projector_logproj_backward_saved(vcbct_projector* p, const 
         return fail(VCBCT_ENOMEM, "device allocation of the adjoint coefficients failed");
     rc = launch_zero(p->d_grad_vox, nv * 4 * p->mix.nch, st);
     if (rc) return rc;
-    rc = launch_adjoint_saved(p->g, p->d_tables, dev_L, spec_of(p), xfactor, dev_gp, p->d_coef, p->d_grad_vox, st);
+    rc = launch_adjoint_saved(p->g, p->d_tables, dev_L, spec_of(p), xfactor, dev_gp, p->d_coef, p->d_grad_vox, st,
+                              p->adj_tile);
     if (rc) return rc;
     return launch_untranspose_mix(p->d_grad_vox, dev_grad, p->g, p->mix, st);
 }

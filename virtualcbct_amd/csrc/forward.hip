@@ -12,6 +12,11 @@
 #include <cstdlib>
 
 #include <climits>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
 #include <cmath>
 
 #include "traverse.h"
@@ -332,24 +337,68 @@ struct CountSeg {
     __device__ __forceinline__ void step(const Seg& a, const Seg& b, const Seg& c) { n += (int)a.ok + (int)b.ok + (int)c.ok; }
 };
 
-// Work decomposition shared by the ray kernels.  A block = 4 waves = 4 adjacent detector columns of one
-// (row group, angle); row group g covers rows [64 g, 64 g + 64).  Blocks are dealt round-robin to the 8
-// XCDs; block b runs on XCD b % 8, so the column range of every (row group, angle) is split into 8
-// contiguous chunks, one per XCD: all XCDs work on the same angle (one z band of the volume in the
-// shared last-level cache) and each XCD's L2 sees adjacent columns.
+// Work decomposition of the ray kernels (TileShape).  One wave = 64 detector rows of one column at one
+// angle; row group g covers rows [64 g, 64 g + 64).  A block is rgb row groups by `cols` adjacent columns of
+// one angle, wave w taking row group w / cols and column w % cols of the block's tile.  Blocks are dealt
+// round-robin to the 8 XCDs (block b runs on XCD b % 8).  With split 0 the column groups of every (row block,
+// angle) are cut into 8 contiguous runs, one per XCD: all XCDs work on the same angle and each XCD's L2 sees
+// adjacent columns.  With split 1 the angles are cut into 8 runs instead: each XCD walks its own run of
+// angles over all columns, so the voxels that neighbouring angles share stay in that XCD's L2.  order 0 walks
+// the angles inside each row block (one z band of the volume at a time), order 1 the row blocks inside each
+// angle.
 struct Tile {
     int rg, a, col;
 };
-__device__ __forceinline__ Tile tile_of(int nA, int W, int quads_per_xcd) {
+struct TileMap {
+    int rgb, cols, order, split;
+    int nrb;  // row blocks, ceil(ceil(H / 64) / rgb)
+    int ncg;  // column groups, ceil(W / cols)
+    int gpx;  // per XCD: column groups (split 0) or angles (split 1)
+};
+__device__ __forceinline__ Tile tile_map(const TileMap& m, int nA) {
     const int b = blockIdx.x;
-    const int xcd = b & 7, local = b >> 3;
-    const int q = local % quads_per_xcd;
-    const int ra = local / quads_per_xcd;  // rg * nA + a
+    const int xcd = b & 7, local = b >> 3, w = threadIdx.x >> 6;
+    int cg, a, rb;
+    if (m.split == 0) {
+        const int q = local % m.gpx, ra = local / m.gpx;
+        cg = xcd * m.gpx + q;
+        if (m.order == 0) a = ra % nA, rb = ra / nA;
+        else rb = ra % m.nrb, a = ra / m.nrb;
+    } else {
+        cg = local % m.ncg;
+        const int ra = local / m.ncg;
+        int al;
+        if (m.order == 0) al = ra % m.gpx, rb = ra / m.gpx;
+        else rb = ra % m.nrb, al = ra / m.nrb;
+        a = xcd * m.gpx + al;
+    }
     Tile t;
-    t.a = ra % nA;
-    t.rg = ra / nA;
-    t.col = ((xcd * quads_per_xcd + q) << 2) + (threadIdx.x >> 6);
+    t.a = a;
+    t.rg = rb * m.rgb + w / m.cols;
+    t.col = cg * m.cols + w % m.cols;
     return t;
+}
+
+// Host side: the map of shape s over nA angles and its block count (-1 when the grid is too large).  Every
+// (row group, angle, column) of the launch falls to exactly one wave; waves past the last column, angle or
+// row group exit at once.
+static long tile_map_host(const ProjGeom& g, int nA, const TileShape& s, TileMap& m) {
+    m.rgb = s.rgb;
+    m.cols = s.cols;
+    m.order = s.order;
+    m.split = s.split;
+    const int nrg = (g.H + 63) / 64;
+    m.nrb = (nrg + s.rgb - 1) / s.rgb;
+    m.ncg = (g.W + s.cols - 1) / s.cols;
+    m.gpx = s.split == 0 ? (m.ncg + 7) / 8 : (nA + 7) / 8;
+    const long nb = s.split == 0 ? 8L * m.gpx * nA * m.nrb : 8L * m.gpx * m.ncg * m.nrb;
+    return nb > INT_MAX ? -1 : nb;
+}
+
+bool tile_shape_valid(const TileShape& s) {
+    const int w = s.rgb * s.cols;
+    return s.rgb >= 1 && s.cols >= 1 && (w == 2 || w == 4 || w == 8) && (s.order == 0 || s.order == 1) &&
+           (s.split == 0 || s.split == 1);
 }
 
 // One wave: 64 detector rows [64 rg, 64 rg + 64) of column jj at angle a.  Wave-uniform arguments.
@@ -436,15 +485,19 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
     }
 }
 
-template <int MODE, class VoxT>
-__global__ __launch_bounds__(256) VCBCT_WAVES8 void k_forward(ProjGeom g, const double* __restrict__ tab, const VoxT* __restrict__ vox,
-                                                 const double* __restrict__ mu, const double* __restrict__ pdf,
-                                                 const float* __restrict__ muf, const float* __restrict__ pdff, int NE,
-                                                 double inv_xf, int a0, int nA, int quads_per_xcd,
-                                                 float* __restrict__ out, float* __restrict__ aux) {
-    __shared__ WaveLds steps[4];
-    const Tile t = tile_of(nA, g.W, quads_per_xcd);
-    if (t.col >= g.W) return;  // whole wave (column) out of range: uniform exit
+// WPB waves per block (TileShape: rgb * cols); the block's waves share nothing but the LDS allocation.
+template <int MODE, class VoxT, int WPB>
+__global__ __launch_bounds__(64 * WPB) VCBCT_WAVES8 void k_forward(ProjGeom g, const double* __restrict__ tab,
+                                                                 const VoxT* __restrict__ vox,
+                                                                 const double* __restrict__ mu,
+                                                                 const double* __restrict__ pdf,
+                                                                 const float* __restrict__ muf,
+                                                                 const float* __restrict__ pdff, int NE,
+                                                                 double inv_xf, int a0, int nA, TileMap tm,
+                                                                 float* __restrict__ out, float* __restrict__ aux) {
+    __shared__ WaveLds steps[WPB];
+    const Tile t = tile_map(tm, nA);
+    if (t.col >= g.W || t.a >= nA || t.rg * 64 >= g.H) return;  // wave out of range: uniform exit
     forward_wave<MODE, VoxT>(g, tab, vox, mu, pdf, muf, pdff, NE, inv_xf, a0 + t.a, t.rg, t.col, out, aux,
                              &steps[threadIdx.x >> 6]);
 }
@@ -612,16 +665,16 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, c
                                                       const float4* __restrict__ vox, const double* __restrict__ mu,
                                                       const double* __restrict__ pdf, const float* __restrict__ muf,
                                                       const double* __restrict__ ch, int nch, int NE, double inv_xf,
-                                                      int quads_per_xcd, const float* __restrict__ tg,
+                                                      TileMap tm, const float* __restrict__ tg,
                                                       float* __restrict__ coef, double* __restrict__ loss) {
     __shared__ WaveLds step_tabs[4];
     WaveLds* steps = &step_tabs[threadIdx.x >> 6];
-    const Tile t = tile_of(g.nAng, g.W, quads_per_xcd);
+    const Tile t = tile_map(tm, g.nAng);
     const int lane = threadIdx.x & 63;
     const int j = t.rg * 64 + lane;
     const int jj = t.col;
     const int a = t.a;
-    if (jj >= g.W) return;
+    if (jj >= g.W || a >= g.nAng || t.rg * 64 >= g.H) return;  // wave out of range: uniform exit
     const bool active = j < g.H;
     const double c = ldg(tab, a), sn = ldg(tab, g.nAng + a), xc = ldg(tab, 2 * g.nAng + a),
                  yc = ldg(tab, 3 * g.nAng + a);
@@ -688,18 +741,18 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, c
 template <int NCH>  // the channel count (the arrays of the spectral loop stay in registers)
 __global__ __launch_bounds__(256) VCBCT_WAVES7 void k_adjoint_coef_multi(ProjGeom g, const double* __restrict__ tab,
                                                                          const float4* __restrict__ vox, SpecSet ss,
-                                                                         double inv_xf, int quads_per_xcd,
+                                                                         double inv_xf, TileMap tm,
                                                                          const float* __restrict__ tg,
                                                                          float* __restrict__ coef,
                                                                          double* __restrict__ loss) {
     __shared__ WaveLds step_tabs[4];
     WaveLds* steps = &step_tabs[threadIdx.x >> 6];
-    const Tile t = tile_of(g.nAng, g.W, quads_per_xcd);
+    const Tile t = tile_map(tm, g.nAng);
     const int lane = threadIdx.x & 63;
     const int j = t.rg * 64 + lane;
     const int jj = t.col;
     const int a = t.a;
-    if (jj >= g.W) return;
+    if (jj >= g.W || a >= g.nAng || t.rg * 64 >= g.H) return;  // wave out of range: uniform exit
     const bool active = j < g.H;
     const double c = ldg(tab, a), sn = ldg(tab, g.nAng + a), xc = ldg(tab, 2 * g.nAng + a),
                  yc = ldg(tab, 3 * g.nAng + a);
@@ -803,17 +856,17 @@ __global__ __launch_bounds__(256) void k_adjoint_coef_saved(ProjGeom g, const fl
     }
 }
 
-__global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_scatter(ProjGeom g, const double* __restrict__ tab, int quads_per_xcd,
-                                                         const float* __restrict__ coef, int nch,
-                                                         float* __restrict__ gvox) {
+__global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_scatter(ProjGeom g, const double* __restrict__ tab,
+                                                                     TileMap tm, const float* __restrict__ coef,
+                                                                     int nch, float* __restrict__ gvox) {
     __shared__ WaveLds step_tabs[4];
     WaveLds* steps = &step_tabs[threadIdx.x >> 6];
-    const Tile t = tile_of(g.nAng, g.W, quads_per_xcd);
+    const Tile t = tile_map(tm, g.nAng);
     const int lane = threadIdx.x & 63;
     const int j = t.rg * 64 + lane;
     const int jj = t.col;
     const int a = t.a;
-    if (jj >= g.W) return;
+    if (jj >= g.W || a >= g.nAng || t.rg * 64 >= g.H) return;  // wave out of range: uniform exit
     const bool active = j < g.H;
     ScatterG sc;
     sc.gvox = gvox;
@@ -849,15 +902,6 @@ int launch_zero(void* p, size_t bytes, hipStream_t st) {
     return 0;
 }
 
-static inline int n_blocks(const ProjGeom& g, int nA, int& qpx) {
-    const int quads = (g.W + 3) / 4;
-    qpx = (quads + 7) / 8;
-    const int rgs = (g.H + 63) / 64;
-    long nb = (long)rgs * nA * qpx * 8;
-    if (nb > INT_MAX) return -1;
-    return (int)nb;
-}
-
 int launch_transpose_u16(const uint16_t* src, uint16_t* dst, const ProjGeom& g, hipStream_t st) {
     dim3 grid((g.N + TT - 1) / TT, (g.Nz + TT - 1) / TT, g.N);
     hipLaunchKernelGGL(k_transpose_u16, grid, dim3(256), 0, st, src, (uint4*)dst, g.N, g.Nz, g.nmat);
@@ -883,60 +927,211 @@ int launch_untranspose_f32(const float* src, float* dst, const ProjGeom& g, hipS
     return 0;
 }
 
-int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int vox_is_f32, const Spectrum& sp,
-                   double xfactor, int a0, int a1, float* out, int mode, float* aux, hipStream_t st) {
-    const int nA = a1 - a0;
-    if (nA <= 0) return 0;
-    int qpx;
-    int nb = n_blocks(g, nA, qpx);
-    if (nb < 0) return fail(VCBCT_EINVAL, "forward: launch too large");
-    const double inv_xf = 1.0 / xfactor;
-#define VCBCT_LAUNCH_FWD(M, T)                                                                              \
-    hipLaunchKernelGGL((k_forward<M, T>), dim3(nb), dim3(256), 0, st, g, tables, (const T*)vox, sp.mu, sp.pdf, \
-                       sp.muf, sp.pdff, sp.NE, inv_xf, a0, nA, qpx, out, aux)
-    if (!vox_is_f32) {
-        if (mode == VCBCT_MODE_INTENSITY) VCBCT_LAUNCH_FWD(VCBCT_MODE_INTENSITY, uint4);
-        else if (mode == VCBCT_MODE_LINEINT) VCBCT_LAUNCH_FWD(VCBCT_MODE_LINEINT, uint4);
-        else if (mode == VCBCT_MODE_SEGCOUNT) VCBCT_LAUNCH_FWD(VCBCT_MODE_SEGCOUNT, uint4);
-        else VCBCT_LAUNCH_FWD(VCBCT_MODE_LOGPROJ, uint4);
+// One forward launch of shape s (valid, and WPB 4 for the line-integral and segment-count modes).
+template <int M, class T>
+static void launch_fwd_t(int wpb, dim3 grid, hipStream_t st, const ProjGeom& g, const double* tables, const void* vox,
+                         const Spectrum& sp, double inv_xf, int a0, int nA, const TileMap& tm, float* out,
+                         float* aux) {
+#define VCBCT_LAUNCH_FWD(W)                                                                                  \
+    hipLaunchKernelGGL((k_forward<M, T, W>), grid, dim3(64 * W), 0, st, g, tables, (const T*)vox, sp.mu, sp.pdf, \
+                       sp.muf, sp.pdff, sp.NE, inv_xf, a0, nA, tm, out, aux)
+    if constexpr (M == VCBCT_MODE_INTENSITY || M == VCBCT_MODE_LOGPROJ) {
+        if (wpb == 2) VCBCT_LAUNCH_FWD(2);
+        else if (wpb == 8) VCBCT_LAUNCH_FWD(8);
+        else VCBCT_LAUNCH_FWD(4);
     } else {
-        if (mode == VCBCT_MODE_INTENSITY) VCBCT_LAUNCH_FWD(VCBCT_MODE_INTENSITY, float4);
-        else if (mode == VCBCT_MODE_LINEINT) VCBCT_LAUNCH_FWD(VCBCT_MODE_LINEINT, float4);
-        else VCBCT_LAUNCH_FWD(VCBCT_MODE_LOGPROJ, float4);
+        VCBCT_LAUNCH_FWD(4);
     }
 #undef VCBCT_LAUNCH_FWD
+}
+
+static int launch_fwd_shape(const ProjGeom& g, const double* tables, const void* vox, int vox_is_f32,
+                            const Spectrum& sp, double inv_xf, int a0, int nA, float* out, int mode, float* aux,
+                            hipStream_t st, const TileShape& s) {
+    TileMap tm;
+    const long nb = tile_map_host(g, nA, s, tm);
+    if (nb < 0) return fail(VCBCT_EINVAL, "forward: launch too large");
+    const int wpb = s.rgb * s.cols;
+    const dim3 grid((unsigned)nb);
+    if (!vox_is_f32) {
+        if (mode == VCBCT_MODE_INTENSITY)
+            launch_fwd_t<VCBCT_MODE_INTENSITY, uint4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+        else if (mode == VCBCT_MODE_LINEINT)
+            launch_fwd_t<VCBCT_MODE_LINEINT, uint4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+        else if (mode == VCBCT_MODE_SEGCOUNT)
+            launch_fwd_t<VCBCT_MODE_SEGCOUNT, uint4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+        else
+            launch_fwd_t<VCBCT_MODE_LOGPROJ, uint4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+    } else {
+        if (mode == VCBCT_MODE_INTENSITY)
+            launch_fwd_t<VCBCT_MODE_INTENSITY, float4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out,
+                                                       aux);
+        else if (mode == VCBCT_MODE_LINEINT)
+            launch_fwd_t<VCBCT_MODE_LINEINT, float4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+        else
+            launch_fwd_t<VCBCT_MODE_LOGPROJ, float4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+    }
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
 
-int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double xfactor,
-                   const float* target_or_gp, int mode, float* coef, float* grad_vox, double* loss, hipStream_t st) {
-    int qpx;
-    int nb = n_blocks(g, g.nAng, qpx);
-    if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
+// ------------------------------------------------------------------------------------------------
+// Tile-shape autotuner.  Per key (kernel, geometry, launch parameters, device) the shape with the shortest
+// launch, timed on the caller's stream and buffers by coordinate descent: the (order, split) pairs at the
+// default block shape, then the other block shapes at the best pair, then the pairs again at the best block
+// shape.  Every candidate computes the same values (a shape is a schedule only); a kernel that accumulates is
+// timed without its accumulator (the loss) or has it cleared afterwards (the gradient).  Nothing is timed
+// under stream capture (the default shape is recorded, not cached) or with VCBCT_NO_AUTOTUNE=1.  It replaces
+// hand-timed launch shapes (P/reprojectionOnGPU.cu:900-983 fixes them per kernel).
+static std::mutex g_ftune_mu;
+static std::unordered_map<std::string, TileShape> g_ftune;
+static std::string g_ftune_log;
+
+std::string forward_tune_log() {
+    std::lock_guard<std::mutex> lk(g_ftune_mu);
+    return g_ftune_log;
+}
+
+// Block shapes (row groups x columns) of 2, 4 and 8 waves (the forward kernel), and of 4 waves (the adjoint
+// kernels, built for 256 lanes).
+static const int kBlocksAll[8][2] = {{1, 4}, {1, 2}, {2, 1}, {2, 2}, {4, 1}, {1, 8}, {2, 4}, {4, 2}};
+static const int kBlocks4[3][2] = {{1, 4}, {2, 2}, {4, 1}};
+
+// 1: `key` is tuned (its shape in s); 0: tune it now; -1: do not tune (s = the default shape).
+static int tuned_shape(const std::string& key, hipStream_t st, TileShape& s) {
+    {
+        std::lock_guard<std::mutex> lk(g_ftune_mu);
+        auto it = g_ftune.find(key);
+        if (it != g_ftune.end()) {
+            s = it->second;
+            return 1;
+        }
+    }
+    s = kDefaultTile;
+    const char* off = getenv("VCBCT_NO_AUTOTUNE");
+    if (off && *off == '1') return -1;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return -1;
+    return 0;
+}
+
+template <class F>
+static int tune_tiles(const std::string& key, hipStream_t st, const int (*blocks)[2], int nblocks, F&& launch,
+                      TileShape& best) {
+    hipEvent_t e0, e1;
+    VCBCT_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(VCBCT_EHIP, "autotune: event creation failed");
+    }
+    int rc = launch(kDefaultTile);  // warm-up
+    std::string rep = key + ":";
+    std::vector<TileShape> done;
+    float best_ms = 1e30f;
+    best = kDefaultTile;
+    auto time_shape = [&](const TileShape& s) {
+        if (rc) return;
+        for (const TileShape& d : done)
+            if (d.rgb == s.rgb && d.cols == s.cols && d.order == s.order && d.split == s.split) return;
+        done.push_back(s);
+        float ms = 0.0f;
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < 2 && !rc; ++r) rc = launch(s);
+        (void)hipEventRecord(e1, st);
+        if (rc) return;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(VCBCT_EHIP, "autotune: timing failed");
+            return;
+        }
+        char b[64];
+        snprintf(b, sizeof b, " %dx%d/o%d/s%d=%.3fms", s.rgb, s.cols, s.order, s.split, ms / 2);
+        rep += b;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = s;
+        }
+    };
+    for (int o = 0; o < 2; ++o)
+        for (int sp = 0; sp < 2; ++sp) time_shape(TileShape{kDefaultTile.rgb, kDefaultTile.cols, o, sp});
+    const TileShape p = best;
+    for (int i = 0; i < nblocks; ++i) time_shape(TileShape{blocks[i][0], blocks[i][1], p.order, p.split});
+    const TileShape q = best;
+    for (int o = 0; o < 2; ++o)
+        for (int sp = 0; sp < 2; ++sp) time_shape(TileShape{q.rgb, q.cols, o, sp});
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    char b[64];
+    snprintf(b, sizeof b, " -> %dx%d/o%d/s%d\n", best.rgb, best.cols, best.order, best.split);
+    std::lock_guard<std::mutex> lk(g_ftune_mu);
+    g_ftune[key] = best;
+    g_ftune_log += rep + b;
+    return 0;
+}
+
+static std::string tune_key(const char* kernel, const ProjGeom& g, int nA, const char* extra) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    char key[224];
+    snprintf(key, sizeof key, "%s dev%d N%d Nz%d H%d W%d nA%d nmat%d %s", kernel, dev, g.N, g.Nz, g.H, g.W, nA,
+             g.nmat, extra);
+    return key;
+}
+
+int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int vox_is_f32, const Spectrum& sp,
+                   double xfactor, int a0, int a1, float* out, int mode, float* aux, hipStream_t st,
+                   TileShape* tile) {
+    const int nA = a1 - a0;
+    if (nA <= 0) return 0;
     const double inv_xf = 1.0 / xfactor;
+    const bool tunable = mode == VCBCT_MODE_INTENSITY || mode == VCBCT_MODE_LOGPROJ;
+    TileShape s = kDefaultTile;
+    if (tunable && tile && tile->rgb > 0) {
+        if (!tile_shape_valid(*tile)) return fail(VCBCT_EINVAL, "forward: invalid tile shape");
+        s = *tile;
+    } else if (tunable) {
+        char extra[32];
+        snprintf(extra, sizeof extra, "mode%d %s", mode, vox_is_f32 ? "f32" : "u16");
+        const std::string key = tune_key("fwd", g, nA, extra);
+        if (tuned_shape(key, st, s) == 0) {
+            auto launch = [&](const TileShape& t) {
+                return launch_fwd_shape(g, tables, vox, vox_is_f32, sp, inv_xf, a0, nA, out, mode, aux, st, t);
+            };
+            int rc = tune_tiles(key, st, kBlocksAll, 8, launch, s);
+            if (rc) return rc;
+        }
+    }
+    if (tile) *tile = s;
+    return launch_fwd_shape(g, tables, vox, vox_is_f32, sp, inv_xf, a0, nA, out, mode, aux, st, s);
+}
+
+// The adjoint's kernels, one launch of shape s each (4-wave shapes).
+static int launch_coef(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double inv_xf,
+                       const float* tg, int mode, float* coef, double* loss, hipStream_t st, const TileShape& s) {
+    TileMap tm;
+    const long nb = tile_map_host(g, g.nAng, s, tm);
+    if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
     if (mode == VCBCT_ADJ_LOSS)
-        hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_LOSS>, dim3(nb), dim3(256), 0, st, g, tables, (const float4*)vox,
-                           sp.mu, sp.pdf, sp.muf, sp.ch, sp.nch, sp.NE, inv_xf, qpx, target_or_gp, coef, loss);
+        hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_LOSS>, dim3((unsigned)nb), dim3(256), 0, st, g, tables,
+                           (const float4*)vox, sp.mu, sp.pdf, sp.muf, sp.ch, sp.nch, sp.NE, inv_xf, tm, tg, coef,
+                           loss);
     else
-        hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_UPSTREAM>, dim3(nb), dim3(256), 0, st, g, tables,
-                           (const float4*)vox, sp.mu, sp.pdf, sp.muf, sp.ch, sp.nch, sp.NE, inv_xf, qpx, target_or_gp,
-                           coef, loss);
-    VCBCT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, sp.nch, grad_vox);
+        hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_UPSTREAM>, dim3((unsigned)nb), dim3(256), 0, st, g, tables,
+                           (const float4*)vox, sp.mu, sp.pdf, sp.muf, sp.ch, sp.nch, sp.NE, inv_xf, tm, tg, coef,
+                           loss);
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
 
-int launch_adjoint_multi(const ProjGeom& g, const double* tables, const float* vox, const SpecSet& ss,
-                         double xfactor, const float* targets, float* coef, float* grad_vox, double* loss,
-                         hipStream_t st) {
-    int qpx;
-    int nb = n_blocks(g, g.nAng, qpx);
+static int launch_coef_multi(const ProjGeom& g, const double* tables, const float* vox, const SpecSet& ss,
+                             double inv_xf, const float* targets, float* coef, double* loss, hipStream_t st,
+                             const TileShape& s) {
+    TileMap tm;
+    const long nb = tile_map_host(g, g.nAng, s, tm);
     if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
 #define VCBCT_LAUNCH_MULTI(C)                                                                                   \
-    hipLaunchKernelGGL(k_adjoint_coef_multi<C>, dim3(nb), dim3(256), 0, st, g, tables, (const float4*)vox, ss, \
-                       1.0 / xfactor, qpx, targets, coef, loss)
+    hipLaunchKernelGGL(k_adjoint_coef_multi<C>, dim3((unsigned)nb), dim3(256), 0, st, g, tables,               \
+                       (const float4*)vox, ss, inv_xf, tm, targets, coef, loss)
     switch (ss.nch) {
         case 1: VCBCT_LAUNCH_MULTI(1); break;
         case 2: VCBCT_LAUNCH_MULTI(2); break;
@@ -948,24 +1143,100 @@ int launch_adjoint_multi(const ProjGeom& g, const double* tables, const float* v
     }
 #undef VCBCT_LAUNCH_MULTI
     VCBCT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, ss.nch, grad_vox);
+    return 0;
+}
+
+static int launch_scatter(const ProjGeom& g, const double* tables, const float* coef, int nch, float* grad_vox,
+                          hipStream_t st, const TileShape& s) {
+    TileMap tm;
+    const long nb = tile_map_host(g, g.nAng, s, tm);
+    if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
+    hipLaunchKernelGGL(k_adjoint_scatter, dim3((unsigned)nb), dim3(256), 0, st, g, tables, tm, coef, nch, grad_vox);
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
 
+// The shape of an adjoint kernel: the caller's (forced.rgb > 0, 4 waves), else the tuned one for `key`, tuned
+// now with `launch` if there is none.
+template <class F>
+static int adjoint_shape(const TileShape& forced, const std::string& key, hipStream_t st, F&& launch, TileShape& s,
+                         bool& tuned_now) {
+    tuned_now = false;
+    if (forced.rgb > 0) {
+        if (!tile_shape_valid(forced) || forced.rgb * forced.cols != 4)
+            return fail(VCBCT_EINVAL, "adjoint: tile shape must have 4 waves");
+        s = forced;
+        return 0;
+    }
+    if (tuned_shape(key, st, s) != 0) return 0;
+    tuned_now = true;
+    return tune_tiles(key, st, kBlocks4, 3, launch, s);
+}
+
+// The scatter's shape, tuned on the caller's coefficients: the timing launches add into grad_vox, which is
+// cleared again afterwards (nch planes of N * N * Nz).
+static int scatter_shape(const ProjGeom& g, const double* tables, const float* coef, int nch, float* grad_vox,
+                         hipStream_t st, const TileShape& forced, TileShape& s) {
+    char extra[32];
+    snprintf(extra, sizeof extra, "nch%d", nch);
+    bool tuned_now = false;
+    auto launch = [&](const TileShape& t) { return launch_scatter(g, tables, coef, nch, grad_vox, st, t); };
+    int rc = adjoint_shape(forced, tune_key("adj-scatter", g, g.nAng, extra), st, launch, s, tuned_now);
+    if (rc || !tuned_now) return rc;
+    return launch_zero(grad_vox, (size_t)nch * g.N * g.N * g.Nz * sizeof(float), st);
+}
+
+int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double xfactor,
+                   const float* target_or_gp, int mode, float* coef, float* grad_vox, double* loss, hipStream_t st,
+                   const TileShape& forced) {
+    const double inv_xf = 1.0 / xfactor;
+    char extra[48];
+    snprintf(extra, sizeof extra, "mode%d nch%d NE%d", mode, sp.nch, sp.NE);
+    TileShape sc, ss;
+    bool tuned_now = false;
+    // tuning launches pass no loss accumulator
+    auto launch = [&](const TileShape& t) {
+        return launch_coef(g, tables, vox, sp, inv_xf, target_or_gp, mode, coef, nullptr, st, t);
+    };
+    int rc = adjoint_shape(forced, tune_key("adj-coef", g, g.nAng, extra), st, launch, sc, tuned_now);
+    if (!rc) rc = launch_coef(g, tables, vox, sp, inv_xf, target_or_gp, mode, coef, loss, st, sc);
+    if (!rc) rc = scatter_shape(g, tables, coef, sp.nch, grad_vox, st, forced, ss);
+    if (!rc) rc = launch_scatter(g, tables, coef, sp.nch, grad_vox, st, ss);
+    return rc;
+}
+
+int launch_adjoint_multi(const ProjGeom& g, const double* tables, const float* vox, const SpecSet& ss,
+                         double xfactor, const float* targets, float* coef, float* grad_vox, double* loss,
+                         hipStream_t st, const TileShape& forced) {
+    const double inv_xf = 1.0 / xfactor;
+    char extra[48];
+    int ne = 0;
+    for (int s = 0; s < ss.nspec; ++s) ne += ss.NE[s];
+    snprintf(extra, sizeof extra, "nspec%d nch%d NE%d", ss.nspec, ss.nch, ne);
+    TileShape sc, sg;
+    bool tuned_now = false;
+    auto launch = [&](const TileShape& t) {
+        return launch_coef_multi(g, tables, vox, ss, inv_xf, targets, coef, nullptr, st, t);
+    };
+    int rc = adjoint_shape(forced, tune_key("adj-multi", g, g.nAng, extra), st, launch, sc, tuned_now);
+    if (!rc) rc = launch_coef_multi(g, tables, vox, ss, inv_xf, targets, coef, loss, st, sc);
+    if (!rc) rc = scatter_shape(g, tables, coef, ss.nch, grad_vox, st, forced, sg);
+    if (!rc) rc = launch_scatter(g, tables, coef, ss.nch, grad_vox, st, sg);
+    return rc;
+}
+
 int launch_adjoint_saved(const ProjGeom& g, const double* tables, const float* Ls, const Spectrum& sp,
-                         double xfactor, const float* gp, float* coef, float* grad_vox, hipStream_t st) {
-    int qpx;
-    int nb = n_blocks(g, g.nAng, qpx);
-    if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
+                         double xfactor, const float* gp, float* coef, float* grad_vox, hipStream_t st,
+                         const TileShape& forced) {
     const size_t nrays = (size_t)g.nAng * g.H * g.W;
     const unsigned nbc = (unsigned)std::min<size_t>((nrays + 255) / 256, 65536);
     hipLaunchKernelGGL(k_adjoint_coef_saved, dim3(nbc), dim3(256), 0, st, g, Ls, sp.mu, sp.pdf, sp.muf, sp.ch, sp.nch,
                        sp.NE, 1.0 / xfactor, gp, coef);
     VCBCT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_adjoint_scatter, dim3(nb), dim3(256), 0, st, g, tables, qpx, coef, sp.nch, grad_vox);
-    VCBCT_HIP(hipGetLastError());
-    return 0;
+    TileShape sg;
+    int rc = scatter_shape(g, tables, coef, sp.nch, grad_vox, st, forced, sg);
+    if (!rc) rc = launch_scatter(g, tables, coef, sp.nch, grad_vox, st, sg);
+    return rc;
 }
 
 }  // namespace vcbct

@@ -140,17 +140,36 @@ int launch_transpose_f32(const float* src, float* dst, const ProjGeom& g, hipStr
 int launch_untranspose_f32(const float* src, float* dst, const ProjGeom& g, hipStream_t st);
 // The adjoint's channel planes [nch][y][x][z] -> the caller's maps [nmat][z][y][x] through the mix.
 int launch_untranspose_mix(const float* src, float* dst, const ProjGeom& g, const ChannelMix& mix, hipStream_t st);
+// Tile shape of the forward kernel: a block is rgb row groups (64 detector rows each) by `cols` adjacent
+// columns of one angle, one wave per (row group, column), rgb * cols in {2, 4, 8}; blocks walk angles inside
+// row blocks (order 0) or row blocks inside angles (order 1); the 8 XCDs divide the columns (split 0) or the
+// angles (split 1) among them.  rgb == 0: not set (the autotuner picks one per geometry).
+struct TileShape {
+    int rgb, cols, order, split;
+};
+// The shape the forward launches use when nothing else is known: 1 x 4, angles inner, columns split.
+constexpr TileShape kDefaultTile = {1, 4, 0, 0};
+bool tile_shape_valid(const TileShape& s);
+// Forward launch over angles [a0, a1).  tile: in, the caller's fixed shape (rgb > 0) or none (rgb == 0: the
+// autotuner's choice for this geometry, timed on the first launch outside stream capture); out, the shape used.
 int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int vox_is_f32, const Spectrum& sp,
-                   double xfactor, int a0, int a1, float* out, int mode, float* aux, hipStream_t st);
+                   double xfactor, int a0, int a1, float* out, int mode, float* aux, hipStream_t st,
+                   TileShape* tile = nullptr);
+// The forward autotuner's record (one line per tuned geometry).
+std::string forward_tune_log();
+// The adjoint launches take the caller's 4-wave tile shape for their traversal kernels (forced.rgb > 0), or
+// autotune them per geometry (forced.rgb == 0) as launch_forward does.
 int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double xfactor,
-                   const float* target_or_gp, int mode, float* coef, float* grad_vox, double* loss, hipStream_t st);
+                   const float* target_or_gp, int mode, float* coef, float* grad_vox, double* loss, hipStream_t st,
+                   const TileShape& forced);
 int launch_adjoint_saved(const ProjGeom& g, const double* tables, const float* Ls, const Spectrum& sp,
-                         double xfactor, const float* gp, float* coef, float* grad_vox, hipStream_t st);
+                         double xfactor, const float* gp, float* coef, float* grad_vox, hipStream_t st,
+                         const TileShape& forced);
 // loss = sum_s sum_rays (p_s - t_s)^2 over the spectra of `ss` and its gradient: one traversal per ray for all
 // spectra (their line integrals are the same), the coefficients summed over the spectra, one scatter.
 int launch_adjoint_multi(const ProjGeom& g, const double* tables, const float* vox, const SpecSet& ss,
                          double xfactor, const float* targets, float* coef, float* grad_vox, double* loss,
-                         hipStream_t st);
+                         hipStream_t st, const TileShape& forced);
 
 // a / b for a divisor b known in advance, with rb = 1 / b correctly rounded: the product a * rb is within an
 // ulp of a / b, and one correction with the exact remainder a - q b (an FMA) rounds it correctly (Markstein's
@@ -220,4 +239,7 @@ struct vcbct_projector {
     } ms[VCBCT_MAX_SPECTRA];
     int nspec_multi;
     vcbct::ChannelMix mix_multi;
+    vcbct::TileShape tile;       // fixed by vcbct_projector_set_tile (rgb == 0: autotuned)
+    vcbct::TileShape tile_used;  // the shape of the last intensity / log-projection launch
+    vcbct::TileShape adj_tile;   // the adjoint kernels' shape, fixed by vcbct_projector_set_adjoint_tile (4 waves)
 };

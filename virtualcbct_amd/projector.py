@@ -217,6 +217,24 @@ class Projector:
         check(lib().vcbct_projector_segment_counts(self.h, C.c_void_p(c.data_ptr()), _stream(stream)))
         return c
 
+    def set_tile(self, row_groups=0, cols=0, order=0, split=0):
+        """Fixes the tile shape of the intensity / log-projection launches (row_groups x cols waves per
+        workgroup, 2, 4 or 8; order 0: angles inside row blocks, 1: row blocks inside angles; split 0: columns,
+        1: angles divided among the XCDs).  row_groups=0 hands it back to the autotuner."""
+        check(lib().vcbct_projector_set_tile(self.h, int(row_groups), int(cols), int(order), int(split)))
+
+    def set_adjoint_tile(self, row_groups=0, cols=0, order=0, split=0):
+        """Fixes the tile shape of the adjoint's traversal kernels (row_groups x cols = 4 waves per workgroup);
+        row_groups=0 hands it back to the autotuner."""
+        check(lib().vcbct_projector_set_adjoint_tile(self.h, int(row_groups), int(cols), int(order), int(split)))
+
+    @property
+    def tile(self):
+        """(row_groups, cols, order, split) of the last intensity / log-projection launch."""
+        a = (C.c_int * 4)()
+        check(lib().vcbct_projector_get_tile(self.h, a))
+        return tuple(a)
+
     def logproj(self, xfactor, out=None, stream=None):
         torch = _torch()
         if out is None:
