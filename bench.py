@@ -383,7 +383,8 @@ def bench_batched(line, inp, dev, stream, e, NE):
                        "plain_rays_bins_per_s": rays3 * NE / (min(plain_ms) * 1e-3),
                        "batched_rays_bins_per_s": rays3 * NE / (min(batch_ms) * 1e-3),
                        "speedup": float(min(plain_ms) / min(batch_ms)), "identical_to_plain": same,
-                       "batch_grid": ("flat" if B.blocks == 0 else f"persistent {B.blocks} blocks"), "autotune": autotune_log().strip().split("\n")}
+                       "batch_grid": ("flat" if B.blocks == 0 else "sequence of tuned per-phantom launches" if B.blocks == -2
+                                     else f"persistent {B.blocks} blocks"), "autotune": autotune_log().strip().split("\n")}
     log(f"batched: plain {min(plain_ms):.2f} ms, batched {min(batch_ms):.2f} ms, graph {min(graph_ms):.2f} ms")
     B.close()
     for Pi in projs:

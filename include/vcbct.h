@@ -388,12 +388,23 @@ int vcbct_batch_add_fdk(vcbct_batch* b, vcbct_fdk_plan* plan, const float* dev_p
 int vcbct_batch_run(vcbct_batch* b, void* stream);
 int vcbct_batch_capture_graph(vcbct_batch* b);
 int vcbct_batch_launch_graph(vcbct_batch* b, void* stream);
-/* Grid the autotuner chose for the batch's forward problems: the persistent grid size in blocks, or 0 for
- * the flat grid (every problem's blocks laid end to end, no work queue); -1 for NULL. */
+/* Grid the autotuner chose for the batch's forward problems: the persistent grid size in blocks, 0 for the
+ * flat grid (every problem's blocks laid end to end, no work queue), -2 for the sequence form (one plain
+ * forward launch per problem, each with its geometry's autotuned tile shape; replayed together from the
+ * captured graph); -1 for NULL. */
 int vcbct_batch_blocks(const vcbct_batch* b);
 /* Fixes the forward grid instead of autotuning it: 0 = flat grid, n > 0 = persistent grid of n blocks
- * (rounded up to a multiple of 8), -1 = autotune (default).  No reference counterpart (tuning control). */
+ * (rounded up to a multiple of 8), -2 = sequence form, -1 = autotune (default).  No reference counterpart
+ * (tuning control). */
 int vcbct_batch_set_grid(vcbct_batch* b, int blocks);
+/* Fixes the tile shape instead of autotuning it: the flat grid's workgroup of row_groups x cols = 4 waves
+ * and its order, and for either grid the split (0: columns, 1: angles divided among the 8 XCDs), as
+ * vcbct_projector_set_tile describes them; row_groups = 0: autotune (default; the tuner searches grid and
+ * split together, then the flat grid's block shape and order).  No reference counterpart (tuning control). */
+int vcbct_batch_set_tile(vcbct_batch* b, int row_groups, int cols, int order, int split);
+/* The shape of the flat or persistent grid in use after the first run: shape4 = {row_groups, cols, order,
+ * split} (the sequence form's launches each use their geometry's autotuned shape instead). */
+int vcbct_batch_get_tile(const vcbct_batch* b, int* shape4);
 /* Text log of the autotuner's decisions (candidates and timings); returns its length. */
 int vcbct_autotune_log(char* buf, int capacity);
 

@@ -117,8 +117,9 @@ def test_fdk_plan_equals_one_shot(torch):
 
 def test_every_grid_form_equals_plain_mixed_voxel_types(torch):
     """u16 volumes and FP32 maps in one batch; the flat grid and persistent grids (including one far smaller
-    than the work, so that waves drain their own XCD queue and steal from the others) all reproduce the
-    plain per-problem launches bit for bit."""
+    than the work, so that waves drain their own XCD queue and steal from the others), with columns or angles
+    divided among the XCDs, and the flat grid's 4-wave block shapes and orders, all reproduce the plain
+    per-problem launches bit for bit (the problems' angle counts, 20, 33 and 12, are not multiples of 8)."""
     from virtualcbct_amd.batch import Batch
     probs = problems(torch)
     # a fourth problem with FP32 maps (the differentiable path's voxel type)
@@ -130,9 +131,14 @@ def test_every_grid_form_equals_plain_mixed_voxel_types(torch):
     Pf.set_spectrum(atten0, pdf0)
     allp = [p[0] for p in probs[:2]] + [Pf] + [probs[2][0]]  # interleaved types: the batch groups them
     plain = [P.forward(1000.0).clone() for P in allp]
-    for grid in (0, 24, 512, 1536):
+    cases = [(grid, (1, 4, 0, split)) for grid in (0, 24, 512, 1536) for split in (0, 1)]
+    cases += [(0, shape) for shape in ((2, 2, 0, 0), (2, 2, 1, 1), (4, 1, 0, 1), (4, 1, 1, 0), (1, 4, 1, 1))]
+    cases += [(-2, (1, 4, 0, 0))]  # the sequence of plain launches
+    for grid, shape in cases:
+        print("grid", grid, "tile", shape, flush=True)
         b = Batch()
         b.set_grid(grid)
+        b.set_tile(*shape)
         outs = []
         for P in allp:
             o = torch.full(P.shape_proj, -1.0, device="cuda")
@@ -140,10 +146,15 @@ def test_every_grid_form_equals_plain_mixed_voxel_types(torch):
             outs.append(o)
         b.run()
         torch.cuda.synchronize()
-        assert b.blocks == grid
+        assert b.blocks == grid and b.tile == shape
         for i, (o, p) in enumerate(zip(outs, plain)):
-            assert torch.equal(o, p), (grid, i)
+            assert torch.equal(o, p), (grid, shape, i)
         b.close()
+    b = Batch()
+    for bad in ((1, 2, 0, 0), (2, 4, 0, 0), (1, 4, 0, 2)):
+        with pytest.raises(Exception, match="tile shape"):
+            b.set_tile(*bad)
+    b.close()
     Pf.close()
 
 

@@ -129,6 +129,8 @@ SIGNATURES = {
     "vcbct_batch_launch_graph": (c_int, [c_void_p, c_void_p]),
     "vcbct_batch_blocks": (c_int, [c_void_p]),
     "vcbct_batch_set_grid": (c_int, [c_void_p, c_int]),
+    "vcbct_batch_set_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    "vcbct_batch_get_tile": (c_int, [c_void_p, C.POINTER(c_int)]),
     "vcbct_autotune_log": (c_int, [C.c_char_p, c_int]),
     "vcbct_read_proj_config": (c_int, [c_char_p, C.POINTER(vcbct_proj_config)]),
     "vcbct_read_flexmap": (c_int, [c_char_p, P_D, P_D, P_D, c_int]),

@@ -46,12 +46,27 @@ class Batch:
 
     @property
     def blocks(self):
-        """Grid in use: 0 = flat, n > 0 = persistent with n blocks (after the first run)."""
+        """Grid in use: 0 = flat, n > 0 = persistent with n blocks, -2 = sequence of plain per-problem launches
+        (after the first run)."""
         return lib().vcbct_batch_blocks(self.h)
 
     def set_grid(self, blocks):
-        """Fix the grid instead of autotuning: 0 = flat, n > 0 = persistent with n blocks, -1 = autotune."""
+        """Fix the grid instead of autotuning: 0 = flat, n > 0 = persistent with n blocks, -2 = sequence of plain
+        per-problem launches (each with its geometry's tuned tile shape), -1 = autotune."""
         check(lib().vcbct_batch_set_grid(self.h, int(blocks)))
+
+    def set_tile(self, row_groups=0, cols=0, order=0, split=0):
+        """Fix the tile shape instead of autotuning it: the flat grid's 4-wave workgroup (row_groups x cols = 4)
+        and order, and for both grids the split (0: columns, 1: angles divided among the XCDs); row_groups=0
+        hands it back to the autotuner."""
+        check(lib().vcbct_batch_set_tile(self.h, int(row_groups), int(cols), int(order), int(split)))
+
+    @property
+    def tile(self):
+        """(row_groups, cols, order, split) in use (after the first run)."""
+        a = (C.c_int * 4)()
+        check(lib().vcbct_batch_get_tile(self.h, a))
+        return tuple(a)
 
     def close(self):
         if getattr(self, "h", None):

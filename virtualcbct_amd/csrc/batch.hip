@@ -18,11 +18,22 @@
 namespace vcbct {
 const char* tune_log();
 static std::mutex g_btune_mu;
-static std::unordered_map<std::string, int> g_batch_blocks;
+struct BatchChoice {
+    int nblocks;  // 0: flat grid
+    TileShape shape;
+};
+static std::unordered_map<std::string, BatchChoice> g_batch_choice;
 static std::string g_btune_log;
 }  // namespace vcbct
 
 using namespace vcbct;
+
+// The grid value of the sequence form: one plain forward launch per problem (each with the tile shape the
+// forward autotuner picked for its geometry), in submission order; in a captured graph they replay together.
+static constexpr int kSequence = -2;
+
+// The 4-wave block shapes (row groups x columns) of the flat grid's kernel.
+static const int kBlockShapes4[3][2] = {{1, 4}, {2, 2}, {4, 1}};
 
 struct vcbct_batch {
     int device;
@@ -34,6 +45,8 @@ struct vcbct_batch {
     int cap;
     int nblocks;      // persistent grid size, or 0 for the flat grid
     int forced;       // grid fixed by the caller (-1: autotune)
+    TileShape tile_forced;  // shape fixed by the caller (rgb == 0: autotune)
+    TileShape tile;   // the shape in use (after finalize)
     int flat_blocks;  // running total of flat-grid blocks (range check while problems are added)
     long items_total;  // running total of persistent work items (same check)
     int n_u16;        // after finalize: problems [0, n_u16) have u16 volumes, the rest FP32 maps
@@ -71,6 +84,8 @@ extern "C" int vcbct_batch_create(vcbct_batch** out, int device) {
     b->cap = 0;
     b->nblocks = 0;
     b->forced = -1;
+    b->tile_forced = TileShape{0, 0, 0, 0};
+    b->tile = kDefaultTile;
     b->flat_blocks = 0;
     b->items_total = 0;
     b->n_u16 = 0;
@@ -119,12 +134,19 @@ extern "C" int vcbct_batch_add_forward(vcbct_batch* b, vcbct_projector* p, float
     P.NE = p->NE;
     P.out = dev_out;
     P.inv_xf = 1.0 / xfactor;
-    P.cols_per_xcd = (p->g.W + 7) / 8;
-    P.quads_per_xcd = ((p->g.W + 3) / 4 + 7) / 8;
-    // block and work-item offsets are assigned at finalize (problems grouped by voxel type); here only the
-    // totals are checked against the 32-bit grid and queue ranges
-    const long blocks = (long)((p->g.H + 63) / 64) * p->g.nAng * P.quads_per_xcd * 8;
-    const long items = (long)((p->g.H + 63) / 64) * p->g.nAng * P.cols_per_xcd;
+    P.xfactor = xfactor;
+    // block and work-item offsets are assigned at finalize (problems grouped by voxel type, laid out for the
+    // chosen shape); here only the totals are checked against the 32-bit grid and queue ranges, for the largest
+    // layout any 4-wave shape gives
+    long blocks = 0;
+    for (const int* bs : {kBlockShapes4[0], kBlockShapes4[1], kBlockShapes4[2]})
+        for (int sp = 0; sp < 2; ++sp) {
+            TileMap tm;
+            const long nb = tile_map_host(p->g, p->g.nAng, TileShape{bs[0], bs[1], 0, sp}, tm);
+            blocks = nb < 0 ? 0x7fffffffL : std::max(blocks, nb);
+        }
+    const long nrg = (p->g.H + 63) / 64;
+    const long items = nrg * std::max((long)p->g.nAng * ((p->g.W + 7) / 8), (long)((p->g.nAng + 7) / 8) * p->g.W);
     if ((long)b->flat_blocks + blocks > 0x7fffffffL || b->items_total + items > 0x7fffffffL - 1024)
         return fail(VCBCT_EINVAL, "batch too large");
     b->flat_blocks += (int)blocks;
@@ -148,6 +170,15 @@ extern "C" int vcbct_batch_add_fdk(vcbct_batch* b, vcbct_fdk_plan* plan, const f
 
 static int batch_forward(vcbct_batch* b, int nblocks, hipStream_t st) {
     if (b->probs.empty()) return 0;
+    if (nblocks == kSequence) {  // one plain launch per problem, each with its geometry's tuned tile shape
+        for (const BatchProblem& P : b->probs) {
+            const Spectrum sp{P.mu, P.pdf, P.muf, P.pdff, P.NE, nullptr, 0};
+            const int rc = launch_forward(P.g, P.tab, P.vox, P.vox_is_f32, sp, P.xfactor, 0, P.g.nAng, P.out,
+                                          VCBCT_MODE_INTENSITY, nullptr, st, nullptr);
+            if (rc) return rc;
+        }
+        return 0;
+    }
     if (nblocks == 0) {  // one flat launch per voxel type present
         const int np = (int)b->probs.size();
         int rc = launch_forward_flat(b->d_probs, b->n_u16, b->flat_u16, 0, st);
@@ -171,6 +202,41 @@ static int batch_launch(vcbct_batch* b, int nblocks, hipStream_t st) {
     return rc;
 }
 
+// Lays the problems out for shape s (flat grid: each problem's tile map and first block; persistent queues:
+// the split's item ranges) and uploads the descriptors and queue totals.
+static int batch_layout(vcbct_batch* b, const TileShape& s) {
+    b->n_u16 = 0;
+    b->flat_u16 = b->flat_f32 = 0;
+    memset(b->totals, 0, sizeof b->totals);
+    for (BatchProblem& P : b->probs) {
+        const int nrg = (P.g.H + 63) / 64;
+        int& flat = P.vox_is_f32 ? b->flat_f32 : b->flat_u16;
+        const long nb = tile_map_host(P.g, P.g.nAng, s, P.tm);
+        if (nb < 0 || (long)flat + nb > 0x7fffffffL) return fail(VCBCT_EINVAL, "batch too large");
+        P.block_start = flat;
+        flat += (int)nb;
+        if (!P.vox_is_f32) ++b->n_u16;
+        P.split = s.split;
+        P.pcols = s.split ? P.g.W : (P.g.W + 7) / 8;
+        P.pang = s.split ? (P.g.nAng + 7) / 8 : P.g.nAng;
+        const int items = nrg * P.pang * P.pcols;
+        int* tot = b->totals + (P.vox_is_f32 ? 8 : 0);
+        for (int q = 0; q < 8; ++q) {
+            P.xcd_start[q] = tot[q];
+            tot[q] += items;
+        }
+    }
+    const size_t np = b->probs.size();
+    if (np) VCBCT_HIP(hipMemcpy(b->d_probs, b->probs.data(), np * sizeof(BatchProblem), hipMemcpyHostToDevice));
+    VCBCT_HIP(hipMemcpy(b->d_total, b->totals, 16 * sizeof(int), hipMemcpyHostToDevice));
+    b->tile = s;
+    return 0;
+}
+
+static bool same_shape(const TileShape& x, const TileShape& y) {
+    return x.rgb == y.rgb && x.cols == y.cols && x.order == y.order && x.split == y.split;
+}
+
 static int batch_finalize(vcbct_batch* b) {
     if (!b->dirty) return 0;
     const size_t np = b->probs.size();
@@ -178,22 +244,6 @@ static int batch_finalize(vcbct_batch* b) {
     // run over all problems in that order
     std::stable_sort(b->probs.begin(), b->probs.end(),
                      [](const BatchProblem& x, const BatchProblem& y) { return x.vox_is_f32 < y.vox_is_f32; });
-    b->n_u16 = 0;
-    b->flat_u16 = b->flat_f32 = 0;
-    memset(b->totals, 0, sizeof b->totals);
-    for (BatchProblem& P : b->probs) {
-        const int nrg = (P.g.H + 63) / 64;
-        int& flat = P.vox_is_f32 ? b->flat_f32 : b->flat_u16;
-        P.block_start = flat;
-        flat += nrg * P.g.nAng * P.quads_per_xcd * 8;
-        if (!P.vox_is_f32) ++b->n_u16;
-        const int items = nrg * P.g.nAng * P.cols_per_xcd;
-        int* tot = b->totals + (P.vox_is_f32 ? 8 : 0);
-        for (int q = 0; q < 8; ++q) {
-            P.xcd_start[q] = tot[q];
-            tot[q] += items;
-        }
-    }
     if (np > (size_t)b->cap) {
         (void)hipFree(b->d_probs);
         b->d_probs = nullptr;
@@ -204,52 +254,103 @@ static int batch_finalize(vcbct_batch* b) {
     if (!b->d_total && (dev_malloc((void**)&b->d_total, 16 * sizeof(int)) != hipSuccess ||
                         dev_malloc((void**)&b->d_counters, 16 * sizeof(int)) != hipSuccess))
         return fail(VCBCT_ENOMEM, "batch counters");
-    if (np) VCBCT_HIP(hipMemcpy(b->d_probs, b->probs.data(), np * sizeof(BatchProblem), hipMemcpyHostToDevice));
-    VCBCT_HIP(hipMemcpy(b->d_total, b->totals, 16 * sizeof(int), hipMemcpyHostToDevice));
-    // candidates: the flat grid (0) and persistent grids of 2, 4, 6 blocks per CU (one queue per XCD)
+    // Candidates: the grid (flat, or persistent with 2, 4, 6 blocks per CU, one queue per XCD) by the split
+    // (columns or angles divided among the XCDs); then, when the flat grid wins, its other 4-wave block
+    // shapes and the other order at the best split.  What the caller fixed is not searched.
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
     cus = (cus + 7) / 8 * 8;
-    int chosen = b->forced;
-    if (chosen < 0) {
+    const bool grid_fixed = b->forced != -1, tile_fixed = b->tile_forced.rgb > 0;
+    char fx[96];
+    snprintf(fx, sizeof fx, " grid%d tile%d,%d,%d,%d", b->forced, b->tile_forced.rgb, b->tile_forced.cols,
+             b->tile_forced.order, b->tile_forced.split);
+    const std::string key = b->signature + fx;
+    BatchChoice choice{grid_fixed ? b->forced : 0, tile_fixed ? b->tile_forced : kDefaultTile};
+    bool found = false;
+    {
         std::lock_guard<std::mutex> lk(g_btune_mu);
-        auto it = g_batch_blocks.find(b->signature);
-        if (it != g_batch_blocks.end()) chosen = it->second;
+        auto it = g_batch_choice.find(key);
+        if (it != g_batch_choice.end()) choice = it->second, found = true;
     }
     const char* off = getenv("VCBCT_NO_AUTOTUNE");
-    if (chosen < 0 && np && !(off && *off == '1')) {
+    // (a fixed sequence form has no shape of its own to search)
+    const bool fixed = (grid_fixed && tile_fixed) || b->forced == kSequence;
+    if (!found && np && !fixed && !(off && *off == '1')) {
         hipEvent_t e0, e1;
         VCBCT_HIP(hipEventCreate(&e0));
-        VCBCT_HIP(hipEventCreate(&e1));
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            return fail(VCBCT_EHIP, "batch autotune: event creation failed");
+        }
         float best = 1e30f;
-        std::string rep = "batch " + b->signature + ":";
-        for (int per : {0, 2, 4, 6}) {
-            int nb = cus * per;
-            int rc = batch_forward(b, nb, nullptr);
-            if (rc) return rc;
+        int rc = 0;
+        std::string rep = "batch " + key + ":";
+        auto time_one = [&](int nb, const TileShape& s) {
+            if (rc) return;
+            if (!same_shape(b->tile, s) || b->dirty) {
+                rc = batch_layout(b, s);
+                b->dirty = false;
+                if (rc) return;
+            }
+            rc = batch_forward(b, nb, nullptr);  // warm
+            if (rc) return;
             (void)hipEventRecord(e0, nullptr);
             rc = batch_forward(b, nb, nullptr);
-            if (rc) return rc;
+            if (rc) return;
             (void)hipEventRecord(e1, nullptr);
-            (void)hipEventSynchronize(e1);
             float ms = 0;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            char s[64];
-            if (per == 0) snprintf(s, sizeof s, " flat=%.3fms", ms);
-            else snprintf(s, sizeof s, " persistent %dx%d=%.3fms", per, cus, ms);
-            rep += s;
-            if (ms < best) { best = ms; chosen = nb; }
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = fail(VCBCT_EHIP, "batch autotune: timing failed");
+                return;
+            }
+            char t[96];
+            if (nb == kSequence) snprintf(t, sizeof t, " sequence=%.3fms", ms);
+            else if (nb == 0) snprintf(t, sizeof t, " flat/%dx%d/o%d/s%d=%.3fms", s.rgb, s.cols, s.order, s.split, ms);
+            else snprintf(t, sizeof t, " persistent %d/s%d=%.3fms", nb, s.split, ms);
+            rep += t;
+            if (ms < best) {
+                best = ms;
+                choice = BatchChoice{nb, s};
+            }
+        };
+        std::vector<int> grids;
+        if (grid_fixed) grids.push_back(b->forced);
+        else grids = {0, 2 * cus, 4 * cus, 6 * cus};
+        if (!grid_fixed) time_one(kSequence, kDefaultTile);  // (its shapes are the forward tuner's, per problem)
+        for (int sp = 0; sp < 2; ++sp) {
+            if (tile_fixed && sp != b->tile_forced.split) continue;
+            const TileShape s = tile_fixed ? b->tile_forced : TileShape{1, 4, 0, sp};
+            for (int nb : grids) time_one(nb, s);
+        }
+        if (!tile_fixed && choice.nblocks == 0) {
+            const int sp = choice.shape.split;
+            for (const int* bs : {kBlockShapes4[1], kBlockShapes4[2]}) time_one(0, TileShape{bs[0], bs[1], 0, sp});
+            const TileShape c = choice.shape;
+            time_one(0, TileShape{c.rgb, c.cols, 1, sp});
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        if (rc) return rc;
         std::lock_guard<std::mutex> lk(g_btune_mu);
-        g_batch_blocks[b->signature] = chosen;
-        g_btune_log += rep + " -> " + (chosen ? "persistent " + std::to_string(chosen) + " blocks" : std::string("flat")) +
-                       "\n";
+        g_batch_choice[key] = choice;
+        char t[96];
+        if (choice.nblocks == kSequence)
+            snprintf(t, sizeof t, " -> sequence\n");
+        else if (choice.nblocks == 0)
+            snprintf(t, sizeof t, " -> flat %dx%d/o%d/s%d\n", choice.shape.rgb, choice.shape.cols, choice.shape.order,
+                     choice.shape.split);
+        else
+            snprintf(t, sizeof t, " -> persistent %d blocks, s%d\n", choice.nblocks, choice.shape.split);
+        g_btune_log += rep + t;
     }
-    b->nblocks = chosen >= 0 ? chosen : 0;
+    int rc = batch_layout(b, choice.shape);
+    if (rc) return rc;
+    b->nblocks = choice.nblocks;
     b->dirty = false;
-    return 0;
+    // the sequence form's per-problem shapes are tuned by their first launch; make that happen here, outside
+    // any capture, rather than in a captured graph (which would record the default shapes)
+    if (b->nblocks == kSequence && np && !(off && *off == '1')) rc = batch_forward(b, kSequence, nullptr);
+    return rc;
 }
 
 extern "C" int vcbct_batch_run(vcbct_batch* b, void* stream) {
@@ -298,9 +399,34 @@ extern "C" int vcbct_batch_launch_graph(vcbct_batch* b, void* stream) {
     return 0;
 }
 
+extern "C" int vcbct_batch_set_tile(vcbct_batch* b, int row_groups, int cols, int order, int split) {
+    if (!b) return fail(VCBCT_EINVAL, "batch is NULL");
+    if (b->have_graph) return fail(VCBCT_ESTATE, "batch already captured; create a new batch");
+    if (row_groups == 0) {
+        b->tile_forced = TileShape{0, 0, 0, 0};
+    } else {
+        const TileShape s{row_groups, cols, order, split};
+        if (!tile_shape_valid(s) || row_groups * cols != 4)
+            return fail(VCBCT_EINVAL, "batch tile shape %dx%d (order %d, split %d): row groups x columns must be "
+                        "4, order and split 0 or 1", row_groups, cols, order, split);
+        b->tile_forced = s;
+    }
+    b->dirty = true;
+    return 0;
+}
+
+extern "C" int vcbct_batch_get_tile(const vcbct_batch* b, int* shape4) {
+    if (!b || !shape4) return fail(VCBCT_EINVAL, "batch_get_tile: NULL argument");
+    shape4[0] = b->tile.rgb;
+    shape4[1] = b->tile.cols;
+    shape4[2] = b->tile.order;
+    shape4[3] = b->tile.split;
+    return 0;
+}
+
 extern "C" int vcbct_batch_set_grid(vcbct_batch* b, int blocks) {
     if (!b) return fail(VCBCT_EINVAL, "batch is NULL");
-    if (blocks < -1 || blocks > (1 << 20)) return fail(VCBCT_EINVAL, "batch grid %d out of range", blocks);
+    if (blocks < kSequence || blocks > (1 << 20)) return fail(VCBCT_EINVAL, "batch grid %d out of range", blocks);
     if (b->have_graph) return fail(VCBCT_ESTATE, "batch already captured; create a new batch");
     b->forced = blocks > 0 ? (blocks + 7) / 8 * 8 : blocks;
     b->dirty = true;

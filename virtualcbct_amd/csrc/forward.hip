@@ -349,14 +349,7 @@ struct CountSeg {
 struct Tile {
     int rg, a, col;
 };
-struct TileMap {
-    int rgb, cols, order, split;
-    int nrb;  // row blocks, ceil(ceil(H / 64) / rgb)
-    int ncg;  // column groups, ceil(W / cols)
-    int gpx;  // per XCD: column groups (split 0) or angles (split 1)
-};
-__device__ __forceinline__ Tile tile_map(const TileMap& m, int nA) {
-    const int b = blockIdx.x;
+__device__ __forceinline__ Tile tile_map(const TileMap& m, int nA, int b) {
     const int xcd = b & 7, local = b >> 3, w = threadIdx.x >> 6;
     int cg, a, rb;
     if (m.split == 0) {
@@ -379,10 +372,12 @@ __device__ __forceinline__ Tile tile_map(const TileMap& m, int nA) {
     return t;
 }
 
+__device__ __forceinline__ Tile tile_map(const TileMap& m, int nA) { return tile_map(m, nA, blockIdx.x); }
+
 // Host side: the map of shape s over nA angles and its block count (-1 when the grid is too large).  Every
 // (row group, angle, column) of the launch falls to exactly one wave; waves past the last column, angle or
 // row group exit at once.
-static long tile_map_host(const ProjGeom& g, int nA, const TileShape& s, TileMap& m) {
+long tile_map_host(const ProjGeom& g, int nA, const TileShape& s, TileMap& m) {
     m.rgb = s.rgb;
     m.cols = s.cols;
     m.order = s.order;
@@ -504,11 +499,11 @@ __global__ __launch_bounds__(64 * WPB) VCBCT_WAVES8 void k_forward(ProjGeom g, c
 
 // ------------------------------------------------------------------------------------------------
 // Batched forward: many problems (volumes of different slice counts, angle counts and spectra) in one
-// persistent launch.  Work item = one wave = (problem, row group, angle, column).  The column range of
-// every (problem, row group, angle) is split into 8 contiguous chunks, one per XCD; each XCD has its own
-// work counter, so the waves of an XCD walk neighbouring columns of one angle (L2 locality).  A wave
-// that finds its XCD's queue empty steals from the other queues, so every queue drains and every wave
-// exits.
+// persistent launch.  Work item = one wave = (problem, row group, angle, column).  Each XCD has its own
+// queue (work counter) holding, per problem, either a contiguous eighth of the columns of every (row group,
+// angle) (split 0) or a contiguous eighth of the angles with all their columns (split 1); consecutive items
+// are neighbouring columns of one angle, so the waves of an XCD share voxels in its L2.  A wave that finds
+// its XCD's queue empty steals from the other queues, so every queue drains and every wave exits.
 __device__ __forceinline__ int find_problem(const BatchProblem* __restrict__ pr, int np, int xcd, int item) {
     int lo = 0, hi = np - 1;
     while (lo < hi) {  // last problem whose start <= item
@@ -538,20 +533,21 @@ __global__ __launch_bounds__(256) VCBCT_WAVES7 void k_forward_batch(const BatchP
             const int p = find_problem(pr, np, q, item);
             const BatchProblem& P = pr[p];
             const int local = item - P.xcd_start[q];
-            const int cl = local % P.cols_per_xcd;
-            const int ra = local / P.cols_per_xcd;  // rg * nA + a
-            const int a = ra % P.g.nAng, rg = ra / P.g.nAng;
-            const int jj = q * P.cols_per_xcd + cl;
-            if (jj >= P.g.W) continue;
+            const int cl = local % P.pcols;
+            const int ra = local / P.pcols;  // rg * pang + angle within the queue's range
+            const int al = ra % P.pang, rg = ra / P.pang;
+            const int a = P.split ? q * P.pang + al : al;
+            const int jj = P.split ? cl : q * P.pcols + cl;
+            if (jj >= P.g.W || a >= P.g.nAng) continue;
             forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.muf, P.pdff, P.NE,
                                                      P.inv_xf, a, rg, jj, P.out, nullptr, &steps[threadIdx.x >> 6]);
         }
     }
 }
 
-// Flat batched grid: the blocks of every problem of one voxel type laid end to end (each problem's block
-// count is a multiple of 8, so block b still runs on XCD b % 8 and the per-problem XCD-aware tile mapping
-// carries over).
+// Flat batched grid: the blocks of every problem of one voxel type laid end to end, each problem decoding its
+// own blocks with its tile map (each problem's block count is a multiple of 8, so block b still runs on XCD
+// b % 8 and the map's XCD assignment carries over).
 template <class VoxT>
 __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_forward_flat(const BatchProblem* __restrict__ pr, int np) {
     __shared__ WaveLds steps[4];
@@ -563,15 +559,10 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_forward_flat(const BatchPr
         else hi = mid - 1;
     }
     const BatchProblem& P = pr[lo];
-    const int local = b - P.block_start;
-    const int xcd = local & 7, l8 = local >> 3;
-    const int q = l8 % P.quads_per_xcd;
-    const int ra = l8 / P.quads_per_xcd;
-    const int a = ra % P.g.nAng, rg = ra / P.g.nAng;
-    const int jj = ((xcd * P.quads_per_xcd + q) << 2) + (threadIdx.x >> 6);
-    if (jj >= P.g.W) return;
+    const Tile t = tile_map(P.tm, P.g.nAng, b - P.block_start);
+    if (t.col >= P.g.W || t.a >= P.g.nAng || t.rg * 64 >= P.g.H) return;  // wave out of range: uniform exit
     forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.muf, P.pdff, P.NE,
-                                             P.inv_xf, a, rg, jj, P.out, nullptr, &steps[threadIdx.x >> 6]);
+                                             P.inv_xf, t.a, t.rg, t.col, P.out, nullptr, &steps[threadIdx.x >> 6]);
 }
 
 int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks, int vox_is_f32, hipStream_t st) {
@@ -1024,7 +1015,14 @@ static int tune_tiles(const std::string& key, hipStream_t st, const int (*blocks
         (void)hipEventDestroy(e0);
         return fail(VCBCT_EHIP, "autotune: event creation failed");
     }
-    int rc = launch(kDefaultTile);  // warm-up
+    // warm-up, timed: launches longer than 10 ms are timed once per candidate, shorter ones twice
+    float warm_ms = 0.0f;
+    (void)hipEventRecord(e0, st);
+    int rc = launch(kDefaultTile);
+    (void)hipEventRecord(e1, st);
+    if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&warm_ms, e0, e1) != hipSuccess))
+        rc = fail(VCBCT_EHIP, "autotune: timing failed");
+    const int reps = warm_ms > 10.0f ? 1 : 2;
     std::string rep = key + ":";
     std::vector<TileShape> done;
     float best_ms = 1e30f;
@@ -1036,7 +1034,7 @@ static int tune_tiles(const std::string& key, hipStream_t st, const int (*blocks
         done.push_back(s);
         float ms = 0.0f;
         (void)hipEventRecord(e0, st);
-        for (int r = 0; r < 2 && !rc; ++r) rc = launch(s);
+        for (int r = 0; r < reps && !rc; ++r) rc = launch(s);
         (void)hipEventRecord(e1, st);
         if (rc) return;
         if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
@@ -1044,7 +1042,8 @@ static int tune_tiles(const std::string& key, hipStream_t st, const int (*blocks
             return;
         }
         char b[64];
-        snprintf(b, sizeof b, " %dx%d/o%d/s%d=%.3fms", s.rgb, s.cols, s.order, s.split, ms / 2);
+        ms /= reps;
+        snprintf(b, sizeof b, " %dx%d/o%d/s%d=%.3fms", s.rgb, s.cols, s.order, s.split, ms);
         rep += b;
         if (ms < best_ms) {
             best_ms = ms;

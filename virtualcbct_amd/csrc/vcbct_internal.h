@@ -111,6 +111,26 @@ struct ChannelMix {
 // Writes B (row-major [nmat][VCBCT_MAX_MAT], first r columns used) and returns r (nmat when no smaller r does).
 int spectral_basis(const double* mu, int nmat, int NE, double tol, double B[VCBCT_MAX_MAT][VCBCT_MAX_MAT]);
 
+// Tile shape of the ray kernels: a block is rgb row groups (64 detector rows each) by `cols` adjacent
+// columns of one angle, one wave per (row group, column), rgb * cols in {2, 4, 8}; blocks walk angles inside
+// row blocks (order 0) or row blocks inside angles (order 1); the 8 XCDs divide the columns (split 0) or the
+// angles (split 1) among them.  rgb == 0: not set (the autotuner picks one per geometry).
+struct TileShape {
+    int rgb, cols, order, split;
+};
+// The shape the ray launches use when nothing else is known: 1 x 4, angles inner, columns split.
+constexpr TileShape kDefaultTile = {1, 4, 0, 0};
+bool tile_shape_valid(const TileShape& s);
+// The block -> (row group, angle, column) map of a shape over nA angles (the kernels decode it), and its block
+// count, a multiple of 8 (-1 when it exceeds the 32-bit grid).
+struct TileMap {
+    int rgb, cols, order, split;
+    int nrb;  // row blocks, ceil(ceil(H / 64) / rgb)
+    int ncg;  // column groups, ceil(W / cols)
+    int gpx;  // per XCD: column groups (split 0) or angles (split 1)
+};
+long tile_map_host(const ProjGeom& g, int nA, const TileShape& s, TileMap& m);
+
 struct vcbct_projector_impl;
 
 // One problem of a batched forward launch (device-resident descriptor).
@@ -124,11 +144,15 @@ struct BatchProblem {
     const float* pdff;
     float* out;
     double inv_xf;
+    double xfactor;     // as given (the sequence form launches the plain forward with it)
     int NE, vox_is_f32;
-    int cols_per_xcd;   // ceil(W / 8)
-    int xcd_start[8];   // first work item of this problem in each XCD queue (persistent form)
-    int quads_per_xcd;  // flat form: ceil(ceil(W / 4) / 8) four-column blocks per XCD per (row group, angle)
-    int block_start;    // flat form: first block of this problem in its voxel type's grid
+    // persistent form: the XCD queues hold (row group, angle, column) items; with split 0 queue q holds columns
+    // [q pcols, (q + 1) pcols) of every angle (pang = nAng), with split 1 angles [q pang, (q + 1) pang) of
+    // every column (pcols = W)
+    int split, pcols, pang;
+    int xcd_start[8];   // first work item of this problem in each XCD queue
+    TileMap tm;         // flat form: the problem's tile map (4-wave shapes)
+    int block_start;    // flat form: first block of this problem in its voxel type's grid (a multiple of 8)
 };
 int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xcd_total, int* d_counters, int nblocks,
                          int vox_is_f32, hipStream_t st);
@@ -140,16 +164,6 @@ int launch_transpose_f32(const float* src, float* dst, const ProjGeom& g, hipStr
 int launch_untranspose_f32(const float* src, float* dst, const ProjGeom& g, hipStream_t st);
 // The adjoint's channel planes [nch][y][x][z] -> the caller's maps [nmat][z][y][x] through the mix.
 int launch_untranspose_mix(const float* src, float* dst, const ProjGeom& g, const ChannelMix& mix, hipStream_t st);
-// Tile shape of the forward kernel: a block is rgb row groups (64 detector rows each) by `cols` adjacent
-// columns of one angle, one wave per (row group, column), rgb * cols in {2, 4, 8}; blocks walk angles inside
-// row blocks (order 0) or row blocks inside angles (order 1); the 8 XCDs divide the columns (split 0) or the
-// angles (split 1) among them.  rgb == 0: not set (the autotuner picks one per geometry).
-struct TileShape {
-    int rgb, cols, order, split;
-};
-// The shape the forward launches use when nothing else is known: 1 x 4, angles inner, columns split.
-constexpr TileShape kDefaultTile = {1, 4, 0, 0};
-bool tile_shape_valid(const TileShape& s);
 // Forward launch over angles [a0, a1).  tile: in, the caller's fixed shape (rgb > 0) or none (rgb == 0: the
 // autotuner's choice for this geometry, timed on the first launch outside stream capture); out, the shape used.
 int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int vox_is_f32, const Spectrum& sp,
