@@ -27,6 +27,28 @@ sys.path.insert(0, ROOT)
 
 METRIC = "forward rays*bins/s, FDK GUPS, adjoint grads/s vs plain GPU at <=1e-4 rel err"
 HBM_PEAK_GBS = 8000.0
+# Tile shapes fixed on the command line (--tile, --adjoint-tile, --batch-grid) instead of autotuned: profiled
+# runs fix the shapes the autotuner picked in an unprofiled run, so that the profiler sees no candidate launches.
+FIXED = {"tile": None, "adjoint_tile": None, "batch_grid": None}
+
+
+def emit(line):
+    """Prints the JSON line, with every tile-shape decision of the run (candidates and timings per kernel and
+    geometry) under "autotune_log"."""
+    from virtualcbct_amd.batch import autotune_log
+    line["autotune_log"] = [l for l in autotune_log().strip().split("\n") if l]
+    print(json.dumps(line), flush=True)
+
+
+def projector(*args):
+    """A Projector with the tile shapes given on the command line (else autotuned)."""
+    from virtualcbct_amd.projector import Projector
+    P = Projector(*args)
+    if FIXED["tile"]:
+        P.set_tile(*FIXED["tile"])
+    if FIXED["adjoint_tile"]:
+        P.set_adjoint_tile(*FIXED["adjoint_tile"])
+    return P
 
 
 def log(*a):
@@ -278,13 +300,12 @@ def bench_detector(line, dev, stream, e, out):
 
 def bench_adjoint(line, inp, dev, stream, e):
     import torch
-    from virtualcbct_amd.projector import Projector
     # adjoint on the differentiable workload (256^2 x 64 FP32 maps, 90 angles, 256^2 detector)
     from virtualcbct_amd import phantoms, io
     N4, Z4, n4, D4 = 256, 64, 90, 256
     ang4 = np.deg2rad(-180.0 + 4.0 * np.arange(n4) + 1e-4)
     c4, s4, x4, y4 = io.trig_table(D4, D4, ang4, np.zeros(n4), np.zeros(n4))
-    P4 = Projector(N4, 0.1, Z4, 0.1, D4, D4, n4, c4, s4, x4, y4)
+    P4 = projector(N4, 0.1, Z4, 0.1, D4, D4, n4, c4, s4, x4, y4)
     maps = torch.from_numpy(phantoms.water_cylinder_f32(N4, Z4, 0.1, 10.0)).to(dev).contiguous()
     P4.set_spectrum(inp["atten"], inp["pdf"])
     # target: log projection of the phantom with densities x (1 + 0.05 N(0, 1)) (seed 4000)
@@ -325,7 +346,6 @@ def bench_adjoint(line, inp, dev, stream, e):
 
 def bench_batched(line, inp, dev, stream, e, NE):
     import torch
-    from virtualcbct_amd.projector import Projector
     # batched: eight phantoms of uneven length (64 .. 256 slices), 360 angles, 91 bins — one batched launch
     # (grid form chosen by the autotuner) and its hipGraph replay against the plain per-phantom launches
     from virtualcbct_amd import phantoms, io
@@ -336,7 +356,7 @@ def bench_batched(line, inp, dev, stream, e, NE):
     projs, outs3 = [], []
     for i, Z in enumerate(zs):
         volt = phantoms.ellipsoid_phantom_torch(512, Z, 0.05, 0.1, 3000 + i, device=dev)
-        Pi = Projector(512, 0.05, Z, 0.1, 400, 400, 360, c3, s3, x3, y3)
+        Pi = projector(512, 0.05, Z, 0.1, 400, 400, 360, c3, s3, x3, y3)
         Pi.set_volume(volt)
         Pi.set_spectrum(inp["atten"], inp["pdf"])
         del volt
@@ -356,6 +376,8 @@ def bench_batched(line, inp, dev, stream, e, NE):
         plain_ms.append(e[0].elapsed_time(e[1]))
     ref0 = outs3[3].clone()
     B = Batch()
+    if FIXED["batch_grid"] is not None:
+        B.set_grid(FIXED["batch_grid"])
     for Pi, o in zip(projs, outs3):
         B.add_forward(Pi, o, 1000.0)
     B.run()  # finalises and autotunes the grid
@@ -401,7 +423,6 @@ def bench_decompose(line, td, dev, stream, e, iters=4):
     import time
     import torch
     from virtualcbct_amd import io, phantoms
-    from virtualcbct_amd.projector import Projector
     from virtualcbct_amd.decompose import FusedDecomposition, decompose_autograd
     spectra = {kv: phantoms.synthetic_spectrum(td, kv) for kv in (80, 120)}
     ang, xs, ys = phantoms.flexmap_angles(360)
@@ -413,7 +434,7 @@ def bench_decompose(line, td, dev, stream, e, iters=4):
         del vol
         projs, targets = [], []
         for kv in (80, 120):
-            P = Projector(512, 0.05, Z, 0.1, 400, 400, 360, c, s, x, y)
+            P = projector(512, 0.05, Z, 0.1, 400, 400, 360, c, s, x, y)
             P.set_spectrum(*spectra[kv])
             P.set_volume(truth)
             targets.append(P.logproj(1.0).clone())
@@ -468,7 +489,6 @@ def bench_phantom(line, td, dev, stream, e):
     import torch
     from oracle import oracle as O
     from virtualcbct_amd import io, material_maps, phantoms
-    from virtualcbct_amd.projector import Projector
     media, organs = phantoms.write_material_tables(os.path.join(td, "tables"), seed=3)
     lab_h = phantoms.synthetic_labels(543, 271, 100, seed=1)
     lab = torch.from_numpy(lab_h).to(dev)
@@ -477,7 +497,7 @@ def bench_phantom(line, td, dev, stream, e):
     maps = material_maps.labels_to_maps(lab, lut)
     ang, xs, ys = phantoms.flexmap_angles(1)
     c, s, x, y = io.trig_table(16, 16, ang, xs, ys)
-    P = Projector(512, 0.05, 100, 0.2, 16, 16, 1, c, s, x, y)
+    P = projector(512, 0.05, 100, 0.2, 16, 16, 1, c, s, x, y)
     P.set_volume_labels(lab, lut)
     torch.cuda.synchronize()
     nrep = 20
@@ -522,11 +542,10 @@ def bench_iterative(line, dev, stream, e):
     import torch
     from oracle import oracle as O
     from virtualcbct_amd import io, iterative, phantoms
-    from virtualcbct_amd.projector import Projector
     N, Nz, H, W, nA = 256, 64, 400, 400, 360
     ang, xs, ys = phantoms.flexmap_angles(nA)
     c, s_, x, y = io.trig_table(W, H, ang, xs, ys)
-    P = Projector(N, 0.1, Nz, 0.1, H, W, nA, c, s_, x, y)
+    P = projector(N, 0.1, Nz, 0.1, H, W, nA, c, s_, x, y)
     rng = np.random.default_rng(0)
     proj_h = rng.integers(20000, 60000, (nA, H, W)).astype(np.uint16)
     proj = torch.from_numpy(proj_h.view(np.int16)).to(dev)
@@ -600,9 +619,18 @@ def main():
                     help="run one secondary section alone and print its JSON (used under the profiler)")
     ap.add_argument("--decompose-iters", type=int, default=200,
                     help="Adam iterations of `--only decompose` (the default bench line times 4)")
+    ap.add_argument("--tile", default=None, help="fix the forward tile shape, row_groups,cols,order,split")
+    ap.add_argument("--adjoint-tile", default=None, help="fix the adjoint kernels' tile shape (4 waves)")
+    ap.add_argument("--batch-grid", type=int, default=None,
+                    help="fix the batched forward's grid (0 flat, n persistent blocks, -2 sequence)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (scale for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    for k in ("tile", "adjoint_tile"):
+        v = getattr(args, k)
+        if v:
+            FIXED[k] = tuple(int(t) for t in v.split(","))
+    FIXED["batch_grid"] = args.batch_grid
     if args.copy_calibration:
         import torch
         n = 1 << 28  # 2^28 floats = 1 GiB
@@ -620,14 +648,13 @@ def main():
 
     import torch
     from virtualcbct_amd import fdk
-    from virtualcbct_amd.projector import Projector
 
     dev = torch.device("cuda:0")
     td = tempfile.mkdtemp(prefix="vcbct_bench_")
     log("building synthetic c2 inputs ...")
     inp = build_c2(td)
     w = inp["w"]
-    P = Projector(w["N"], w["s"], w["Nz"], w["t"], w["H"], w["W"], w["n_ang"], inp["cs"], inp["sn"], inp["xc"],
+    P = projector(w["N"], w["s"], w["Nz"], w["t"], w["H"], w["W"], w["n_ang"], inp["cs"], inp["sn"], inp["xc"],
                   inp["yc"])
     P.set_volume(inp["vol"])
     P.set_spectrum(inp["atten"], inp["pdf"])
@@ -643,7 +670,7 @@ def main():
             bench_fdk(line, inp, dev, stream, out)
         else:
             bench_detector(line, dev, stream, [torch.cuda.Event(enable_timing=True) for _ in range(2)], out)
-        print(json.dumps(line), flush=True)
+        emit(line)
         return
     if args.only:
         line = {"section": args.only}
@@ -658,7 +685,7 @@ def main():
             bench_iterative(line, dev, stream, e)
         else:
             bench_batched(line, inp, dev, stream, e, NE)
-        print(json.dumps(line), flush=True)
+        emit(line)
         return
 
     log(f"forward warmup x{args.warmup} ...")
@@ -725,7 +752,7 @@ def main():
         line["cpu_baseline"] = cpu_baseline(inp)
         log(f"cpu baseline: {line['cpu_baseline']['value']:.4e} rays*bins/s on {line['cpu_baseline']['cores']} threads")
     P.close()
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 if __name__ == "__main__":

@@ -390,8 +390,8 @@ int vcbct_batch_capture_graph(vcbct_batch* b);
 int vcbct_batch_launch_graph(vcbct_batch* b, void* stream);
 /* Grid the autotuner chose for the batch's forward problems: the persistent grid size in blocks, 0 for the
  * flat grid (every problem's blocks laid end to end, no work queue), -2 for the sequence form (one plain
- * forward launch per problem, each with its geometry's autotuned tile shape; replayed together from the
- * captured graph); -1 for NULL. */
+ * forward launch per problem, each with the tile shape its projector had fixed when it was added, else its
+ * geometry's autotuned one; replayed together from the captured graph); -1 for NULL. */
 int vcbct_batch_blocks(const vcbct_batch* b);
 /* Fixes the forward grid instead of autotuning it: 0 = flat grid, n > 0 = persistent grid of n blocks
  * (rounded up to a multiple of 8), -2 = sequence form, -1 = autotune (default).  No reference counterpart
@@ -403,7 +403,8 @@ int vcbct_batch_set_grid(vcbct_batch* b, int blocks);
  * split together, then the flat grid's block shape and order).  No reference counterpart (tuning control). */
 int vcbct_batch_set_tile(vcbct_batch* b, int row_groups, int cols, int order, int split);
 /* The shape of the flat or persistent grid in use after the first run: shape4 = {row_groups, cols, order,
- * split} (the sequence form's launches each use their geometry's autotuned shape instead). */
+ * split} (the sequence form's launches each use their projector's fixed or their geometry's autotuned shape
+ * instead). */
 int vcbct_batch_get_tile(const vcbct_batch* b, int* shape4);
 /* Text log of the autotuner's decisions (candidates and timings); returns its length. */
 int vcbct_autotune_log(char* buf, int capacity);

@@ -970,7 +970,7 @@ static int launch_fwd_shape(const ProjGeom& g, const double* tables, const void*
 // Tile-shape autotuner.  Per key (kernel, geometry, launch parameters, device) the shape with the shortest
 // launch, timed on the caller's stream and buffers by coordinate descent: the (order, split) pairs at the
 // default block shape, then the other block shapes at the best pair, then the pairs again at the best block
-// shape.  Every candidate computes the same values (a shape is a schedule only); a kernel that accumulates is
+// shape (a candidate must beat the best so far by 1 %).  Every candidate computes the same values (a shape is a schedule only); a kernel that accumulates is
 // timed without its accumulator (the loss) or has it cleared afterwards (the gradient).  Nothing is timed
 // under stream capture (the default shape is recorded, not cached) or with VCBCT_NO_AUTOTUNE=1.  It replaces
 // hand-timed launch shapes (P/reprojectionOnGPU.cu:900-983 fixes them per kernel).
@@ -1045,7 +1045,9 @@ static int tune_tiles(const std::string& key, hipStream_t st, const int (*blocks
         ms /= reps;
         snprintf(b, sizeof b, " %dx%d/o%d/s%d=%.3fms", s.rgb, s.cols, s.order, s.split, ms);
         rep += b;
-        if (ms < best_ms) {
+        // a candidate replaces the best only when it is faster by more than 1 % (timing noise does not flip
+        // the choice between equal shapes from run to run)
+        if (ms < best_ms * 0.99f) {
             best_ms = ms;
             best = s;
         }

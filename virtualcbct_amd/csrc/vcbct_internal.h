@@ -145,6 +145,7 @@ struct BatchProblem {
     float* out;
     double inv_xf;
     double xfactor;     // as given (the sequence form launches the plain forward with it)
+    TileShape fixed;    // the projector's fixed tile shape when it was added (rgb == 0: autotuned)
     int NE, vox_is_f32;
     // persistent form: the XCD queues hold (row group, angle, column) items; with split 0 queue q holds columns
     // [q pcols, (q + 1) pcols) of every angle (pang = nAng), with split 1 angles [q pang, (q + 1) pang) of
