@@ -713,13 +713,22 @@ def main():
     torch.cuda.synchronize()
     alg_bytes = 12.0 * segs + 4.0 * rays
     k_avg_s = float(np.mean(kms)) * 1e-3
+    # what the kernel gathers per segment: the u16 record (16 B) or 2 / 4 FP32 spectral channels
+    nfc = P.forward_channels
+    rec, rec_bytes = {0: ("uint4", 16), 2: ("float2", 8), 1: ("float2", 8)}.get(nfc, ("Ch4", 16))
+    gathered = rec_bytes * segs + 4.0 * rays
     achieved = alg_bytes / k_avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
                 "traffic_source": "profiles/forward_c2_traffic.txt: rocprofv3 FETCH_SIZE and WRITE_SIZE of this "
                                   "kernel, scaled by their readings for a plain 1 GiB copy (not measured in this run)",
-                "kernel": f"k_forward<0, uint4, {P.tile[0] * P.tile[1]}>", "segments_per_launch": segs,
-                "algorithmic_bytes_per_launch": alg_bytes}
+                "kernel": f"k_forward<0, {rec}, {P.tile[0] * P.tile[1]}>", "segments_per_launch": segs,
+                "algorithmic_bytes_per_launch": alg_bytes,
+                "algorithmic_bytes_note": "12 B per charged segment (six u16 maps) + 4 B per ray, the traversal's "
+                                          "definition; the kernel gathers the volume's spectral channels instead",
+                "gathered_bytes_per_segment": rec_bytes,
+                "gathered_bytes_per_launch": gathered,
+                "gathered_gbs": gathered / k_avg_s / 1e9}
     from virtualcbct_amd.batch import autotune_log
     tile = {"row_groups": P.tile[0], "cols": P.tile[1], "order": P.tile[2], "split": P.tile[3],
             "autotune": [l for l in autotune_log().strip().split("\n") if l.startswith("fwd ")]}

@@ -201,12 +201,13 @@ int vcbct_projector_adjoint(vcbct_projector* p, const float* dev_target, float* 
  * dev_grad [mat][z][y][x] (overwritten) = sum_rays gp * mubar_m/xf * rr. */
 int vcbct_projector_logproj_backward(vcbct_projector* p, const float* dev_gp, float* dev_grad, double xfactor,
                                      void* stream);
-/* The log projection that also keeps its six line integrals per ray (dev_L: FP32 [nmat][n_angles*H*W], the
- * rays in the projection layout), and the backward pass from them: the coefficients are formed per ray from
- * dev_L (the same FP32 sums the backward's own traversal would form, so the same values) and only the
- * scatter traverses the rays again.  No reference counterpart; the autograd path uses this pair (24 B per ray
- * kept instead of a second traversal; the per-energy terms are never stored).  The backward does not read the
- * volume. */
+/* The log projection that also keeps its line integrals per ray (dev_L: FP32 [nmat][n_angles*H*W], the rays
+ * in the projection layout; the six maps' integrals, or in the first vcbct_projector_forward_channels rows the
+ * spectral channels' when the forward traverses channels), and the backward pass from them: the coefficients
+ * are formed per ray from dev_L (the same FP32 sums the backward's own traversal would form, so the same
+ * values) and only the scatter traverses the rays again.  No reference counterpart; the autograd path uses this
+ * pair (at most 24 B per ray kept instead of a second traversal; the per-energy terms are never stored).  The
+ * backward does not read the volume; the spectrum must be the one the log projection used. */
 int vcbct_projector_logproj_save(vcbct_projector* p, float* dev_p, float* dev_L, double xfactor, void* stream);
 /* Several spectra over the same rays, for a multi-spectrum (dual-energy) loss (no reference counterpart):
  * nspec (1..4) attenuation tables and pdfs concatenated, host_atten [spec][mat][NE[spec]] and host_pdf
@@ -232,6 +233,16 @@ int vcbct_projector_multi_loss_grad(vcbct_projector* p, const float* dev_targets
 int vcbct_adam_step(size_t n, float* dev_x, const float* dev_grad, float* dev_m, float* dev_v, int* dev_step,
                     double lr, double beta1, double beta2, double eps, double lower, void* stream);
 
+/* Number of spectral channels the forward traverses (no reference counterpart), 0 when it traverses the maps.
+ * With at most 4 channels (vcbct_projector_adjoint_channels) and a basis that reproduces every energy column of
+ * the table to 1e-6 of its largest entry, the intensity and log projections and the adjoint's coefficient
+ * kernels gather the channel volume c_k = sum_m B_mk rho_m (2 or 4 FP32 per voxel, built on the device from the
+ * volume when first needed after a volume or spectrum is set) in place of the six maps, and sum the spectrum over
+ * the channel table B^T mu; without a basis (num_material <= 4) the channels are the maps (from a u16 volume
+ * every sum is then the u16 path's, bit for bit).
+ * vcbct_projector_logproj_save then keeps the channels' line integrals (this many rows of dev_L).  Requires a
+ * spectrum (VCBCT_ESTATE otherwise). */
+int vcbct_projector_forward_channels(const vcbct_projector* p);
 /* Number of channels the adjoint scatters (no reference counterpart): num_material, or r < num_material when
  * the spectrum's attenuation table has a rank-r form mu = B (B^T mu) to 1e-6 of its largest entry (B orthonormal,
  * from the table's eigenvectors).  Attenuation between ~10 and ~150 keV is close to a photoelectric and a

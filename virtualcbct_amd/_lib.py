@@ -66,6 +66,7 @@ SIGNATURES = {
     "vcbct_projector_line_integrals": (c_int, [c_void_p, c_void_p, c_void_p]),
     "vcbct_projector_segment_counts": (c_int, [c_void_p, c_void_p, c_void_p]),
     "vcbct_projector_set_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    "vcbct_projector_forward_channels": (c_int, [c_void_p]),
     "vcbct_projector_get_tile": (c_int, [c_void_p, C.POINTER(c_int)]),
     "vcbct_projector_set_adjoint_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     "vcbct_reprojection": (c_int, [c_void_p, P_U16, P_F, P_F, c_int, c_double, P_F]),

@@ -135,7 +135,7 @@ extern "C" int vcbct_batch_add_forward(vcbct_batch* b, vcbct_projector* p, float
     P.out = dev_out;
     P.inv_xf = 1.0 / xfactor;
     P.xfactor = xfactor;
-    P.fixed = p->tile;
+    P.proj = p;
     // block and work-item offsets are assigned at finalize (problems grouped by voxel type, laid out for the
     // chosen shape); here only the totals are checked against the 32-bit grid and queue ranges, for the largest
     // layout any 4-wave shape gives
@@ -173,10 +173,9 @@ static int batch_forward(vcbct_batch* b, int nblocks, hipStream_t st) {
     if (b->probs.empty()) return 0;
     if (nblocks == kSequence) {  // one plain launch per problem, each with its projector's or geometry's shape
         for (const BatchProblem& P : b->probs) {
-            const Spectrum sp{P.mu, P.pdf, P.muf, P.pdff, P.NE, nullptr, 0};
-            TileShape t = P.fixed;
-            const int rc = launch_forward(P.g, P.tab, P.vox, P.vox_is_f32, sp, P.xfactor, 0, P.g.nAng, P.out,
-                                          VCBCT_MODE_INTENSITY, nullptr, st, &t);
+            // (the channel volume is rebuilt each time: a replayed graph then projects the volume as it is now)
+            const int rc = projector_forward(P.proj, VCBCT_MODE_INTENSITY, P.vox_is_f32 != 0, P.out, nullptr,
+                                             P.xfactor, 0, P.g.nAng, st, true);
             if (rc) return rc;
         }
         return 0;

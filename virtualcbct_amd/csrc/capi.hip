@@ -202,6 +202,8 @@ extern "C" void vcbct_projector_destroy(vcbct_projector* p) {
     (void)hipFree(p->d_muf);
     (void)hipFree(p->d_pdff);
     (void)hipFree(p->d_ch);
+    (void)hipFree(p->d_chf);
+    (void)hipFree(p->d_chv);
     free_multi(p);
     free(p);
 }
@@ -235,6 +237,7 @@ extern "C" int vcbct_projector_set_volume_u16(vcbct_projector* p, const uint16_t
     }
     if (rc) return rc;
     p->have_u16 = 1;
+    p->chv_key = 0;
     return 0;
 }
 
@@ -247,7 +250,33 @@ extern "C" int vcbct_projector_set_volume_f32(vcbct_projector* p, const float* d
     int rc = launch_transpose_f32(dev_maps, p->d_vox_f32, p->g, (hipStream_t)stream);
     if (rc) return rc;
     p->have_f32 = 1;
+    p->chv_key = 0;
     return 0;
+}
+
+// Whether the forward may sum over the channels of a basis B (r columns) of the table mu [nm][NE]: every
+// energy column reproduced to 1e-6 of its largest entry, |mu_me - (B B^T mu)_me| <= 1e-6 max_m' |mu_m'e|, so
+// that each bin's exponent sum_m mu_me L_m (L >= 0) is off by at most ~1e-6 of its own size (nonnegative
+// tables: the largest entry bounds the sum from below once it is weighted by the line integrals that carry it;
+// in practice the bins that dominate a projection hold similar entries).  r == nm: the identity, exact.
+static bool basis_columns_ok(const double* mu, int nm, int NE, int r, const double B[VCBCT_MAX_MAT][VCBCT_MAX_MAT]) {
+    if (r >= nm) return true;
+    for (int e = 0; e < NE; ++e) {
+        double cmax = 0.0, rmax = 0.0;
+        double proj[VCBCT_MAX_MAT];
+        for (int k = 0; k < r; ++k) {
+            proj[k] = 0.0;
+            for (int m = 0; m < nm; ++m) proj[k] += B[m][k] * mu[(size_t)m * NE + e];
+        }
+        for (int m = 0; m < nm; ++m) {
+            double v = 0.0;
+            for (int k = 0; k < r; ++k) v += B[m][k] * proj[k];
+            cmax = std::max(cmax, std::fabs(mu[(size_t)m * NE + e]));
+            rmax = std::max(rmax, std::fabs(mu[(size_t)m * NE + e] - v));
+        }
+        if (rmax > 1e-6 * cmax) return false;
+    }
+    return true;
 }
 
 extern "C" int vcbct_projector_set_spectrum(vcbct_projector* p, const float* atten, const float* pdf, int NE) {
@@ -289,19 +318,31 @@ extern "C" int vcbct_projector_set_spectrum(vcbct_projector* p, const float* att
         for (int k = 0; k < VCBCT_MAX_MAT; ++k)
             p->mix.b[m][k] = r < nm ? (float)B[m][k] : (m == k && m < nm ? 1.0f : 0.0f);
     (void)hipFree(p->d_ch);
+    (void)hipFree(p->d_chf);
     p->d_ch = nullptr;
+    p->d_chf = nullptr;
+    p->chv_key = 0;
+    p->fwd_ch = 0;
     if (r < nm) {
         std::vector<double> ch((size_t)r * NE);
+        std::vector<float> chf((size_t)r * NE);
         for (int k = 0; k < r; ++k)
             for (int e = 0; e < NE; ++e) {
                 double s = 0.0;
                 for (int m = 0; m < nm; ++m) s += B[m][k] * mu[(size_t)m * NE + e];
                 ch[(size_t)k * NE + e] = s;
+                chf[(size_t)k * NE + e] = (float)s;
             }
-        if (dev_malloc((void**)&p->d_ch, ch.size() * sizeof(double)) != hipSuccess)
-            return fail(VCBCT_ENOMEM, "device allocation of the adjoint channel table failed");
+        if (dev_malloc((void**)&p->d_ch, ch.size() * sizeof(double)) != hipSuccess ||
+            dev_malloc((void**)&p->d_chf, chf.size() * sizeof(float)) != hipSuccess)
+            return fail(VCBCT_ENOMEM, "device allocation of the channel tables failed");
         VCBCT_HIP(hipMemcpy(p->d_ch, ch.data(), ch.size() * sizeof(double), hipMemcpyHostToDevice));
+        VCBCT_HIP(hipMemcpy(p->d_chf, chf.data(), chf.size() * sizeof(float), hipMemcpyHostToDevice));
     }
+    // The forward and the coefficient kernels traverse the channel volume (2 or 4 FP32 channels per voxel in
+    // place of 6 u16 or 6 FP32 maps) when there are at most 4 channels and the basis keeps every bin's
+    // exponent to ~1e-6 of itself; without a basis the channels are the maps themselves (nmat <= 4).
+    p->fwd_ch = (p->mix.nch <= 4 && basis_columns_ok(mu.data(), nm, NE, r, B)) ? 1 : 0;
     p->NE = NE;
     return 0;
 }
@@ -312,11 +353,14 @@ static void free_multi(vcbct_projector* p) {
         (void)hipFree(p->ms[s].d_pdf);
         (void)hipFree(p->ms[s].d_muf);
         (void)hipFree(p->ms[s].d_ch);
+        (void)hipFree(p->ms[s].d_chf);
         p->ms[s].d_mu = p->ms[s].d_pdf = p->ms[s].d_ch = nullptr;
-        p->ms[s].d_muf = nullptr;
+        p->ms[s].d_muf = p->ms[s].d_chf = nullptr;
         p->ms[s].NE = 0;
     }
     p->nspec_multi = 0;
+    p->fwd_ch_multi = 0;
+    if (p->chv_key == 3) p->chv_key = 0;
 }
 
 extern "C" int vcbct_projector_set_spectra(vcbct_projector* p, int nspec, const float* atten, const float* pdf,
@@ -353,6 +397,7 @@ extern "C" int vcbct_projector_set_spectra(vcbct_projector* p, int nspec, const 
         const float* at = atten + (size_t)nm * off[s];
         const float* pd = pdf + off[s];
         std::vector<double> mu((size_t)nm * ne), pdd(ne), ch((size_t)nch * ne);
+        std::vector<float> chf((size_t)nch * ne);
         for (size_t i = 0; i < mu.size(); ++i) mu[i] = at[i];
         for (int e = 0; e < ne; ++e) pdd[e] = pd[e];
         for (int k = 0; k < nch; ++k)
@@ -363,12 +408,15 @@ extern "C" int vcbct_projector_set_spectra(vcbct_projector* p, int nspec, const 
                 else
                     v = mu[(size_t)k * ne + e];
                 ch[(size_t)k * ne + e] = v;
+                // without a basis the FP32 channel table is the table as given
+                chf[(size_t)k * ne + e] = r < nm ? (float)v : at[(size_t)k * ne + e];
             }
         auto& t = p->ms[s];
         if (dev_malloc((void**)&t.d_mu, mu.size() * sizeof(double)) != hipSuccess ||
             dev_malloc((void**)&t.d_pdf, pdd.size() * sizeof(double)) != hipSuccess ||
             dev_malloc((void**)&t.d_muf, mu.size() * sizeof(float)) != hipSuccess ||
-            dev_malloc((void**)&t.d_ch, ch.size() * sizeof(double)) != hipSuccess) {
+            dev_malloc((void**)&t.d_ch, ch.size() * sizeof(double)) != hipSuccess ||
+            dev_malloc((void**)&t.d_chf, chf.size() * sizeof(float)) != hipSuccess) {
             free_multi(p);
             return fail(VCBCT_ENOMEM, "device allocation of the spectra failed");
         }
@@ -376,9 +424,11 @@ extern "C" int vcbct_projector_set_spectra(vcbct_projector* p, int nspec, const 
         VCBCT_HIP(hipMemcpy(t.d_pdf, pdd.data(), pdd.size() * sizeof(double), hipMemcpyHostToDevice));
         VCBCT_HIP(hipMemcpy(t.d_muf, at, mu.size() * sizeof(float), hipMemcpyHostToDevice));
         VCBCT_HIP(hipMemcpy(t.d_ch, ch.data(), ch.size() * sizeof(double), hipMemcpyHostToDevice));
+        VCBCT_HIP(hipMemcpy(t.d_chf, chf.data(), chf.size() * sizeof(float), hipMemcpyHostToDevice));
         t.NE = ne;
     }
     p->nspec_multi = nspec;
+    p->fwd_ch_multi = (nch <= 4 && basis_columns_ok(joint.data(), nm, tot, r, B)) ? 1 : 0;
     return 0;
 }
 
@@ -386,6 +436,12 @@ extern "C" int vcbct_projector_multi_channels(const vcbct_projector* p) {
     if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
     if (p->nspec_multi < 1) return fail(VCBCT_ESTATE, "no spectra set (vcbct_projector_set_spectra)");
     return p->mix_multi.nch;
+}
+
+extern "C" int vcbct_projector_forward_channels(const vcbct_projector* p) {
+    if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
+    if (p->NE < 1) return fail(VCBCT_ESTATE, "no spectrum set (vcbct_projector_set_spectrum)");
+    return p->fwd_ch ? p->mix.nch : 0;
 }
 
 extern "C" int vcbct_projector_adjoint_channels(const vcbct_projector* p) {
@@ -410,9 +466,68 @@ static Spectrum spec_of(const vcbct_projector* p) {
     s.pdff = p->d_pdff;
     s.NE = p->NE;
     s.ch = p->d_ch ? p->d_ch : p->d_mu;
+    s.chf = p->d_chf ? p->d_chf : p->d_muf;
     s.nch = p->mix.nch;
+    s.nl = p->g.nmat;
     return s;
 }
+
+// The spectrum as the kernels over the channel volume take it: the exponents over the channel tables.
+static Spectrum spec_channels(const vcbct_projector* p) {
+    Spectrum s = spec_of(p);
+    s.mu = s.ch;
+    s.muf = s.chf;
+    s.nl = s.nch;
+    return s;
+}
+
+// The channel volume for key (1: the u16 volume, 2: the FP32 maps, both in the spectrum's basis; 3: the FP32 maps
+// in the spectra's joint basis), built now unless it already holds that (force: rebuild regardless, for callers
+// that replay captured work after the volume may have changed).  Sets *kind to VOX_CH2 / VOX_CH4.
+static int ensure_channels(vcbct_projector* p, int key, hipStream_t st, bool force, int* kind) {
+    const ChannelMix& mix = key == 3 ? p->mix_multi : p->mix;
+    const int c = mix.nch <= 2 ? 2 : 4;
+    *kind = c == 2 ? VOX_CH2 : VOX_CH4;
+    if (p->chv_key == key && !force) return 0;
+    const size_t bytes = vox_count(p->g) * c * sizeof(float);
+    if (p->chv_bytes < bytes) {
+        (void)hipFree(p->d_chv);
+        p->d_chv = nullptr;
+        p->chv_bytes = 0;
+        p->chv_key = 0;
+        if (dev_malloc((void**)&p->d_chv, bytes) != hipSuccess)
+            return fail(VCBCT_ENOMEM, "device allocation of the channel volume (%zu bytes) failed", bytes);
+        p->chv_bytes = bytes;
+    }
+    const bool f32 = key != 1;
+    int rc = launch_channels(f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, f32 ? 1 : 0, p->d_chv, p->g,
+                             mix, st);
+    if (rc) return rc;
+    p->chv_key = key;
+    return 0;
+}
+
+namespace vcbct {
+// The forward launch of a projector (intensity or log projection) over its channel volume when its spectrum
+// allows it, else over its maps; the tile shape fixed on the projector or autotuned.
+int projector_forward(vcbct_projector* p, int mode, bool use_f32, float* out, float* aux, double xfactor, int a0,
+                      int a1, hipStream_t st, bool rebuild_channels) {
+    TileShape ts = p->tile;
+    int rc;
+    if (p->fwd_ch) {
+        int kind = 0;
+        rc = ensure_channels(p, use_f32 ? 2 : 1, st, rebuild_channels, &kind);
+        if (rc) return rc;
+        rc = launch_forward(p->g, p->d_tables, p->d_chv, kind, spec_channels(p), xfactor, a0, a1, out, mode, aux, st,
+                            &ts);
+    } else {
+        rc = launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16,
+                            use_f32 ? VOX_F32 : VOX_U16, spec_of(p), xfactor, a0, a1, out, mode, aux, st, &ts);
+    }
+    if (!rc) p->tile_used = ts;
+    return rc;
+}
+}  // namespace vcbct
 
 extern "C" int vcbct_projector_forward(vcbct_projector* p, float* dev_out, double xfactor, int a0, int a1, void* stream) {
     if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
@@ -424,12 +539,8 @@ extern "C" int vcbct_projector_forward(vcbct_projector* p, float* dev_out, doubl
     if (a1 < 0) a1 = p->g.nAng;
     if (a0 < 0 || a0 > a1 || a1 > p->g.nAng) return fail(VCBCT_EINVAL, "angle range [%d, %d) invalid", a0, a1);
     VCBCT_HIP(hipSetDevice(p->device));
-    TileShape ts = p->tile;
-    rc = launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
-                        spec_of(p), xfactor, a0, a1, dev_out, VCBCT_MODE_INTENSITY, nullptr,
-                        (hipStream_t)stream, &ts);
-    if (!rc) p->tile_used = ts;
-    return rc;
+    return projector_forward(p, VCBCT_MODE_INTENSITY, use_f32, dev_out, nullptr, xfactor, a0, a1, (hipStream_t)stream,
+                             false);
 }
 
 extern "C" int vcbct_projector_line_integrals(vcbct_projector* p, float* dev_L, void* stream) {
@@ -439,9 +550,10 @@ extern "C" int vcbct_projector_line_integrals(vcbct_projector* p, float* dev_L, 
     if (rc) return rc;
     if (!dev_L) return fail(VCBCT_EINVAL, "line_integrals: output is NULL");
     VCBCT_HIP(hipSetDevice(p->device));
-    Spectrum sp{nullptr, nullptr, nullptr, nullptr, 0, nullptr, 0};
-    return launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
-                          sp, 1.0, 0, p->g.nAng, nullptr, VCBCT_MODE_LINEINT, dev_L, (hipStream_t)stream);
+    Spectrum sp{nullptr, nullptr, nullptr, nullptr, 0, nullptr, 0, p->g.nmat, nullptr};
+    return launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16,
+                          use_f32 ? VOX_F32 : VOX_U16, sp, 1.0, 0, p->g.nAng, nullptr, VCBCT_MODE_LINEINT, dev_L,
+                          (hipStream_t)stream);
 }
 
 extern "C" int vcbct_reprojection_angles(vcbct_projector* p, const uint16_t* host_vol, const float* atten,
@@ -485,12 +597,8 @@ extern "C" int vcbct_projector_logproj(vcbct_projector* p, float* dev_p, double 
     if (!dev_p) return fail(VCBCT_EINVAL, "logproj: output is NULL");
     if (!(xfactor > 0)) return fail(VCBCT_EINVAL, "xfactor must be positive");
     VCBCT_HIP(hipSetDevice(p->device));
-    TileShape ts = p->tile;
-    rc = launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
-                        spec_of(p), xfactor, 0, p->g.nAng, dev_p, VCBCT_MODE_LOGPROJ, nullptr,
-                        (hipStream_t)stream, &ts);
-    if (!rc) p->tile_used = ts;
-    return rc;
+    return projector_forward(p, VCBCT_MODE_LOGPROJ, use_f32, dev_p, nullptr, xfactor, 0, p->g.nAng,
+                             (hipStream_t)stream, false);
 }
 
 static int run_adjoint(vcbct_projector* p, const float* tg, float* dev_grad, double* dev_loss, double xfactor, int mode,
@@ -512,8 +620,16 @@ static int run_adjoint(vcbct_projector* p, const float* tg, float* dev_grad, dou
     rc = launch_zero(p->d_grad_vox, nv * 4 * p->mix.nch, st);
     if (rc) return rc;
     if (dev_loss && (rc = launch_zero(dev_loss, sizeof(double), st))) return rc;
-    rc = launch_adjoint(p->g, p->d_tables, p->d_vox_f32, spec_of(p), xfactor, tg, mode, p->d_coef, p->d_grad_vox,
-                        dev_loss, st, p->adj_tile);
+    if (p->fwd_ch) {  // the coefficient kernel over the channel volume
+        int kind = 0;
+        rc = ensure_channels(p, 2, st, false, &kind);
+        if (!rc)
+            rc = launch_adjoint(p->g, p->d_tables, p->d_chv, kind, spec_channels(p), xfactor, tg, mode, p->d_coef,
+                                p->d_grad_vox, dev_loss, st, p->adj_tile);
+    } else {
+        rc = launch_adjoint(p->g, p->d_tables, p->d_vox_f32, VOX_F32, spec_of(p), xfactor, tg, mode, p->d_coef,
+                            p->d_grad_vox, dev_loss, st, p->adj_tile);
+    }
     if (rc) return rc;
     return launch_untranspose_mix(p->d_grad_vox, dev_grad, p->g, p->mix, st);
 }
@@ -552,6 +668,7 @@ extern "C" int vcbct_projector_multi_loss_grad(vcbct_projector* p, const float* 
         ss.pdf[s] = p->ms[s].d_pdf;
         ss.muf[s] = p->ms[s].d_muf;
         ss.ch[s] = p->ms[s].d_ch;
+        ss.chf[s] = p->ms[s].d_chf;
         ss.NE[s] = p->ms[s].NE;
     }
     ss.nspec = p->nspec_multi;
@@ -559,8 +676,16 @@ extern "C" int vcbct_projector_multi_loss_grad(vcbct_projector* p, const float* 
     rc = launch_zero(p->d_grad_vox, nv * 4 * ss.nch, st);
     if (rc) return rc;
     if (dev_loss && (rc = launch_zero(dev_loss, sizeof(double), st))) return rc;
-    rc = launch_adjoint_multi(p->g, p->d_tables, p->d_vox_f32, ss, xfactor, dev_targets, p->d_coef, p->d_grad_vox,
-                              dev_loss, st, p->adj_tile);
+    if (p->fwd_ch_multi) {  // the coefficient kernel over the joint basis's channel volume
+        int kind = 0;
+        rc = ensure_channels(p, 3, st, false, &kind);
+        if (!rc)
+            rc = launch_adjoint_multi(p->g, p->d_tables, p->d_chv, kind, ss, xfactor, dev_targets, p->d_coef,
+                                      p->d_grad_vox, dev_loss, st, p->adj_tile);
+    } else {
+        rc = launch_adjoint_multi(p->g, p->d_tables, p->d_vox_f32, VOX_F32, ss, xfactor, dev_targets, p->d_coef,
+                                  p->d_grad_vox, dev_loss, st, p->adj_tile);
+    }
     if (rc) return rc;
     return launch_untranspose_mix(p->d_grad_vox, dev_grad, p->g, p->mix_multi, st);
 }
@@ -574,12 +699,8 @@ extern "C" int vcbct_projector_logproj_save(vcbct_projector* p, float* dev_p, fl
     if (!dev_p || !dev_L) return fail(VCBCT_EINVAL, "logproj_save: NULL argument");
     if (!(xfactor > 0)) return fail(VCBCT_EINVAL, "xfactor must be positive");
     VCBCT_HIP(hipSetDevice(p->device));
-    TileShape ts = p->tile;
-    rc = launch_forward(p->g, p->d_tables, use_f32 ? (const void*)p->d_vox_f32 : (const void*)p->d_vox_u16, use_f32,
-                        spec_of(p), xfactor, 0, p->g.nAng, dev_p, VCBCT_MODE_LOGPROJ, dev_L,
-                        (hipStream_t)stream, &ts);
-    if (!rc) p->tile_used = ts;
-    return rc;
+    return projector_forward(p, VCBCT_MODE_LOGPROJ, use_f32, dev_p, dev_L, xfactor, 0, p->g.nAng, (hipStream_t)stream,
+                             false);
 }
 
 extern "C" int vcbct_projector_set_tile(vcbct_projector* p, int row_groups, int cols, int order, int split) {
@@ -636,8 +757,9 @@ extern "C" int vcbct_projector_logproj_backward_saved(vcbct_projector* p, const 
         return fail(VCBCT_ENOMEM, "device allocation of the adjoint coefficients failed");
     rc = launch_zero(p->d_grad_vox, nv * 4 * p->mix.nch, st);
     if (rc) return rc;
-    rc = launch_adjoint_saved(p->g, p->d_tables, dev_L, spec_of(p), xfactor, dev_gp, p->d_coef, p->d_grad_vox, st,
-                              p->adj_tile);
+    // dev_L holds what logproj_save kept: the channel line integrals when the forward runs over channels
+    rc = launch_adjoint_saved(p->g, p->d_tables, dev_L, p->fwd_ch ? spec_channels(p) : spec_of(p), xfactor, dev_gp,
+                              p->d_coef, p->d_grad_vox, st, p->adj_tile);
     if (rc) return rc;
     return launch_untranspose_mix(p->d_grad_vox, dev_grad, p->g, p->mix, st);
 }
@@ -646,7 +768,7 @@ extern "C" int vcbct_projector_segment_counts(vcbct_projector* p, float* dev_cou
     if (!p) return fail(VCBCT_EINVAL, "projector is NULL");
     if (!dev_counts) return fail(VCBCT_EINVAL, "segment_counts: output is NULL");
     VCBCT_HIP(hipSetDevice(p->device));
-    Spectrum sp{nullptr, nullptr, nullptr, nullptr, 0, nullptr, 0};
-    return launch_forward(p->g, p->d_tables, nullptr, 0, sp, 1.0, 0, p->g.nAng, nullptr, VCBCT_MODE_SEGCOUNT,
+    Spectrum sp{nullptr, nullptr, nullptr, nullptr, 0, nullptr, 0, p->g.nmat, nullptr};
+    return launch_forward(p->g, p->d_tables, nullptr, VOX_U16, sp, 1.0, 0, p->g.nAng, nullptr, VCBCT_MODE_SEGCOUNT,
                           dev_counts, (hipStream_t)stream);
 }

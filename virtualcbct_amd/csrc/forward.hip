@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 #include <cmath>
@@ -241,6 +242,118 @@ struct AccumL<float4> {
     }
 };
 
+// Channel volumes: the spectral channels c_k = sum_m B_mk rho_m of the maps in FP32, two (float2, 8 bytes) or
+// four (Ch4, 16 bytes) to a voxel record; L[k] accumulates the channel line integrals (L[k] = 0 beyond the
+// channel count).  The first gather of a step is software-pipelined as for the u16 maps.  With the identity
+// basis (channels = maps) the channels of a u16 volume are its values exactly, and every product and sum is the
+// u16 path's, in its order, bit for bit.
+struct alignas(16) Ch4 {
+    float x, y, z, w;
+};
+
+template <>
+struct AccumL<float2> {
+    const float2* __restrict__ vox;
+    int Nz;
+    float L[VCBCT_MAX_MAT];
+    float2 p0 = make_float2(0.f, 0.f);  // the previous step's first gather
+    float r0 = 0.f;                     // and its length (0: nothing charged)
+    __device__ __forceinline__ void step(const Seg& s0, const Seg& s1, const Seg& s2) {
+        const float2 q0 = ldg(vox, (size_t)(s0.ok ? s0.cell : 0));
+        float2 q1, q2;
+        if (s1.ok) q1 = ldg(vox, (size_t)s1.cell);
+        if (s2.ok) q2 = ldg(vox, (size_t)s2.cell);
+        add(p0, r0);
+        if (s1.ok) add(q1, (float)s1.rr);
+        if (s2.ok) add(q2, (float)s2.rr);
+        p0 = q0;
+        r0 = s0.ok ? (float)s0.rr : 0.f;
+    }
+    __device__ __forceinline__ void finish() {
+        add(p0, r0);
+        r0 = 0.f;
+    }
+    __device__ __forceinline__ void add(const float2& q, float rr) {
+#pragma clang fp contract(fast)
+        L[0] += q.x * rr;
+        L[1] += q.y * rr;
+    }
+};
+
+template <>
+struct AccumL<Ch4> {
+    const Ch4* __restrict__ vox;
+    int Nz;
+    float L[VCBCT_MAX_MAT];
+    Ch4 p0 = {0.f, 0.f, 0.f, 0.f};
+    float r0 = 0.f;
+    __device__ __forceinline__ void step(const Seg& s0, const Seg& s1, const Seg& s2) {
+        const Ch4 q0 = ldg(vox, (size_t)(s0.ok ? s0.cell : 0));
+        Ch4 q1, q2;
+        if (s1.ok) q1 = ldg(vox, (size_t)s1.cell);
+        if (s2.ok) q2 = ldg(vox, (size_t)s2.cell);
+        add(p0, r0);
+        if (s1.ok) add(q1, (float)s1.rr);
+        if (s2.ok) add(q2, (float)s2.rr);
+        p0 = q0;
+        r0 = s0.ok ? (float)s0.rr : 0.f;
+    }
+    __device__ __forceinline__ void finish() {
+        add(p0, r0);
+        r0 = 0.f;
+    }
+    __device__ __forceinline__ void add(const Ch4& q, float rr) {
+#pragma clang fp contract(fast)
+        L[0] += q.x * rr;
+        L[1] += q.y * rr;
+        L[2] += q.z * rr;
+        L[3] += q.w * rr;
+    }
+};
+
+// The channel volume of a material volume, one thread per voxel: c_k = sum_m b[m][k] rho_m in FP32, m
+// ascending (with the identity mix c_k = rho_k exactly).  SrcT uint4: one u16 record per voxel; float4: two
+// FP32 records per voxel.  DstT float2 or Ch4 (channels past mix.nch are zero).
+template <class SrcT, class DstT>
+__global__ __launch_bounds__(256) void k_channels(const SrcT* __restrict__ src, DstT* __restrict__ dst, size_t nvox,
+                                                  ChannelMix mix, int nmat) {
+    constexpr int C = sizeof(DstT) / sizeof(float);
+    for (size_t v = (size_t)blockIdx.x * 256 + threadIdx.x; v < nvox; v += (size_t)gridDim.x * 256) {
+        float rho[VCBCT_MAX_MAT];
+        if constexpr (std::is_same<SrcT, uint4>::value) {
+            const uint4 q = ldg(src, v);
+            rho[0] = (float)(q.x & 0xffffu);
+            rho[1] = (float)(q.x >> 16);
+            rho[2] = (float)(q.y & 0xffffu);
+            rho[3] = (float)(q.y >> 16);
+            rho[4] = (float)(q.z & 0xffffu);
+            rho[5] = (float)(q.z >> 16);
+        } else {
+            const float4 a = ldg(src, 2 * v), b = ldg(src, 2 * v + 1);
+            rho[0] = a.x;
+            rho[1] = a.y;
+            rho[2] = a.z;
+            rho[3] = a.w;
+            rho[4] = b.x;
+            rho[5] = b.y;
+        }
+        float c[C];
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            float acc = 0.0f;
+            if (k < mix.nch) {
+#pragma unroll
+                for (int m = 0; m < VCBCT_MAX_MAT; ++m)
+                    if (m < nmat) acc = fmaf(mix.b[m][k], rho[m], acc);
+            }
+            c[k] = acc;
+        }
+        DstT r;
+        __builtin_memcpy(&r, c, sizeof r);
+        dst[v] = r;  // (a kernel argument: the compiler knows it addresses global memory)
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Spectral sums of opaque rays.  The fast paths form each energy term with an FP32 exponential, which
 // underflows once mu.L / xf exceeds ~87 (103 with denormals) in a bin; a ray that is that opaque in every bin
@@ -401,8 +514,8 @@ template <int MODE, class VoxT>
 __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __restrict__ tab,
                                              const VoxT* __restrict__ vox, const double* __restrict__ mu,
                                              const double* __restrict__ pdf, const float* __restrict__ muf,
-                                             const float* __restrict__ pdff, int NE, double inv_xf, int a, int rg,
-                                             int jj, float* __restrict__ out, float* __restrict__ aux,
+                                             const float* __restrict__ pdff, int NE, double inv_xf, int nl, int a,
+                                             int rg, int jj, float* __restrict__ out, float* __restrict__ aux,
                                              WaveLds* steps) {
     const int lane = threadIdx.x & 63;
     const int j = rg * 64 + lane;
@@ -430,11 +543,11 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
         for (int m = 0; m < g.nmat; ++m) stg(aux, o * g.nmat + m, (float)acc.L[m]);
         return;
     }
-    if (MODE == VCBCT_MODE_LOGPROJ && aux) {  // the six line integrals, kept for the backward pass ([mat][ray])
+    if (MODE == VCBCT_MODE_LOGPROJ && aux) {  // the nl line integrals, kept for the backward pass ([term][ray])
         const size_t nrays = (size_t)g.nAng * g.H * g.W;
 #pragma unroll
         for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-            if (m < g.nmat) stg(aux, (size_t)m * nrays + o, acc.L[m]);
+            if (m < nl) stg(aux, (size_t)m * nrays + o, acc.L[m]);
     }
     bool zero = true;
 #pragma unroll
@@ -455,11 +568,11 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
                 float tr = 0.0f;
 #pragma unroll
                 for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                    if (m < g.nmat) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
+                    if (m < nl) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
                 s += ldg(pdff, e) * exp2f(tr * k2) * lf;
             }
             // opaque ray: the FP32 exponentials underflow; the reference's own FP64 terms instead
-            if (s < 1e-30f) s = intensity_exact(acc.L, g.nmat, mu, pdf, NE, inv_xf, leng);
+            if (s < 1e-30f) s = intensity_exact(acc.L, nl, mu, pdf, NE, inv_xf, leng);
         }
         stg(out, o, s);
     } else {  // VCBCT_MODE_LOGPROJ: p = -log(sum_e pdf_e e^{-mu.L/xf} / sum_e pdf_e); FP32 terms, FP64 sums
@@ -469,13 +582,13 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
             float tr = 0.0f;
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                if (m < g.nmat) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
+                if (m < nl) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
             const double pe = ldg(pdf, e);
             sw += pe;
             se += pe * (double)exp2f(tr * k2);
         }
         // opaque ray (the FP32 exponentials underflow): FP64 terms relative to the smallest exponent
-        const double p = se < 1e-30 * sw ? logproj_opaque(acc.L, g.nmat, mu, pdf, NE, inv_xf, sw) : -log(se / sw);
+        const double p = se < 1e-30 * sw ? logproj_opaque(acc.L, nl, mu, pdf, NE, inv_xf, sw) : -log(se / sw);
         stg(out, o, (float)p);
     }
 }
@@ -488,12 +601,12 @@ __global__ __launch_bounds__(64 * WPB) VCBCT_WAVES8 void k_forward(ProjGeom g, c
                                                                  const double* __restrict__ pdf,
                                                                  const float* __restrict__ muf,
                                                                  const float* __restrict__ pdff, int NE,
-                                                                 double inv_xf, int a0, int nA, TileMap tm,
+                                                                 double inv_xf, int nl, int a0, int nA, TileMap tm,
                                                                  float* __restrict__ out, float* __restrict__ aux) {
     __shared__ WaveLds steps[WPB];
     const Tile t = tile_map(tm, nA);
     if (t.col >= g.W || t.a >= nA || t.rg * 64 >= g.H) return;  // wave out of range: uniform exit
-    forward_wave<MODE, VoxT>(g, tab, vox, mu, pdf, muf, pdff, NE, inv_xf, a0 + t.a, t.rg, t.col, out, aux,
+    forward_wave<MODE, VoxT>(g, tab, vox, mu, pdf, muf, pdff, NE, inv_xf, nl, a0 + t.a, t.rg, t.col, out, aux,
                              &steps[threadIdx.x >> 6]);
 }
 
@@ -540,7 +653,8 @@ __global__ __launch_bounds__(256) VCBCT_WAVES7 void k_forward_batch(const BatchP
             const int jj = P.split ? cl : q * P.pcols + cl;
             if (jj >= P.g.W || a >= P.g.nAng) continue;
             forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.muf, P.pdff, P.NE,
-                                                     P.inv_xf, a, rg, jj, P.out, nullptr, &steps[threadIdx.x >> 6]);
+                                                     P.inv_xf, P.g.nmat, a, rg, jj, P.out, nullptr,
+                                                     &steps[threadIdx.x >> 6]);
         }
     }
 }
@@ -562,7 +676,8 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_forward_flat(const BatchPr
     const Tile t = tile_map(P.tm, P.g.nAng, b - P.block_start);
     if (t.col >= P.g.W || t.a >= P.g.nAng || t.rg * 64 >= P.g.H) return;  // wave out of range: uniform exit
     forward_wave<VCBCT_MODE_INTENSITY, VoxT>(P.g, P.tab, (const VoxT*)P.vox, P.mu, P.pdf, P.muf, P.pdff, P.NE,
-                                             P.inv_xf, t.a, t.rg, t.col, P.out, nullptr, &steps[threadIdx.x >> 6]);
+                                             P.inv_xf, P.g.nmat, t.a, t.rg, t.col, P.out, nullptr,
+                                             &steps[threadIdx.x >> 6]);
 }
 
 int launch_forward_flat(const BatchProblem* d_problems, int np, int total_blocks, int vox_is_f32, hipStream_t st) {
@@ -651,12 +766,14 @@ struct ScatterG {
     }
 };
 
-template <int MODE>
+// VoxT float4: the FP32 maps (mu / muf the material tables, nl = nmat); float2 / Ch4: their channel volume
+// (mu / muf the channel tables, nl = nch).
+template <int MODE, class VoxT>
 __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, const double* __restrict__ tab,
-                                                      const float4* __restrict__ vox, const double* __restrict__ mu,
+                                                      const VoxT* __restrict__ vox, const double* __restrict__ mu,
                                                       const double* __restrict__ pdf, const float* __restrict__ muf,
                                                       const double* __restrict__ ch, int nch, int NE, double inv_xf,
-                                                      TileMap tm, const float* __restrict__ tg,
+                                                      int nl, TileMap tm, const float* __restrict__ tg,
                                                       float* __restrict__ coef, double* __restrict__ loss) {
     __shared__ WaveLds step_tabs[4];
     WaveLds* steps = &step_tabs[threadIdx.x >> 6];
@@ -669,12 +786,13 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, c
     const bool active = j < g.H;
     const double c = ldg(tab, a), sn = ldg(tab, g.nAng + a), xc = ldg(tab, 2 * g.nAng + a),
                  yc = ldg(tab, 3 * g.nAng + a);
-    AccumL<float4> acc;
+    AccumL<VoxT> acc;
     acc.vox = vox;
     acc.Nz = g.Nz;
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0f;
     traverse_wave(g, c, sn, xc, yc, j, jj, active, acc, steps);
+    acc.finish();  // (the pipelined accumulators add their last gather here)
 
     double lossc = 0.0;
     if (active) {
@@ -685,7 +803,7 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, c
             float tr = 0.0f;
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                if (m < g.nmat) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
+                if (m < nl) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
             const double pe = ldg(pdf, e);
             const double we = pe * (double)exp2f(tr * k2);
             sw += pe;
@@ -698,7 +816,7 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, c
         const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - j) * g.W + jj;
         double p;
         if (se < 1e-30 * sw) {  // opaque ray: FP64 terms relative to the smallest exponent
-            p = mubar_opaque(acc.L, g.nmat, mu, pdf, NE, inv_xf, sw, mub, ch, nch);
+            p = mubar_opaque(acc.L, nl, mu, pdf, NE, inv_xf, sw, mub, ch, nch);
         } else {
             p = -log(se / sw);
 #pragma unroll
@@ -729,9 +847,11 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_coef(ProjGeom g, c
 // for every spectrum s the log projection p_s, its residual against the target t_s ([s][ray]) and its
 // coefficients; the loss sums (p_s - t_s)^2 over spectra and rays, and the stored coefficient of a channel is
 // the sum over the spectra, c_k = sum_s 2 (p_s - t_s) chbar_sk / xf, so that one scatter serves all spectra.
-template <int NCH>  // the channel count (the arrays of the spectral loop stay in registers)
+// NCH: the channel count (the arrays of the spectral loop stay in registers).  VoxT float4: the FP32 maps (the
+// exponents over the material tables); float2 / Ch4: the joint basis's channel volume (over the channel tables).
+template <int NCH, class VoxT>
 __global__ __launch_bounds__(256) VCBCT_WAVES7 void k_adjoint_coef_multi(ProjGeom g, const double* __restrict__ tab,
-                                                                         const float4* __restrict__ vox, SpecSet ss,
+                                                                         const VoxT* __restrict__ vox, SpecSet ss,
                                                                          double inv_xf, TileMap tm,
                                                                          const float* __restrict__ tg,
                                                                          float* __restrict__ coef,
@@ -747,12 +867,13 @@ __global__ __launch_bounds__(256) VCBCT_WAVES7 void k_adjoint_coef_multi(ProjGeo
     const bool active = j < g.H;
     const double c = ldg(tab, a), sn = ldg(tab, g.nAng + a), xc = ldg(tab, 2 * g.nAng + a),
                  yc = ldg(tab, 3 * g.nAng + a);
-    AccumL<float4> acc;
+    AccumL<VoxT> acc;
     acc.vox = vox;
     acc.Nz = g.Nz;
 #pragma unroll
     for (int m = 0; m < VCBCT_MAX_MAT; ++m) acc.L[m] = 0.0f;
     traverse_wave(g, c, sn, xc, yc, j, jj, active, acc, steps);
+    acc.finish();  // (the pipelined accumulators add their last gather here)
 
     double lossc = 0.0;
     if (active) {
@@ -764,16 +885,18 @@ __global__ __launch_bounds__(256) VCBCT_WAVES7 void k_adjoint_coef_multi(ProjGeo
         for (int m = 0; m < NCH; ++m) cacc[m] = 0.0;
         for (int s = 0; s < ss.nspec; ++s) {
             const int NE = ss.NE[s];
-            const double* __restrict__ mu = ss.mu[s];
+            constexpr bool CHV = !std::is_same<VoxT, float4>::value;  // channel volume
+            const int nl = CHV ? NCH : g.nmat;
+            const double* __restrict__ mu = CHV ? ss.ch[s] : ss.mu[s];
             const double* __restrict__ pdf = ss.pdf[s];
-            const float* __restrict__ muf = ss.muf[s];
+            const float* __restrict__ muf = CHV ? ss.chf[s] : ss.muf[s];
             const double* __restrict__ ch = ss.ch[s];
             double sw = 0.0, se = 0.0, mub[VCBCT_MAX_MAT] = {0, 0, 0, 0, 0, 0};
             for (int e = 0; e < NE; ++e) {
                 float tr = 0.0f;
 #pragma unroll
                 for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                    if (m < g.nmat) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
+                    if (m < nl) tr = fmaf(ldg(muf, m * NE + e), acc.L[m], tr);
                 const double pe = ldg(pdf, e);
                 const double we = pe * (double)exp2f(tr * k2);
                 sw += pe;
@@ -783,7 +906,7 @@ __global__ __launch_bounds__(256) VCBCT_WAVES7 void k_adjoint_coef_multi(ProjGeo
             }
             double p;
             if (se < 1e-30 * sw) {  // opaque ray: FP64 terms relative to the smallest exponent
-                p = mubar_opaque(acc.L, g.nmat, mu, pdf, NE, inv_xf, sw, mub, ch, NCH);
+                p = mubar_opaque(acc.L, nl, mu, pdf, NE, inv_xf, sw, mub, ch, NCH);
             } else {
                 p = -log(se / sw);
 #pragma unroll
@@ -812,20 +935,20 @@ __global__ __launch_bounds__(256) void k_adjoint_coef_saved(ProjGeom g, const fl
                                                             const double* __restrict__ pdf,
                                                             const float* __restrict__ muf,
                                                             const double* __restrict__ ch, int nch, int NE,
-                                                            double inv_xf, const float* __restrict__ gp,
+                                                            double inv_xf, int nl, const float* __restrict__ gp,
                                                             float* __restrict__ coef) {
     const size_t nrays = (size_t)g.nAng * g.H * g.W;
     for (size_t o = (size_t)blockIdx.x * 256 + threadIdx.x; o < nrays; o += (size_t)gridDim.x * 256) {
         float L[VCBCT_MAX_MAT];
 #pragma unroll
-        for (int m = 0; m < VCBCT_MAX_MAT; ++m) L[m] = m < g.nmat ? ldg(Ls, (size_t)m * nrays + o) : 0.0f;
+        for (int m = 0; m < VCBCT_MAX_MAT; ++m) L[m] = m < nl ? ldg(Ls, (size_t)m * nrays + o) : 0.0f;
         double sw = 0.0, se = 0.0, mub[VCBCT_MAX_MAT] = {0, 0, 0, 0, 0, 0};
         const float k2 = (float)(-inv_xf * 1.4426950408889634);
         for (int e = 0; e < NE; ++e) {
             float tr = 0.0f;
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-                if (m < g.nmat) tr = fmaf(ldg(muf, m * NE + e), L[m], tr);
+                if (m < nl) tr = fmaf(ldg(muf, m * NE + e), L[m], tr);
             const double pe = ldg(pdf, e);
             const double we = pe * (double)exp2f(tr * k2);
             sw += pe;
@@ -835,7 +958,7 @@ __global__ __launch_bounds__(256) void k_adjoint_coef_saved(ProjGeom g, const fl
                 if (m < nch) mub[m] += we * ldg(ch, m * NE + e);
         }
         if (se < 1e-30 * sw) {  // opaque ray: FP64 terms relative to the smallest exponent
-            (void)mubar_opaque(L, g.nmat, mu, pdf, NE, inv_xf, sw, mub, ch, nch);
+            (void)mubar_opaque(L, nl, mu, pdf, NE, inv_xf, sw, mub, ch, nch);
         } else {
 #pragma unroll
             for (int m = 0; m < VCBCT_MAX_MAT; ++m) mub[m] /= se;
@@ -925,7 +1048,7 @@ static void launch_fwd_t(int wpb, dim3 grid, hipStream_t st, const ProjGeom& g, 
                          float* aux) {
 #define VCBCT_LAUNCH_FWD(W)                                                                                  \
     hipLaunchKernelGGL((k_forward<M, T, W>), grid, dim3(64 * W), 0, st, g, tables, (const T*)vox, sp.mu, sp.pdf, \
-                       sp.muf, sp.pdff, sp.NE, inv_xf, a0, nA, tm, out, aux)
+                       sp.muf, sp.pdff, sp.NE, inv_xf, sp.nl, a0, nA, tm, out, aux)
     if constexpr (M == VCBCT_MODE_INTENSITY || M == VCBCT_MODE_LOGPROJ) {
         if (wpb == 2) VCBCT_LAUNCH_FWD(2);
         else if (wpb == 8) VCBCT_LAUNCH_FWD(8);
@@ -936,31 +1059,72 @@ static void launch_fwd_t(int wpb, dim3 grid, hipStream_t st, const ProjGeom& g, 
 #undef VCBCT_LAUNCH_FWD
 }
 
-static int launch_fwd_shape(const ProjGeom& g, const double* tables, const void* vox, int vox_is_f32,
+template <int M>
+static void launch_fwd_m(int kind, int wpb, dim3 grid, hipStream_t st, const ProjGeom& g, const double* tables,
+                         const void* vox, const Spectrum& sp, double inv_xf, int a0, int nA, const TileMap& tm,
+                         float* out, float* aux) {
+    if constexpr (M == VCBCT_MODE_INTENSITY || M == VCBCT_MODE_LOGPROJ) {
+        if (kind == VOX_CH2) {
+            launch_fwd_t<M, float2>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+            return;
+        }
+        if (kind == VOX_CH4) {
+            launch_fwd_t<M, Ch4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+            return;
+        }
+    }
+    if (kind == VOX_F32) {
+        if constexpr (M != VCBCT_MODE_SEGCOUNT)
+            launch_fwd_t<M, float4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+    } else {
+        launch_fwd_t<M, uint4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+    }
+}
+
+static int launch_fwd_shape(const ProjGeom& g, const double* tables, const void* vox, int vox_kind,
                             const Spectrum& sp, double inv_xf, int a0, int nA, float* out, int mode, float* aux,
                             hipStream_t st, const TileShape& s) {
     TileMap tm;
     const long nb = tile_map_host(g, nA, s, tm);
     if (nb < 0) return fail(VCBCT_EINVAL, "forward: launch too large");
+    const bool chv = vox_kind == VOX_CH2 || vox_kind == VOX_CH4;
+    if (chv && !(mode == VCBCT_MODE_INTENSITY || mode == VCBCT_MODE_LOGPROJ))
+        return fail(VCBCT_EINVAL, "forward: channel volumes serve the intensity and log projections only");
+    if (mode == VCBCT_MODE_SEGCOUNT && vox_kind != VOX_U16) return fail(VCBCT_EINVAL, "forward: bad volume kind");
     const int wpb = s.rgb * s.cols;
     const dim3 grid((unsigned)nb);
-    if (!vox_is_f32) {
-        if (mode == VCBCT_MODE_INTENSITY)
-            launch_fwd_t<VCBCT_MODE_INTENSITY, uint4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
-        else if (mode == VCBCT_MODE_LINEINT)
-            launch_fwd_t<VCBCT_MODE_LINEINT, uint4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
-        else if (mode == VCBCT_MODE_SEGCOUNT)
-            launch_fwd_t<VCBCT_MODE_SEGCOUNT, uint4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+    if (mode == VCBCT_MODE_INTENSITY)
+        launch_fwd_m<VCBCT_MODE_INTENSITY>(vox_kind, wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+    else if (mode == VCBCT_MODE_LINEINT)
+        launch_fwd_m<VCBCT_MODE_LINEINT>(vox_kind, wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+    else if (mode == VCBCT_MODE_SEGCOUNT)
+        launch_fwd_m<VCBCT_MODE_SEGCOUNT>(vox_kind, wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+    else
+        launch_fwd_m<VCBCT_MODE_LOGPROJ>(vox_kind, wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+    VCBCT_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_channels(const void* vox, int src_is_f32, float* dst, const ProjGeom& g, const ChannelMix& mix,
+                    hipStream_t st) {
+    const size_t nv = (size_t)g.N * g.N * g.Nz;
+    if (mix.nch < 1 || mix.nch > 4) return fail(VCBCT_EINVAL, "channels: %d channels (1 to 4)", mix.nch);
+    const unsigned nb = (unsigned)std::min<size_t>((nv + 255) / 256, 1 << 20);
+    if (nb == 0) return 0;
+    if (mix.nch <= 2) {
+        if (src_is_f32)
+            hipLaunchKernelGGL((k_channels<float4, float2>), dim3(nb), dim3(256), 0, st, (const float4*)vox,
+                               (float2*)dst, nv, mix, g.nmat);
         else
-            launch_fwd_t<VCBCT_MODE_LOGPROJ, uint4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+            hipLaunchKernelGGL((k_channels<uint4, float2>), dim3(nb), dim3(256), 0, st, (const uint4*)vox,
+                               (float2*)dst, nv, mix, g.nmat);
     } else {
-        if (mode == VCBCT_MODE_INTENSITY)
-            launch_fwd_t<VCBCT_MODE_INTENSITY, float4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out,
-                                                       aux);
-        else if (mode == VCBCT_MODE_LINEINT)
-            launch_fwd_t<VCBCT_MODE_LINEINT, float4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+        if (src_is_f32)
+            hipLaunchKernelGGL((k_channels<float4, Ch4>), dim3(nb), dim3(256), 0, st, (const float4*)vox, (Ch4*)dst,
+                               nv, mix, g.nmat);
         else
-            launch_fwd_t<VCBCT_MODE_LOGPROJ, float4>(wpb, grid, st, g, tables, vox, sp, inv_xf, a0, nA, tm, out, aux);
+            hipLaunchKernelGGL((k_channels<uint4, Ch4>), dim3(nb), dim3(256), 0, st, (const uint4*)vox, (Ch4*)dst, nv,
+                               mix, g.nmat);
     }
     VCBCT_HIP(hipGetLastError());
     return 0;
@@ -970,10 +1134,11 @@ static int launch_fwd_shape(const ProjGeom& g, const double* tables, const void*
 // Tile-shape autotuner.  Per key (kernel, geometry, launch parameters, device) the shape with the shortest
 // launch, timed on the caller's stream and buffers by coordinate descent: the (order, split) pairs at the
 // default block shape, then the other block shapes at the best pair, then the pairs again at the best block
-// shape (a candidate must beat the best so far by 1 %).  Every candidate computes the same values (a shape is a schedule only); a kernel that accumulates is
-// timed without its accumulator (the loss) or has it cleared afterwards (the gradient).  Nothing is timed
-// under stream capture (the default shape is recorded, not cached) or with VCBCT_NO_AUTOTUNE=1.  It replaces
-// hand-timed launch shapes (P/reprojectionOnGPU.cu:900-983 fixes them per kernel).
+// shape (a candidate must beat the best so far by 1 %).  Every candidate computes the same values (a shape is
+// a schedule only); a kernel that accumulates is timed without its accumulator (the loss) or has it cleared
+// afterwards (the gradient).  Nothing is timed under stream capture (the default shape is recorded, not cached)
+// or with VCBCT_NO_AUTOTUNE=1.  It replaces hand-timed launch shapes (P/reprojectionOnGPU.cu:900-983 fixes them
+// per kernel).
 static std::mutex g_ftune_mu;
 static std::unordered_map<std::string, TileShape> g_ftune;
 static std::string g_ftune_log;
@@ -1079,7 +1244,11 @@ static std::string tune_key(const char* kernel, const ProjGeom& g, int nA, const
     return key;
 }
 
-int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int vox_is_f32, const Spectrum& sp,
+static const char* vox_name(int kind) {
+    return kind == VOX_F32 ? "f32" : kind == VOX_CH2 ? "ch2" : kind == VOX_CH4 ? "ch4" : "u16";
+}
+
+int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int vox_kind, const Spectrum& sp,
                    double xfactor, int a0, int a1, float* out, int mode, float* aux, hipStream_t st,
                    TileShape* tile) {
     const int nA = a1 - a0;
@@ -1092,55 +1261,71 @@ int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int
         s = *tile;
     } else if (tunable) {
         char extra[32];
-        snprintf(extra, sizeof extra, "mode%d %s", mode, vox_is_f32 ? "f32" : "u16");
+        snprintf(extra, sizeof extra, "mode%d %s", mode, vox_name(vox_kind));
         const std::string key = tune_key("fwd", g, nA, extra);
         if (tuned_shape(key, st, s) == 0) {
             auto launch = [&](const TileShape& t) {
-                return launch_fwd_shape(g, tables, vox, vox_is_f32, sp, inv_xf, a0, nA, out, mode, aux, st, t);
+                return launch_fwd_shape(g, tables, vox, vox_kind, sp, inv_xf, a0, nA, out, mode, aux, st, t);
             };
             int rc = tune_tiles(key, st, kBlocksAll, 8, launch, s);
             if (rc) return rc;
         }
     }
     if (tile) *tile = s;
-    return launch_fwd_shape(g, tables, vox, vox_is_f32, sp, inv_xf, a0, nA, out, mode, aux, st, s);
+    return launch_fwd_shape(g, tables, vox, vox_kind, sp, inv_xf, a0, nA, out, mode, aux, st, s);
 }
 
 // The adjoint's kernels, one launch of shape s each (4-wave shapes).
-static int launch_coef(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double inv_xf,
-                       const float* tg, int mode, float* coef, double* loss, hipStream_t st, const TileShape& s) {
+static int launch_coef(const ProjGeom& g, const double* tables, const void* vox, int vox_kind, const Spectrum& sp,
+                       double inv_xf, const float* tg, int mode, float* coef, double* loss, hipStream_t st,
+                       const TileShape& s) {
     TileMap tm;
     const long nb = tile_map_host(g, g.nAng, s, tm);
     if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
-    if (mode == VCBCT_ADJ_LOSS)
-        hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_LOSS>, dim3((unsigned)nb), dim3(256), 0, st, g, tables,
-                           (const float4*)vox, sp.mu, sp.pdf, sp.muf, sp.ch, sp.nch, sp.NE, inv_xf, tm, tg, coef,
-                           loss);
-    else
-        hipLaunchKernelGGL(k_adjoint_coef<VCBCT_ADJ_UPSTREAM>, dim3((unsigned)nb), dim3(256), 0, st, g, tables,
-                           (const float4*)vox, sp.mu, sp.pdf, sp.muf, sp.ch, sp.nch, sp.NE, inv_xf, tm, tg, coef,
-                           loss);
+#define VCBCT_LAUNCH_COEF(M, T)                                                                               \
+    hipLaunchKernelGGL((k_adjoint_coef<M, T>), dim3((unsigned)nb), dim3(256), 0, st, g, tables, (const T*)vox,  \
+                       sp.mu, sp.pdf, sp.muf, sp.ch, sp.nch, sp.NE, inv_xf, sp.nl, tm, tg, coef, loss)
+    if (mode == VCBCT_ADJ_LOSS) {
+        if (vox_kind == VOX_CH2) VCBCT_LAUNCH_COEF(VCBCT_ADJ_LOSS, float2);
+        else if (vox_kind == VOX_CH4) VCBCT_LAUNCH_COEF(VCBCT_ADJ_LOSS, Ch4);
+        else VCBCT_LAUNCH_COEF(VCBCT_ADJ_LOSS, float4);
+    } else {
+        if (vox_kind == VOX_CH2) VCBCT_LAUNCH_COEF(VCBCT_ADJ_UPSTREAM, float2);
+        else if (vox_kind == VOX_CH4) VCBCT_LAUNCH_COEF(VCBCT_ADJ_UPSTREAM, Ch4);
+        else VCBCT_LAUNCH_COEF(VCBCT_ADJ_UPSTREAM, float4);
+    }
+#undef VCBCT_LAUNCH_COEF
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
 
-static int launch_coef_multi(const ProjGeom& g, const double* tables, const float* vox, const SpecSet& ss,
-                             double inv_xf, const float* targets, float* coef, double* loss, hipStream_t st,
-                             const TileShape& s) {
+static int launch_coef_multi(const ProjGeom& g, const double* tables, const void* vox, int vox_kind,
+                             const SpecSet& ss, double inv_xf, const float* targets, float* coef, double* loss,
+                             hipStream_t st, const TileShape& s) {
     TileMap tm;
     const long nb = tile_map_host(g, g.nAng, s, tm);
     if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
-#define VCBCT_LAUNCH_MULTI(C)                                                                                   \
-    hipLaunchKernelGGL(k_adjoint_coef_multi<C>, dim3((unsigned)nb), dim3(256), 0, st, g, tables,               \
-                       (const float4*)vox, ss, inv_xf, tm, targets, coef, loss)
-    switch (ss.nch) {
-        case 1: VCBCT_LAUNCH_MULTI(1); break;
-        case 2: VCBCT_LAUNCH_MULTI(2); break;
-        case 3: VCBCT_LAUNCH_MULTI(3); break;
-        case 4: VCBCT_LAUNCH_MULTI(4); break;
-        case 5: VCBCT_LAUNCH_MULTI(5); break;
-        case 6: VCBCT_LAUNCH_MULTI(6); break;
-        default: return fail(VCBCT_EINVAL, "adjoint: %d channels", ss.nch);
+#define VCBCT_LAUNCH_MULTI(C, T)                                                                                \
+    hipLaunchKernelGGL((k_adjoint_coef_multi<C, T>), dim3((unsigned)nb), dim3(256), 0, st, g, tables,          \
+                       (const T*)vox, ss, inv_xf, tm, targets, coef, loss)
+    if (vox_kind == VOX_CH2) {
+        if (ss.nch == 1) VCBCT_LAUNCH_MULTI(1, float2);
+        else if (ss.nch == 2) VCBCT_LAUNCH_MULTI(2, float2);
+        else return fail(VCBCT_EINVAL, "adjoint: %d channels in a two-channel volume", ss.nch);
+    } else if (vox_kind == VOX_CH4) {
+        if (ss.nch == 3) VCBCT_LAUNCH_MULTI(3, Ch4);
+        else if (ss.nch == 4) VCBCT_LAUNCH_MULTI(4, Ch4);
+        else return fail(VCBCT_EINVAL, "adjoint: %d channels in a four-channel volume", ss.nch);
+    } else {
+        switch (ss.nch) {
+            case 1: VCBCT_LAUNCH_MULTI(1, float4); break;
+            case 2: VCBCT_LAUNCH_MULTI(2, float4); break;
+            case 3: VCBCT_LAUNCH_MULTI(3, float4); break;
+            case 4: VCBCT_LAUNCH_MULTI(4, float4); break;
+            case 5: VCBCT_LAUNCH_MULTI(5, float4); break;
+            case 6: VCBCT_LAUNCH_MULTI(6, float4); break;
+            default: return fail(VCBCT_EINVAL, "adjoint: %d channels", ss.nch);
+        }
     }
 #undef VCBCT_LAUNCH_MULTI
     VCBCT_HIP(hipGetLastError());
@@ -1187,40 +1372,40 @@ static int scatter_shape(const ProjGeom& g, const double* tables, const float* c
     return launch_zero(grad_vox, (size_t)nch * g.N * g.N * g.Nz * sizeof(float), st);
 }
 
-int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double xfactor,
-                   const float* target_or_gp, int mode, float* coef, float* grad_vox, double* loss, hipStream_t st,
-                   const TileShape& forced) {
+int launch_adjoint(const ProjGeom& g, const double* tables, const void* vox, int vox_kind, const Spectrum& sp,
+                   double xfactor, const float* target_or_gp, int mode, float* coef, float* grad_vox, double* loss,
+                   hipStream_t st, const TileShape& forced) {
     const double inv_xf = 1.0 / xfactor;
-    char extra[48];
-    snprintf(extra, sizeof extra, "mode%d nch%d NE%d", mode, sp.nch, sp.NE);
+    char extra[64];
+    snprintf(extra, sizeof extra, "mode%d nch%d NE%d %s", mode, sp.nch, sp.NE, vox_name(vox_kind));
     TileShape sc, ss;
     bool tuned_now = false;
     // tuning launches pass no loss accumulator
     auto launch = [&](const TileShape& t) {
-        return launch_coef(g, tables, vox, sp, inv_xf, target_or_gp, mode, coef, nullptr, st, t);
+        return launch_coef(g, tables, vox, vox_kind, sp, inv_xf, target_or_gp, mode, coef, nullptr, st, t);
     };
     int rc = adjoint_shape(forced, tune_key("adj-coef", g, g.nAng, extra), st, launch, sc, tuned_now);
-    if (!rc) rc = launch_coef(g, tables, vox, sp, inv_xf, target_or_gp, mode, coef, loss, st, sc);
+    if (!rc) rc = launch_coef(g, tables, vox, vox_kind, sp, inv_xf, target_or_gp, mode, coef, loss, st, sc);
     if (!rc) rc = scatter_shape(g, tables, coef, sp.nch, grad_vox, st, forced, ss);
     if (!rc) rc = launch_scatter(g, tables, coef, sp.nch, grad_vox, st, ss);
     return rc;
 }
 
-int launch_adjoint_multi(const ProjGeom& g, const double* tables, const float* vox, const SpecSet& ss,
+int launch_adjoint_multi(const ProjGeom& g, const double* tables, const void* vox, int vox_kind, const SpecSet& ss,
                          double xfactor, const float* targets, float* coef, float* grad_vox, double* loss,
                          hipStream_t st, const TileShape& forced) {
     const double inv_xf = 1.0 / xfactor;
-    char extra[48];
+    char extra[64];
     int ne = 0;
     for (int s = 0; s < ss.nspec; ++s) ne += ss.NE[s];
-    snprintf(extra, sizeof extra, "nspec%d nch%d NE%d", ss.nspec, ss.nch, ne);
+    snprintf(extra, sizeof extra, "nspec%d nch%d NE%d %s", ss.nspec, ss.nch, ne, vox_name(vox_kind));
     TileShape sc, sg;
     bool tuned_now = false;
     auto launch = [&](const TileShape& t) {
-        return launch_coef_multi(g, tables, vox, ss, inv_xf, targets, coef, nullptr, st, t);
+        return launch_coef_multi(g, tables, vox, vox_kind, ss, inv_xf, targets, coef, nullptr, st, t);
     };
     int rc = adjoint_shape(forced, tune_key("adj-multi", g, g.nAng, extra), st, launch, sc, tuned_now);
-    if (!rc) rc = launch_coef_multi(g, tables, vox, ss, inv_xf, targets, coef, loss, st, sc);
+    if (!rc) rc = launch_coef_multi(g, tables, vox, vox_kind, ss, inv_xf, targets, coef, loss, st, sc);
     if (!rc) rc = scatter_shape(g, tables, coef, ss.nch, grad_vox, st, forced, sg);
     if (!rc) rc = launch_scatter(g, tables, coef, ss.nch, grad_vox, st, sg);
     return rc;
@@ -1232,7 +1417,7 @@ int launch_adjoint_saved(const ProjGeom& g, const double* tables, const float* L
     const size_t nrays = (size_t)g.nAng * g.H * g.W;
     const unsigned nbc = (unsigned)std::min<size_t>((nrays + 255) / 256, 65536);
     hipLaunchKernelGGL(k_adjoint_coef_saved, dim3(nbc), dim3(256), 0, st, g, Ls, sp.mu, sp.pdf, sp.muf, sp.ch, sp.nch,
-                       sp.NE, 1.0 / xfactor, gp, coef);
+                       sp.NE, 1.0 / xfactor, sp.nl, gp, coef);
     VCBCT_HIP(hipGetLastError());
     TileShape sg;
     int rc = scatter_shape(g, tables, coef, sp.nch, grad_vox, st, forced, sg);

@@ -316,6 +316,7 @@ extern "C" int vcbct_projector_set_volume_labels(vcbct_projector* p, const uint8
                        nz, col0, row0, slice0, (const uint4*)lut, keep, (uint4*)p->d_vox_u16, g.N, g.Nz, pitch);
     VCBCT_HIP(hipGetLastError());
     p->have_u16 = 1;
+    p->chv_key = 0;
     return 0;
 }
 

@@ -86,7 +86,16 @@ struct Spectrum {
     // ch = mu, nch = nmat, B = identity when the table has no lower-rank form (see spectral_basis).
     const double* ch;   // [nch][NE]
     int nch;
+    // The forward's spectral sum over the line integrals of the traversed volume: nl = nmat with the tables
+    // mu / muf above for material volumes; for channel volumes (VOX_CH2 / VOX_CH4) the caller passes the channel
+    // tables as mu (FP64) and muf (FP32) and nl = nch.  chf: the channel table in FP32 (as muf for mu).
+    int nl;
+    const float* chf;
 };
+// Voxel records of the traversed volume ([y][x][z][record]): the u16 elemental maps (8 x u16), the FP32 maps
+// (8 x FP32), or the spectral channels c_k = sum_m B_mk rho_m of either in FP32, two (float2) or four (float4)
+// to a record.
+enum VoxKind { VOX_U16 = 0, VOX_F32 = 1, VOX_CH2 = 2, VOX_CH4 = 3 };
 
 // Several spectra over the same rays (dual-energy decomposition): each spectrum's tables, and the channels of
 // the joint basis of all of them (the basis of the concatenated attenuation tables).
@@ -96,6 +105,7 @@ struct SpecSet {
     const double* pdf[VCBCT_MAX_SPECTRA];  // [NE_s]
     const float* muf[VCBCT_MAX_SPECTRA];   // [nmat][NE_s] as given
     const double* ch[VCBCT_MAX_SPECTRA];   // [nch][NE_s], the channel table of spectrum s
+    const float* chf[VCBCT_MAX_SPECTRA];   // the same in FP32
     int NE[VCBCT_MAX_SPECTRA];
     int nspec, nch;
 };
@@ -145,7 +155,7 @@ struct BatchProblem {
     float* out;
     double inv_xf;
     double xfactor;     // as given (the sequence form launches the plain forward with it)
-    TileShape fixed;    // the projector's fixed tile shape when it was added (rgb == 0: autotuned)
+    vcbct_projector* proj;  // host pointer (the sequence form launches through the projector)
     int NE, vox_is_f32;
     // persistent form: the XCD queues hold (row group, angle, column) items; with split 0 queue q holds columns
     // [q pcols, (q + 1) pcols) of every angle (pang = nAng), with split 1 angles [q pang, (q + 1) pang) of
@@ -167,22 +177,35 @@ int launch_untranspose_f32(const float* src, float* dst, const ProjGeom& g, hipS
 int launch_untranspose_mix(const float* src, float* dst, const ProjGeom& g, const ChannelMix& mix, hipStream_t st);
 // Forward launch over angles [a0, a1).  tile: in, the caller's fixed shape (rgb > 0) or none (rgb == 0: the
 // autotuner's choice for this geometry, timed on the first launch outside stream capture); out, the shape used.
-int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int vox_is_f32, const Spectrum& sp,
+int launch_forward(const ProjGeom& g, const double* tables, const void* vox, int vox_kind, const Spectrum& sp,
                    double xfactor, int a0, int a1, float* out, int mode, float* aux, hipStream_t st,
                    TileShape* tile = nullptr);
+// The channel volume of a material volume (u16 or FP32 maps in the [y][x][z][8] layout): per voxel
+// c_k = sum_m mix.b[m][k] rho_m for k < mix.nch, as float2 (nch <= 2) or float4 records (nch <= 4), zero-padded.
+int launch_channels(const void* vox, int src_is_f32, float* dst, const ProjGeom& g, const ChannelMix& mix,
+                    hipStream_t st);
+// The forward launch of a projector (intensity or log projection, aux: the saved line integrals) from its u16
+// volume or its FP32 maps (use_f32), over their channel volume when its spectrum allows it (built on demand;
+// rebuild_channels: regardless of the cached one), with the projector's fixed or the autotuned tile shape.
+int projector_forward(vcbct_projector* p, int mode, bool use_f32, float* out, float* aux, double xfactor, int a0,
+                      int a1, hipStream_t st, bool rebuild_channels);
 // The forward autotuner's record (one line per tuned geometry).
 std::string forward_tune_log();
 // The adjoint launches take the caller's 4-wave tile shape for their traversal kernels (forced.rgb > 0), or
-// autotune them per geometry (forced.rgb == 0) as launch_forward does.
-int launch_adjoint(const ProjGeom& g, const double* tables, const float* vox, const Spectrum& sp, double xfactor,
+// autotune them per geometry (forced.rgb == 0) as launch_forward does.  vox: the FP32 maps (VOX_F32, the
+// spectral sum over sp.mu / sp.muf) or their channel volume (VOX_CH2 / VOX_CH4, over the channel tables).
+int launch_adjoint(const ProjGeom& g, const double* tables, const void* vox, int vox_kind, const Spectrum& sp,
+                   double xfactor,
                    const float* target_or_gp, int mode, float* coef, float* grad_vox, double* loss, hipStream_t st,
                    const TileShape& forced);
+// Ls: the sp.nl line integrals per ray logproj_save kept (materials, or channels when sp holds the channel
+// tables in mu / muf).
 int launch_adjoint_saved(const ProjGeom& g, const double* tables, const float* Ls, const Spectrum& sp,
                          double xfactor, const float* gp, float* coef, float* grad_vox, hipStream_t st,
                          const TileShape& forced);
 // loss = sum_s sum_rays (p_s - t_s)^2 over the spectra of `ss` and its gradient: one traversal per ray for all
 // spectra (their line integrals are the same), the coefficients summed over the spectra, one scatter.
-int launch_adjoint_multi(const ProjGeom& g, const double* tables, const float* vox, const SpecSet& ss,
+int launch_adjoint_multi(const ProjGeom& g, const double* tables, const void* vox, int vox_kind, const SpecSet& ss,
                          double xfactor, const float* targets, float* coef, float* grad_vox, double* loss,
                          hipStream_t st, const TileShape& forced);
 
@@ -243,17 +266,27 @@ struct vcbct_projector {
     float* d_muf;          // [nmat][NE] as given
     float* d_pdff;         // [NE] as given
     double* d_ch;          // the adjoint's channel table [nch][NE] (the basis form of mu; unused when nch == nmat)
+    float* d_chf;          // the same in FP32 (when nch == nmat: a copy of the table as given)
     vcbct::ChannelMix mix; // channels -> maps
+    int fwd_ch;            // the forward and the coefficient kernels traverse the channel volume (nch <= 4)
     int NE;
     int have_u16, have_f32;
     // spectra of the multi-spectrum loss (vcbct_projector_set_spectra), with their joint channel basis
     struct {
         double *d_mu, *d_pdf, *d_ch;
-        float* d_muf;
+        float *d_muf, *d_chf;
         int NE;
     } ms[VCBCT_MAX_SPECTRA];
     int nspec_multi;
     vcbct::ChannelMix mix_multi;
+    int fwd_ch_multi;      // the multi-spectrum coefficient kernel traverses the joint basis's channel volume
+    // The channel volume (float2 or float4 records), built on demand from the volume and the basis it was
+    // asked for; chv_key names what it holds (0: nothing valid; 1: the u16 volume in the spectrum's basis; 2: the
+    // FP32 maps in the spectrum's basis; 3: the FP32 maps in the spectra's joint basis).  Setting a volume or a
+    // spectrum clears it.
+    float* d_chv;
+    size_t chv_bytes;
+    int chv_key;
     vcbct::TileShape tile;       // fixed by vcbct_projector_set_tile (rgb == 0: autotuned)
     vcbct::TileShape tile_used;  // the shape of the last intensity / log-projection launch
     vcbct::TileShape adj_tile;   // the adjoint kernels' shape, fixed by vcbct_projector_set_adjoint_tile (4 waves)

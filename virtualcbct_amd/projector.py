@@ -186,6 +186,14 @@ class Projector:
         return grad, loss
 
     @property
+    def forward_channels(self):
+        """Spectral channels the forward traverses in place of the maps (2 or 4 FP32 per voxel), 0 for the maps."""
+        rc = lib().vcbct_projector_forward_channels(self.h)
+        if rc < 0:
+            check(rc)
+        return rc
+
+    @property
     def adjoint_channels(self):
         """Channels the adjoint scatters: num_material, or the rank of the attenuation table's spectral basis."""
         rc = lib().vcbct_projector_adjoint_channels(self.h)
