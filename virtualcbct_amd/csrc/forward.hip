@@ -723,11 +723,12 @@ int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xc
 // (rows closer than slices).  So the lanes of a slot that name the same voxel are first combined: cells
 // form runs over consecutive lanes (the lanes of a wave share the xy cell of a slot and their z is monotone
 // in the row); the run head sums its run with shuffles and alone issues the atomics.
+// NCH: the scattered channels (compile time, so the run merge and the atomics touch only those).
+template <int NCH>
 struct ScatterG {
     float* __restrict__ gvox;
-    float c[VCBCT_MAX_MAT];
-    size_t plane;  // N * N * Nz: one material plane of the [mat][y][x][z] scatter layout
-    int nmat;
+    float c[NCH];
+    size_t plane;  // N * N * Nz: one channel plane of the [channel][y][x][z] scatter layout
     __device__ __forceinline__ void seg(const Seg& s) {
         const int lane = threadIdx.x & 63;
         const bool ok = s.ok;
@@ -742,22 +743,22 @@ struct ScatterG {
         const int len = after ? __ffsll((long long)after) : 64 - lane;
         const int maxlen = wave_max(ok && start ? len : 1);
         const float rf = (float)s.rr;
-        float v[VCBCT_MAX_MAT];
+        float v[NCH];
 #pragma unroll
-        for (int m = 0; m < VCBCT_MAX_MAT; ++m) v[m] = ok ? c[m] * rf : 0.f;
+        for (int m = 0; m < NCH; ++m) v[m] = ok ? c[m] * rf : 0.f;
         for (int i = 1; i < maxlen; ++i) {
 #pragma unroll
-            for (int m = 0; m < VCBCT_MAX_MAT; ++m) {
+            for (int m = 0; m < NCH; ++m) {
                 const float w = __shfl_down(v[m], i, 64);
                 if (i < len) v[m] += w;
             }
         }
         if (!(ok && start)) return;
-        // planar layout: the heads (neighbouring z of one (x, y)) hit a contiguous run of one material plane
+        // planar layout: the heads (neighbouring z of one (x, y)) hit a contiguous run of one channel plane
         float* p = gvox + (size_t)s.cell;
 #pragma unroll
-        for (int m = 0; m < VCBCT_MAX_MAT; ++m)
-            if (m < nmat && v[m] != 0.f) atomicAdd(p + m * plane, v[m]);
+        for (int m = 0; m < NCH; ++m)
+            if (v[m] != 0.f) atomicAdd(p + m * plane, v[m]);
     }
     __device__ __forceinline__ void step(const Seg& a, const Seg& b, const Seg& c2) {
         seg(a);
@@ -970,9 +971,10 @@ __global__ __launch_bounds__(256) void k_adjoint_coef_saved(ProjGeom g, const fl
     }
 }
 
+template <int NCH>
 __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_scatter(ProjGeom g, const double* __restrict__ tab,
                                                                      TileMap tm, const float* __restrict__ coef,
-                                                                     int nch, float* __restrict__ gvox) {
+                                                                     float* __restrict__ gvox) {
     __shared__ WaveLds step_tabs[4];
     WaveLds* steps = &step_tabs[threadIdx.x >> 6];
     const Tile t = tile_map(tm, g.nAng);
@@ -982,16 +984,15 @@ __global__ __launch_bounds__(256) VCBCT_WAVES8 void k_adjoint_scatter(ProjGeom g
     const int a = t.a;
     if (jj >= g.W || a >= g.nAng || t.rg * 64 >= g.H) return;  // wave out of range: uniform exit
     const bool active = j < g.H;
-    ScatterG sc;
+    ScatterG<NCH> sc;  // the scattered channels: the maps, or their spectral basis
     sc.gvox = gvox;
-    sc.nmat = nch;  // the scattered channels (the maps, or their spectral basis)
     sc.plane = (size_t)g.N * g.N * g.Nz;
     bool any = false;
     const size_t nrays = (size_t)g.nAng * g.H * g.W;
     const size_t o = (size_t)a * g.H * g.W + (size_t)(g.H - 1 - (active ? j : 0)) * g.W + jj;
 #pragma unroll
-    for (int m = 0; m < VCBCT_MAX_MAT; ++m) {
-        sc.c[m] = (active && m < nch) ? ldg(coef, (size_t)m * nrays + o) : 0.f;
+    for (int m = 0; m < NCH; ++m) {
+        sc.c[m] = active ? ldg(coef, (size_t)m * nrays + o) : 0.f;
         any = any || sc.c[m] != 0.f;
     }
     if (!__any(any)) return;  // nothing to scatter for this wave (e.g. zero upstream gradient)
@@ -1337,7 +1338,18 @@ static int launch_scatter(const ProjGeom& g, const double* tables, const float* 
     TileMap tm;
     const long nb = tile_map_host(g, g.nAng, s, tm);
     if (nb < 0) return fail(VCBCT_EINVAL, "adjoint: launch too large");
-    hipLaunchKernelGGL(k_adjoint_scatter, dim3((unsigned)nb), dim3(256), 0, st, g, tables, tm, coef, nch, grad_vox);
+#define VCBCT_LAUNCH_SCATTER(C)                                                                                 \
+    hipLaunchKernelGGL(k_adjoint_scatter<C>, dim3((unsigned)nb), dim3(256), 0, st, g, tables, tm, coef, grad_vox)
+    switch (nch) {
+        case 1: VCBCT_LAUNCH_SCATTER(1); break;
+        case 2: VCBCT_LAUNCH_SCATTER(2); break;
+        case 3: VCBCT_LAUNCH_SCATTER(3); break;
+        case 4: VCBCT_LAUNCH_SCATTER(4); break;
+        case 5: VCBCT_LAUNCH_SCATTER(5); break;
+        case 6: VCBCT_LAUNCH_SCATTER(6); break;
+        default: return fail(VCBCT_EINVAL, "adjoint: %d channels", nch);
+    }
+#undef VCBCT_LAUNCH_SCATTER
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
