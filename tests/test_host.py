@@ -38,7 +38,7 @@ def test_compat_library_exports_reference_entry_points():
     if not os.path.exists(path):
         pytest.skip("compat shim not built")
     Lc = C.CDLL(path)
-    # C++-linkage names of initializeGPU / reprojectionOnGPU / terminateGPU (P/reprojectionOnGPU.h:76-93) and
+    # C++-linkage names of initializeGPU / reprojectionOnGPU / terminateGPU (P/reprojectionOnGPU.h:2-19) and
     # prepareConstMap / filterSinogram / reconstructImage (F/filterSinogram.h, F/reconstructImage.h)
     for mangled in ("_Z13initializeGPUididiiiPdS_S_S_dddddddi", "_Z12terminateGPUv",
                     "_Z17reprojectionOnGPUiiiPtPfS0_S0_ddiiiiPdS1_S1_S1_S1_dS1_dddddddS0_S0_iiPc",
