@@ -2,7 +2,7 @@
  * extern "C" entry points over the reference's own FDK CPU code, compiled from the reference sources
  * where they lie (recipe: oracle/Makefile).  TEST INFRASTRUCTURE ONLY: used to pin the oracle and as the
  * timed CPU baseline.  The functions called are the reference's, unmodified:
- *   prepareConstMap / filterSinogram   F/filterSinogram.cpp:65-186
+ *   prepareConstMap / filterSinogram   F/filterSinogram.cpp:10-65, 67-131
  *   reconstructImage                   F/reconstructImage.cpp:10-184
  *   prepareMaskData, prepareTrigonometricTable  F/misc.cpp:9-37, 181-200
  *   prepareAngles                      F/misc.cpp:87-180
