@@ -193,8 +193,8 @@ struct AccumL<uint4> {
         add(p0, r0);
         r0 = 0.f;
     }
-    __device__ __forceinline__ uint3 ld(int cell) const {
-        const uint4 q = ldg(vox, (size_t)cell);
+    __device__ __forceinline__ uint3 ld(int cell) const {  // (cells are nonnegative)
+        const uint4 q = ldg(vox, (size_t)(unsigned)cell);
         return make_uint3(q.x, q.y, q.z);
     }
     __device__ __forceinline__ void add(const uint3& q, float rr) {
@@ -228,7 +228,7 @@ struct AccumL<float4> {
     }
     __device__ __forceinline__ void finish() {}
     __device__ __forceinline__ Q ld(const Seg& s) const {
-        const size_t i = 2 * (size_t)s.cell;
+        const size_t i = 2 * (size_t)(unsigned)s.cell;
         const float4 b = ldg(vox, i + 1);
         return Q{ldg(vox, i), make_float2(b.x, b.y)};
     }
@@ -262,9 +262,9 @@ struct AccumL<float2> {
     __device__ __forceinline__ void step(const Seg& s0, const Seg& s1, const Seg& s2) {
         add(p0, r0);
         float2 q1, q2;
-        if (s1.ok) q1 = ldg(vox, (size_t)s1.cell);
-        if (s2.ok) q2 = ldg(vox, (size_t)s2.cell);
-        p0 = ldg(vox, (size_t)(s0.ok ? s0.cell : 0));
+        if (s1.ok) q1 = ldg(vox, (size_t)(unsigned)s1.cell);
+        if (s2.ok) q2 = ldg(vox, (size_t)(unsigned)s2.cell);
+        p0 = ldg(vox, (size_t)(unsigned)(s0.ok ? s0.cell : 0));
         r0 = s0.ok ? (float)s0.rr : 0.f;
         if (s1.ok) add(q1, (float)s1.rr);
         if (s2.ok) add(q2, (float)s2.rr);
@@ -290,9 +290,9 @@ struct AccumL<Ch4> {
     __device__ __forceinline__ void step(const Seg& s0, const Seg& s1, const Seg& s2) {
         add(p0, r0);
         Ch4 q1, q2;
-        if (s1.ok) q1 = ldg(vox, (size_t)s1.cell);
-        if (s2.ok) q2 = ldg(vox, (size_t)s2.cell);
-        p0 = ldg(vox, (size_t)(s0.ok ? s0.cell : 0));
+        if (s1.ok) q1 = ldg(vox, (size_t)(unsigned)s1.cell);
+        if (s2.ok) q2 = ldg(vox, (size_t)(unsigned)s2.cell);
+        p0 = ldg(vox, (size_t)(unsigned)(s0.ok ? s0.cell : 0));
         r0 = s0.ok ? (float)s0.rr : 0.f;
         if (s1.ok) add(q1, (float)s1.rr);
         if (s2.ok) add(q2, (float)s2.rr);
@@ -754,7 +754,7 @@ struct ScatterG {
         }
         if (!(ok && start)) return;
         // planar layout: the heads (neighbouring z of one (x, y)) hit a contiguous run of one channel plane
-        float* p = gvox + (size_t)s.cell;
+        float* p = gvox + (size_t)(unsigned)s.cell;
 #pragma unroll
         for (int m = 0; m < NCH; ++m)
             if (v[m] != 0.f) atomicAdd(p + m * plane, v[m]);

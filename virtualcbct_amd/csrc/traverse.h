@@ -475,7 +475,9 @@ __host__ __device__ __forceinline__ bool any_lane(bool b) {
 // Per-lane constants of the table walk: the z slope and its inverse, the segment length per unit of d, and
 // the z offset of the midpoint rule.
 struct LaneZ {
-    double hdz;   // dz / 2: zm = (a + b) * hdz + sz for the piece [a, b]
+    // the slice index of the piece [a, b] is (int)(zm * ninv_t + nzm) with zm = (a + b) dz / 2 + sz; folded
+    // into one FMA of (a + b): zA = dz / 2 * ninv_t, zB = sz * ninv_t + nzm (per-lane constants)
+    double zA, zB;
     double K;     // segment length per |du|
 };
 
@@ -484,8 +486,7 @@ __host__ __device__ __forceinline__ Seg make_piece(const ProjGeom& g, const RayC
                                                    const LaneZ& lz, double a, double b, int cb, bool on) {
 #pragma clang fp contract(fast)
     Seg sg;
-    const double zm = (a + b) * lz.hdz + r.sz;
-    const int zmc = (int)(zm * c.ninv_t + c.nzm);
+    const int zmc = (int)((a + b) * lz.zA + lz.zB);
     sg.ok = on && (unsigned)zmc < (unsigned)g.Nz;
     sg.cell = (int)((unsigned)cb + (unsigned)zmc);
     sg.rr = fabs(b - a) * lz.K;
@@ -530,7 +531,8 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
     const bool on = active && r.hit;
     const ChargeC c = charge_consts(g, r);
     LaneZ lz;
-    lz.hdz = 0.5 * r.dz;
+    lz.zA = 0.5 * r.dz * c.ninv_t;
+    lz.zB = r.sz * c.ninv_t + c.nzm;
     lz.K = r.K;
 
     double u0 = r.uin;
@@ -548,7 +550,7 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
         b0 = plane_u(r, k - 1, g.s) - r.su;
     }
     const double inv_t = g.inv_t;
-    int zmaprev = (int)(((b0 + b0) * lz.hdz + r.sz) * inv_t);
+    int zmaprev = (int)((b0 * r.dz + r.sz) * inv_t);
     const double dt = r.dirz * g.t;
     const Seg none{0, 0.0, false};
     StepRec* tab = lds->steps;
@@ -597,7 +599,7 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
 #else
             const int flags = tab[i].flags;
 #endif
-            const double zk = (dk + dk) * lz.hdz + r.sz;  // dk * dz + sz exactly (doubling and halving are exact)
+            const double zk = dk * r.dz + r.sz;  // (one FMA)
             const int zma = (int)(zk * inv_t);
             if (flags & STEP_SKIP) {  // nothing charged in this step: only the z state moves on
                 b0 = dk;
