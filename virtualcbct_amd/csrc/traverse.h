@@ -627,7 +627,8 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
             const bool tie = splitv && splitz && chz == chv;
             const bool z0 = splitz && !tie && (!splitv || chz > chv);  // z crossing inside the first piece
             const bool z1 = splitz && !tie && !z0;                     // inside the second
-            const double a1 = splitv ? dv : dk;                        // first piece [b0, a1]
+            // first piece [b0, a1]: a1 = dv, as the table's dv equals dk when the minor crossing splits nothing
+            const double a1 = dv;
             if (on && (z0 || z1)) {  // the piece beyond the z crossing is held
                 *hp = HeldPiece{dzc, z0 ? a1 : dk, z0 ? cb0 : cb1, z0 ? (flags & (STEP_OK0 | STEP_CIRC))
                                                                       : ((flags & STEP_OK1) ? STEP_OK0 : 0) |
@@ -651,9 +652,14 @@ __host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const
             bool on1c = on1;
             if (circ) {  // a piece near the circle's boundary: the midpoint tests
                 on0 = on0 && in_circle(r, c, b0, e0);
-                on1c = on1c && in_circle(r, c, dv, e1);
+                if (splitv) on1c = on1c && in_circle(r, c, dv, e1);
             }
-            vis.step(make_piece(g, r, c, lz, b0, e0, cb0, on0), make_piece(g, r, c, lz, dv, e1, cb1, on1c), none);
+            // the second xy piece exists only where the minor crossing splits the step (wave-uniform): its slice,
+            // cell and length are formed only then
+            const Seg q0 = make_piece(g, r, c, lz, b0, e0, cb0, on0);
+            Seg q1 = none;
+            if (splitv) q1 = make_piece(g, r, c, lz, dv, e1, cb1, on1c);
+            vis.step(q0, q1, none);
             b0 = dk;
             zmaprev = zma;
         }
