@@ -29,9 +29,13 @@ namespace vcbct {
 // gathers, and more resident waves hide that better than spilling a few registers costs: on the
 // 512x512x128 / 360-angle workload, k_forward takes 47.1 ms at its natural 75 VGPRs (6 waves per SIMD),
 // 45.0 ms held to 7 waves and 43.1 ms held to 8 (64 VGPRs, 56 bytes per lane spilled to scratch).  The
-// persistent batch kernel is fastest held to 7.
+// persistent batch kernel is fastest held to 7.  With the later step loop (76 VGPRs unbounded) the intensity
+// form takes 22.4 ms at 6 waves and 21.1 ms at both 7 and 8, while the log projection of the 256x256x64 /
+// 90-angle workload takes 1.58 ms at 7 (72 VGPRs, 28 bytes spilled) against 1.74 ms at 8: the projection
+// forms other than the intensity are held to 7.
 #define VCBCT_WAVES8 __attribute__((amdgpu_waves_per_eu(8, 8)))
 #define VCBCT_WAVES7 __attribute__((amdgpu_waves_per_eu(7, 8)))
+#define VCBCT_WAVES_FWD(MODE) __attribute__((amdgpu_waves_per_eu((MODE) == VCBCT_MODE_INTENSITY ? 8 : 7, 8)))
 
 // ------------------------------------------------------------------------------------------------
 // Volume re-layout.  Reference u16 [z][mat][y][x] -> device [y][x][z][8] (16-byte voxel, all materials;
@@ -594,7 +598,7 @@ __device__ __forceinline__ void forward_wave(const ProjGeom& g, const double* __
 
 // WPB waves per block (TileShape: rgb * cols); the block's waves share nothing but the LDS allocation.
 template <int MODE, class VoxT, int WPB>
-__global__ __launch_bounds__(64 * WPB) VCBCT_WAVES8 void k_forward(ProjGeom g, const double* __restrict__ tab,
+__global__ __launch_bounds__(64 * WPB) VCBCT_WAVES_FWD(MODE) void k_forward(ProjGeom g, const double* __restrict__ tab,
                                                                  const VoxT* __restrict__ vox,
                                                                  const double* __restrict__ mu,
                                                                  const double* __restrict__ pdf,
