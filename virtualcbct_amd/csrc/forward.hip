@@ -725,7 +725,7 @@ int launch_forward_batch(const BatchProblem* d_problems, int np, const int* d_xc
 // per address as much as 64 scattered lanes), and neighbouring detector rows often charge the same voxel
 // (rows closer than slices).  So the lanes of a slot that name the same voxel are first combined: cells
 // form runs over consecutive lanes (the lanes of a wave share the xy cell of a slot and their z is monotone
-// in the row); the run head sums its run with shuffles and alone issues the atomics.
+// in the row); the run head sums its run (a segmented sum in log steps) and alone issues the atomics.
 // NCH: the scattered channels (compile time, so the run merge and the atomics touch only those).
 template <int NCH>
 struct ScatterG {
@@ -741,20 +741,27 @@ struct ScatterG {
         const bool start = lane == 0 || prev != key;
         // lanes outside the current exec mask (a caller's divergent branch) end every run
         const unsigned long long starts = __ballot(start) | ~__ballot(true);
-        // run [lane, next start): its length, for run heads
+        // [lane, next start): the rest of the lane's run
         const unsigned long long after = lane == 63 ? 0ull : (starts >> (lane + 1));
         const int len = after ? __ffsll((long long)after) : 64 - lane;
-        const int maxlen = wave_max(ok && start ? len : 1);
         const float rf = (float)s.rr;
         float v[NCH];
 #pragma unroll
         for (int m = 0; m < NCH; ++m) v[m] = ok ? c[m] * rf : 0.f;
-        for (int i = 1; i < maxlen; ++i) {
+        // Segmented sum in log steps: after the step of offset d every lane holds the sum of its run's values
+        // over [lane, min(lane + 2d, run end)), so the heads hold their runs' sums once 2d reaches the longest
+        // run.  cont (wave-uniform) marks the lanes that continue a run; after the step of offset d its bit i
+        // is set where lanes i .. i + 2d - 1 all continue one, i.e. where a run is still longer than 2d.
+        unsigned long long cont = ~starts;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            if (!cont) break;
 #pragma unroll
             for (int m = 0; m < NCH; ++m) {
-                const float w = __shfl_down(v[m], i, 64);
-                if (i < len) v[m] += w;
+                const float w = __shfl_down(v[m], d, 64);
+                if (d < len) v[m] += w;
             }
+            cont &= cont >> d;
         }
         if (!(ok && start)) return;
         // planar layout: the heads (neighbouring z of one (x, y)) hit a contiguous run of one channel plane
