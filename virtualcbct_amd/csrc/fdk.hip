@@ -286,6 +286,19 @@ __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, f
 }
 
 // ------------------------------------------------------------------------------------------------
+// The FP32 tap interpolation both backprojectors share (so that they agree bit for bit): a + w (b - a) as one
+// FMA.  (FP32 and continuous in the row coordinate; the FP64 horizontal part keeps the reference's order.)
+__device__ __forceinline__ float bp_lerp(float a, float b, float w) { return fmaf(w, b - a, a); }
+
+// One projection as a buffer resource (GFX9 raw buffer: byte-addressed, stride 0, loads past `bytes` read 0).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t bp_rsrc(const float* p, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, bytes, 0x00020000);
+}
+__device__ __forceinline__ float bp_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// ------------------------------------------------------------------------------------------------
 // Backprojection.  Thread = one (x, y) voxel column, ZB consecutive slices (ZB picked by the autotuner
 // among 2, 4, 8); block = 16 x 16 columns.
 // Per angle the geometry (U, its interpolation weight and base column, 1/L2) is computed once per column
@@ -293,6 +306,9 @@ __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, f
 // interpolation follows F/reconstructImage.cpp:76-175: the horizontal weight is frac(U) but the base column
 // is floor(W/2 + U + xshift_px - 0.5), and the three edge cases at the last column / row sample one tap pair
 // or one tap.  (A precomputed per-(column, angle) table was measured slower: each z-block re-reads it.)
+// A slice's taps are raw buffer loads of one projection: one 32-bit byte offset per slice (row * W + column,
+// a 24-bit multiply; the launcher checks H, W < 2^24 and H W 4 < 2^31), the next column as the instruction's
+// immediate offset and the next row as its scalar offset.
 template <int ZB>
 __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, const float* __restrict__ sino, int R,
                                                      int Z, double zstart, double step, double scale, int H, int W,
@@ -318,6 +334,7 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
     const float hH = (float)(H / 2);
     const float ipW = (float)(1.0 / pitchW);
     const double rpW = 1.0 / pitchW;  // correctly rounded: the divisions by pitchW below are exact products
+    const int rowb = W * 4;           // byte offset of the next row
     for (int th = 0; th < n; ++th) {
         const double c = atab[th], s = atab[n + th], dbeta = atab[2 * n + th], xs = atab[3 * n + th];
         // horizontal: FP64 in the reference's operation order (its base column floor(W/2 + U + xs - 0.5)
@@ -330,35 +347,32 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
         double w = U - fU;
         int UU = (int)floor(hW + U + xs - 0.5);
         const bool ucol = (UU >= 0 && UU < W - 1), uedge = (UU == W - 1);
-        if (!ucol && !uedge) continue;
+        const bool inpanel = ucol || uedge;
         // vertical: an ordinary bilinear tap, continuous in the row coordinate, and the angle weight: FP32
-        const float rL2 = 1.0f / (float)L2;
+        const float rL2 = __builtin_amdgcn_rcpf((float)L2);
         const float iL = rL2 * ipW;
         const float wgt = rL2 * rL2 * (float)(dbeta * xfactor);
-        const float* P = sino + (size_t)th * H * W;
+        const __amdgpu_buffer_rsrc_t P = bp_rsrc(sino + (size_t)th * H * W, H * W * 4);
         const float wf = (float)w;
+        const unsigned UUc = inpanel ? (unsigned)UU : 0u;
+        // Branch-free: the four taps are loaded unconditionally (row clamped into the panel; a tap past the
+        // projection's end reads 0) and selects keep the reference's cases, so every kept value is the one the
+        // branching form computes, and no angle waits for the previous angle's loads before issuing its own.
 #pragma unroll
         for (int i = 0; i < ZB; ++i) {
             const float ZETA = hH + zz[i] * iL;
             const float fz = floorf(ZETA);
             const int NZ = (int)fz;
             const float wz = ZETA - fz;
-            float sum;
-            if (NZ >= 0 && NZ < H - 1) {
-                const float* r0 = P + (size_t)NZ * W + UU;
-                if (ucol) {
-                    float a0 = r0[0], a1 = r0[1], b0 = r0[W], b1 = r0[W + 1];
-                    sum = (a0 * (1.f - wf) + a1 * wf) * (1.f - wz) + (b0 * (1.f - wf) + b1 * wf) * wz;
-                } else {
-                    sum = r0[0] * (1.f - wz) + r0[W] * wz;
-                }
-            } else if (NZ == H - 1) {
-                const float* r0 = P + (size_t)NZ * W + UU;
-                sum = ucol ? (r0[0] * (1.f - wf) + r0[1] * wf) : r0[0];
-            } else {
-                continue;
-            }
-            acc[i] += sum * wgt;
+            const int NZc = min(max(NZ, 0), H - 1);
+            const int vo = (int)(__umul24((unsigned)NZc, (unsigned)W) + UUc) * 4;
+            const float a0 = bp_ld(P, vo, 0), a1 = bp_ld(P, vo + 4, 0);
+            const float b0 = bp_ld(P, vo, rowb), b1 = bp_ld(P, vo + 4, rowb);
+            const float h0 = ucol ? bp_lerp(a0, a1, wf) : a0;  // the last column: its one tap
+            const float h1 = ucol ? bp_lerp(b0, b1, wf) : b0;
+            const float sum = NZ < H - 1 ? bp_lerp(h0, h1, wz) : h0;  // the last row: its one row of taps
+            // a tap set the reference skips adds 0 x wgt (the weight is always finite): acc unchanged
+            acc[i] = fmaf((inpanel && (unsigned)NZ < (unsigned)H) ? sum : 0.f, wgt, acc[i]);
         }
     }
 #pragma unroll
@@ -441,7 +455,7 @@ __global__ __launch_bounds__(256) void k_backproject_lds(float* __restrict__ rec
         int UU = (int)floor(hW + U + xs - 0.5);
         const bool ucol = (UU >= 0 && UU < W - 1), uedge = (UU == W - 1);
         if (!ucol && !uedge) continue;
-        const float rL2 = 1.0f / (float)L2;
+        const float rL2 = __builtin_amdgcn_rcpf((float)L2);
         const float iL = rL2 * ipW;
         const float wgt = rL2 * rL2 * (float)(dbeta * xfactor);
         const float wf = (float)w;
@@ -460,16 +474,16 @@ __global__ __launch_bounds__(256) void k_backproject_lds(float* __restrict__ rec
             if (NZ >= 0 && NZ < H - 1) {
                 if (ucol) {
                     float a0 = tap(NZ, UU), a1 = tap(NZ, UU + 1), b0 = tap(NZ + 1, UU), b1 = tap(NZ + 1, UU + 1);
-                    sum = (a0 * (1.f - wf) + a1 * wf) * (1.f - wz) + (b0 * (1.f - wf) + b1 * wf) * wz;
+                    sum = bp_lerp(bp_lerp(a0, a1, wf), bp_lerp(b0, b1, wf), wz);
                 } else {
-                    sum = tap(NZ, UU) * (1.f - wz) + tap(NZ + 1, UU) * wz;
+                    sum = bp_lerp(tap(NZ, UU), tap(NZ + 1, UU), wz);
                 }
             } else if (NZ == H - 1) {
-                sum = ucol ? (tap(NZ, UU) * (1.f - wf) + tap(NZ, UU + 1) * wf) : tap(NZ, UU);
+                sum = ucol ? bp_lerp(tap(NZ, UU), tap(NZ, UU + 1), wf) : tap(NZ, UU);
             } else {
                 continue;
             }
-            acc[i] += sum * wgt;
+            acc[i] = fmaf(sum, wgt, acc[i]);
         }
     }
     if (!active) return;
@@ -583,6 +597,9 @@ static int launch_bp(int zb, int R, int Z, int zstart, double step, double scale
                      const BpFoot* d_foot = nullptr) {
     dim3 grid((R + 15) / 16, (R + 15) / 16, (Z + zb - 1) / zb);
     const double pitchW = pitch_iso * 800 / W;
+    // the taps' 24-bit row multiply and 32-bit byte offsets within one projection
+    if (H >= (1 << 24) || W >= (1 << 24) || (long long)H * W * 4 >= (1ll << 31))
+        return fail(VCBCT_EINVAL, "reconstruct_image: projection too large (H W 4 bytes must stay below 2^31)");
 #define VCBCT_BP(ZBV)                                                                                                  \
     hipLaunchKernelGGL(k_backproject<ZBV>, grid, dim3(256), 0, st, drec, ds, R, Z, (double)zstart, step, scale, H, W, n, \
                        d_atab, (double)xfactor, pitchW)
