@@ -306,9 +306,9 @@ __device__ __forceinline__ float bp_ld(__amdgpu_buffer_rsrc_t r, int voff, int s
 // interpolation follows F/reconstructImage.cpp:76-175: the horizontal weight is frac(U) but the base column
 // is floor(W/2 + U + xshift_px - 0.5), and the three edge cases at the last column / row sample one tap pair
 // or one tap.  (A precomputed per-(column, angle) table was measured slower: each z-block re-reads it.)
-// A slice's taps are raw buffer loads of one projection: one 32-bit byte offset per slice (row * W + column,
-// a 24-bit multiply; the launcher checks H, W < 2^24 and H W 4 < 2^31), the next column as the instruction's
-// immediate offset and the next row as its scalar offset.
+// A slice's taps are raw buffer loads of one projection: one 32-bit byte offset per slice (row * 4W + 4 column,
+// a 24-bit multiply-add; the launcher checks H < 2^24, 4W < 2^24 and 4HW < 2^31), the next column as the
+// instruction's immediate offset and the next row as its scalar offset.
 template <int ZB>
 __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, const float* __restrict__ sino, int R,
                                                      int Z, double zstart, double step, double scale, int H, int W,
@@ -335,6 +335,7 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
     const float ipW = (float)(1.0 / pitchW);
     const double rpW = 1.0 / pitchW;  // correctly rounded: the divisions by pitchW below are exact products
     const int rowb = W * 4;           // byte offset of the next row
+    const float hm1 = (float)(H - 1);  // the last row (rows are exact in FP32: H < 2^24)
     for (int th = 0; th < n; ++th) {
         const double c = atab[th], s = atab[n + th], dbeta = atab[2 * n + th], xs = atab[3 * n + th];
         // horizontal: FP64 in the reference's operation order (its base column floor(W/2 + U + xs - 0.5)
@@ -353,26 +354,26 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
         const float iL = rL2 * ipW;
         const float wgt = rL2 * rL2 * (float)(dbeta * xfactor);
         const __amdgpu_buffer_rsrc_t P = bp_rsrc(sino + (size_t)th * H * W, H * W * 4);
-        const float wf = (float)w;
-        const unsigned UUc = inpanel ? (unsigned)UU : 0u;
+        // the last column: horizontal weight 0, so the lerps keep its one tap (a + 0 (b - a) = a)
+        const float wf = ucol ? (float)w : 0.f;
+        const unsigned UU4 = inpanel ? (unsigned)UU * 4u : 0u;
         // Branch-free: the four taps are loaded unconditionally (row clamped into the panel; a tap past the
-        // projection's end reads 0) and selects keep the reference's cases, so every kept value is the one the
-        // branching form computes, and no angle waits for the previous angle's loads before issuing its own.
+        // projection's end reads 0) and weights and selects keep the reference's cases, so every kept value is
+        // the one the branching form computes, and no angle waits for the previous angle's loads.
 #pragma unroll
         for (int i = 0; i < ZB; ++i) {
-            const float ZETA = hH + zz[i] * iL;
+            const float ZETA = fmaf(zz[i], iL, hH);
             const float fz = floorf(ZETA);
-            const int NZ = (int)fz;
             const float wz = ZETA - fz;
-            const int NZc = min(max(NZ, 0), H - 1);
-            const int vo = (int)(__umul24((unsigned)NZc, (unsigned)W) + UUc) * 4;
+            const float fzc = __builtin_amdgcn_fmed3f(fz, 0.f, hm1);  // the row clamped into the panel
+            const int vo = (int)(__umul24((unsigned)(int)fzc, (unsigned)rowb) + UU4);
             const float a0 = bp_ld(P, vo, 0), a1 = bp_ld(P, vo + 4, 0);
             const float b0 = bp_ld(P, vo, rowb), b1 = bp_ld(P, vo + 4, rowb);
-            const float h0 = ucol ? bp_lerp(a0, a1, wf) : a0;  // the last column: its one tap
-            const float h1 = ucol ? bp_lerp(b0, b1, wf) : b0;
-            const float sum = NZ < H - 1 ? bp_lerp(h0, h1, wz) : h0;  // the last row: its one row of taps
-            // a tap set the reference skips adds 0 x wgt (the weight is always finite): acc unchanged
-            acc[i] = fmaf((inpanel && (unsigned)NZ < (unsigned)H) ? sum : 0.f, wgt, acc[i]);
+            const float h0 = bp_lerp(a0, a1, wf), h1 = bp_lerp(b0, b1, wf);
+            const float sum = fz < hm1 ? bp_lerp(h0, h1, wz) : h0;  // the last row: its one row of taps
+            // a tap set the reference skips (a row outside the panel: fz clamped) adds 0 x wgt (the weight is
+            // always finite): acc unchanged
+            acc[i] = fmaf((inpanel && fzc == fz) ? sum : 0.f, wgt, acc[i]);
         }
     }
 #pragma unroll
@@ -466,7 +467,7 @@ __global__ __launch_bounds__(256) void k_backproject_lds(float* __restrict__ rec
         };
 #pragma unroll
         for (int i = 0; i < ZB; ++i) {
-            const float ZETA = hH + zz[i] * iL;
+            const float ZETA = fmaf(zz[i], iL, hH);
             const float fz = floorf(ZETA);
             const int NZ = (int)fz;
             const float wz = ZETA - fz;
@@ -598,7 +599,7 @@ static int launch_bp(int zb, int R, int Z, int zstart, double step, double scale
     dim3 grid((R + 15) / 16, (R + 15) / 16, (Z + zb - 1) / zb);
     const double pitchW = pitch_iso * 800 / W;
     // the taps' 24-bit row multiply and 32-bit byte offsets within one projection
-    if (H >= (1 << 24) || W >= (1 << 24) || (long long)H * W * 4 >= (1ll << 31))
+    if (H >= (1 << 24) || W >= (1 << 22) || (long long)H * W * 4 >= (1ll << 31))
         return fail(VCBCT_EINVAL, "reconstruct_image: projection too large (H W 4 bytes must stay below 2^31)");
 #define VCBCT_BP(ZBV)                                                                                                  \
     hipLaunchKernelGGL(k_backproject<ZBV>, grid, dim3(256), 0, st, drec, ds, R, Z, (double)zstart, step, scale, H, W, n, \
