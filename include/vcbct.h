@@ -270,7 +270,9 @@ int vcbct_filter_sinogram(int proj_height, int proj_width, int n_proj, const flo
 
 /* Replaces reconstructImage (F/reconstructImage.cpp:10-184): voxel-driven cone-beam backprojection
  * with the reference's interpolation rule, angle weights and edge cases; accumulates (+=) into
- * dev_recon [Z][R][R] FP32 (inside the (R-1)/2 circle mask).  Angles/xshift are host arrays. */
+ * dev_recon [Z][R][R] FP32 (inside the (R-1)/2 circle mask).  Angles/xshift are host arrays.  One
+ * projection (proj_height x proj_width FP32) must stay below 2^31 bytes, as must 4 proj_width below 2^24
+ * (VCBCT_EINVAL otherwise; the same holds for every backprojection entry point and FDK plan). */
 int vcbct_reconstruct_image(int recon_size, int recon_slices, int zstart_cm, double recon_step,
                             double recon_scale, float* dev_recon, int proj_height, int proj_width, int n_proj,
                             const float* dev_filtered, int xfactor, const double* host_angles_rad,
