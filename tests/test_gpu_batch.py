@@ -183,10 +183,11 @@ def test_fdk_plan_untuned_equals_one_shot(torch, monkeypatch):
 
 
 def test_fdk_plan_half_fan(torch):
-    """A half-fan plan weights and pads the projections inside its filter call: its filtered panel equals the
-    one-shot weighting + filter bit for bit, its reconstruction the one-shot half-fan FDK (to the
-    backprojector variant's rounding) and the oracle's chain (FP64 filter and backprojection of the oracle's
-    weighted, padded panel); captured in a batch graph, every replay equals the eager result."""
+    """A half-fan plan weights and pads the projections inside its filter's load (2w (-log p) for a real column,
+    0 for a padding one): its filtered panel equals the one-shot weighting + filter (-log of the FP32 p^(2w)) to
+    FP32 rounding, its reconstruction the one-shot half-fan FDK and the oracle's chain (FP64 filter and
+    backprojection of the oracle's weighted, padded panel); captured in a batch graph, every replay equals the
+    eager result."""
     from virtualcbct_amd import fdk
     from virtualcbct_amd.batch import Batch
     rng = np.random.default_rng(6)
@@ -199,12 +200,15 @@ def test_fdk_plan_half_fan(torch):
     assert plan.width == fdk.half_fan_width(W, xs) > W
     filt = plan.filter(proj)
     pw, xo, yo, pit = fdk.offset_weight_device(proj, xs, ys)
-    assert torch.equal(filt, fdk.filter_sinogram_device(pw, xo, yo, pit))
+    one_f = fdk.filter_sinogram_device(pw, xo, yo, pit)
+    ef = float((filt - one_f).abs().max()) / float(one_f.abs().max())
+    print(f"half-fan plan filter vs one-shot weighting + filter: {ef:.3e} of max")
+    assert ef < 1e-5
     rec = torch.zeros(plan.shape_recon, device="cuda")
     plan.backproject(filt, rec)
     one = fdk.fdk_half_fan_device(proj, -2, 0.45, 0.4, 40, 9, 1000, ang, xs, ys)
     torch.cuda.synchronize()
-    assert torch.allclose(rec, one, rtol=1e-6, atol=1e-4)
+    assert torch.allclose(rec, one, rtol=1e-5, atol=1e-3)
     pw_o, xo_o, yo_o = O.offset_weight(proj.cpu().numpy(), xs, ys)
     ref = O.reconstruct_image(40, 9, -2, 0.45, 0.4, O.filter_sinogram(pw_o, xo_o, yo_o, pitch_iso=pit), 1000, ang,
                               xo_o, pitch_iso=pit)

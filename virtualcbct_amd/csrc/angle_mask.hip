@@ -60,6 +60,11 @@ __host__ __device__ inline double offset_exponent(const OffsetPar& q, int col) {
     return 2.0 * (s * s);
 }
 
+void offset_exponent_table(const OffsetPar* par, int n, int W, float* out) {
+    for (int k = 0; k < n; ++k)
+        for (int c = 0; c < W; ++c) out[(size_t)k * W + c] = (float)offset_exponent(par[k], c);
+}
+
 constexpr int OW_ROWS = 16;  // rows of one projection per block
 
 // out [n][H][Wo]: column j of the padded panel holds real column j - off (off = P when the padding goes

@@ -210,10 +210,16 @@ __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T
     return x;
 }
 
+// ecol (half-fan plans): the offset weighting fused into the load.  The caller's panel has Wsrc columns,
+// placed at column `off` of the W-column padded panel; a real column c of projection th contributes
+// ecol[th][c] (-log p) (= -log p^(2w), the weighted pixel's line integral; exponent 0 gives 0 whatever p), a
+// padding column 0 (-log 1).  HF false: no weighting, Wsrc = W and off = 0.
+template <bool HF>
 __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, float* __restrict__ out, int H, int W,
                                                long nrows, int L, int logL, const float2* __restrict__ Hf,
                                                const float2* __restrict__ tw, const float* __restrict__ xc_tab,
-                                               const float* __restrict__ yc_tab, float pitchW, int staged) {
+                                               const float* __restrict__ yc_tab, float pitchW, int staged,
+                                               const float* __restrict__ ecol, int Wsrc, int off) {
     // LDS: [A][B] ping-pong buffers, then (staged) the twiddle and response tables; without staging (L above
     // 2048, where the four would exceed 64 KB) the tables are read from global memory
     extern __shared__ float2 lds[];
@@ -234,18 +240,19 @@ __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, f
     // the next row pair's pixels are loaded into registers while the current pair is transformed (the
     // block barriers wait for LDS traffic only, so these loads stay in flight across them)
     float pa[FPF], pb[FPF];
+    const int ws = HF ? Wsrc : W, of = HF ? off : 0;
     auto prefetch = [&](long r1) {
-        const float* p1 = proj + r1 * W;
-        const float* p2 = proj + (r1 + 1) * W;
+        const float* p1 = proj + r1 * ws;
+        const float* p2 = proj + (r1 + 1) * ws;
         const bool has2 = r1 + 1 < nrows;
 #pragma unroll
         for (int i = 0; i < FPF; ++i) {
-            const int k = threadIdx.x + i * FT;
+            const int c = threadIdx.x + i * FT - of;  // the caller's column (outside [0, ws): padding)
             pa[i] = 1.f;
             pb[i] = 1.f;
-            if (k < W) {
-                pa[i] = p1[k];
-                if (has2) pb[i] = p2[k];
+            if (c >= 0 && c < ws) {
+                pa[i] = p1[c];
+                if (has2) pb[i] = p2[c];
             }
         }
     };
@@ -255,10 +262,24 @@ __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, f
         const long r1 = first + 2 * sig, r2 = r1 + 1;
         if (r1 >= nrows) break;  // uniform over the block
         const bool has2 = r2 < nrows;
+        const float* e1 = HF ? ecol + (r1 / H) * Wsrc : nullptr;
+        const float* e2 = HF ? ecol + (r2 / H) * Wsrc : nullptr;
 #pragma unroll
         for (int i = 0; i < FPF; ++i) {
             const int k = threadIdx.x + i * FT;
-            if (k < W) A[k] = make_float2(-logf(pa[i]), has2 ? -logf(pb[i]) : 0.f);
+            if (k >= W) continue;
+            float q1 = -logf(pa[i]), q2 = has2 ? -logf(pb[i]) : 0.f;
+            if (HF) {
+                const int c = k - off;
+                if (c >= 0 && c < Wsrc) {
+                    const float x1 = e1[c], x2 = has2 ? e2[c] : 0.f;
+                    q1 = x1 == 0.f ? 0.f : x1 * q1;
+                    q2 = x2 == 0.f ? 0.f : x2 * q2;
+                } else {
+                    q1 = q2 = 0.f;
+                }
+            }
+            A[k] = make_float2(q1, q2);
         }
         for (int k = W + threadIdx.x; k < L; k += FT) A[k] = make_float2(0.f, 0.f);
         __syncthreads();
@@ -576,8 +597,10 @@ static std::vector<float> filter_table(int n, int W, const double* xshift, const
     return tab;
 }
 
+// ecol non-null: the half-fan weighting fused into the filter (k_filter), the caller's panel Wsrc columns wide at
+// column `off` of the W-column padded panel.
 static int launch_filter(const FilterPlan* P, int H, int W, int n, const float* dp, float* dq, const float* d_ctab,
-                         double pitch_iso, hipStream_t st) {
+                         double pitch_iso, hipStream_t st, const float* ecol = nullptr, int Wsrc = 0, int off = 0) {
     const long nrows = (long)n * H;
     const long npairs = (nrows + 1) / 2;
     const long nblk = (npairs + FSIG - 1) / FSIG;
@@ -586,8 +609,14 @@ static int launch_filter(const FilterPlan* P, int H, int W, int n, const float* 
     const size_t sig_bytes = 2 * (size_t)P->L * sizeof(float2);  // the two signal buffers
     const size_t tab_bytes = 2 * (size_t)P->L * sizeof(float2);
     const int staged = sig_bytes + tab_bytes <= 65536;
-    hipLaunchKernelGGL(k_filter, dim3((unsigned)nblk), dim3(FT), sig_bytes + (staged ? tab_bytes : 0), st, dp, dq,
-                       H, W, nrows, P->L, P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, staged);
+    if (ecol)
+        hipLaunchKernelGGL(k_filter<true>, dim3((unsigned)nblk), dim3(FT), sig_bytes + (staged ? tab_bytes : 0), st,
+                           dp, dq, H, W, nrows, P->L, P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, staged,
+                           ecol, Wsrc, off);
+    else
+        hipLaunchKernelGGL(k_filter<false>, dim3((unsigned)nblk), dim3(FT), sig_bytes + (staged ? tab_bytes : 0), st,
+                           dp, dq, H, W, nrows, P->L, P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, staged,
+                           nullptr, W, 0);
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
@@ -635,11 +664,10 @@ struct vcbct_fdk_plan {
     vcbct::FilterPlan* fp;
     float* d_ctab;
     double* d_atab;
-    // half-fan plans: W is the padded panel; the caller's projections have hf_W columns, weighted and padded
-    // into d_pad (the plan's own [n][H][W] buffer) before the filter
+    // half-fan plans: W is the padded panel; the caller's projections have hf_W columns at column hf_off of
+    // it, weighted inside the filter's load with the exponents d_ecol [n][hf_W]
     int hf_W, hf_off;
-    float* d_pad;
-    vcbct::OffsetPar* d_opar;
+    float* d_ecol;
     // backprojector form: LDS-staged (with its footprint table) or not, picked with zb by the autotuner
     int bp_lds;
     vcbct::BpFoot* d_foot;
@@ -785,10 +813,11 @@ extern "C" int vcbct_fdk_plan_create_half_fan(vcbct_fdk_plan** out, int H, int W
     if (rc) return rc;
     p->hf_W = W;
     p->hf_off = off;
-    if (dev_malloc((void**)&p->d_opar, (size_t)n * sizeof(OffsetPar)) != hipSuccess ||
-        dev_malloc((void**)&p->d_pad, (size_t)n * H * Wo * sizeof(float)) != hipSuccess)
-        rc = fail(VCBCT_ENOMEM, "fdk_plan_create_half_fan: device allocation of the padded panel failed");
-    if (!rc && hipMemcpy(p->d_opar, par.data(), (size_t)n * sizeof(OffsetPar), hipMemcpyHostToDevice) != hipSuccess)
+    std::vector<float> ecol((size_t)n * W);
+    offset_exponent_table(par.data(), n, W, ecol.data());
+    if (dev_malloc((void**)&p->d_ecol, ecol.size() * sizeof(float)) != hipSuccess)
+        rc = fail(VCBCT_ENOMEM, "fdk_plan_create_half_fan: device allocation of the weighting table failed");
+    if (!rc && hipMemcpy(p->d_ecol, ecol.data(), ecol.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(VCBCT_EHIP, "fdk_plan_create_half_fan: upload failed");
     if (rc) {
         vcbct_fdk_plan_destroy(p);
@@ -814,8 +843,7 @@ extern "C" int vcbct_fdk_plan_set_window(vcbct_fdk_plan* p, double alpha) {
 extern "C" void vcbct_fdk_plan_destroy(vcbct_fdk_plan* p) {
     if (!p) return;
     (void)hipSetDevice(p->device);
-    (void)hipFree(p->d_pad);
-    (void)hipFree(p->d_opar);
+    (void)hipFree(p->d_ecol);
     (void)hipFree(p->d_foot);
     (void)hipFree(p->d_ctab);
     (void)hipFree(p->d_atab);
@@ -824,12 +852,9 @@ extern "C" void vcbct_fdk_plan_destroy(vcbct_fdk_plan* p) {
 
 extern "C" int vcbct_fdk_plan_filter(vcbct_fdk_plan* p, const float* dev_proj, float* dev_filtered, void* stream) {
     if (!p || !dev_proj || !dev_filtered) return fail(VCBCT_EINVAL, "fdk_plan_filter: NULL argument");
-    if (p->d_pad) {  // half-fan: weight and pad into the plan's buffer, then filter the padded panel
-        int rc = launch_offset_weight(dev_proj, p->d_pad, p->H, p->hf_W, p->W, p->hf_off, p->n, p->d_opar,
-                                      (hipStream_t)stream);
-        if (rc) return rc;
-        dev_proj = p->d_pad;
-    }
+    if (p->d_ecol)  // half-fan: weighted and padded inside the filter's load
+        return launch_filter(p->fp, p->H, p->W, p->n, dev_proj, dev_filtered, p->d_ctab, p->pitch,
+                             (hipStream_t)stream, p->d_ecol, p->hf_W, p->hf_off);
     return launch_filter(p->fp, p->H, p->W, p->n, dev_proj, dev_filtered, p->d_ctab, p->pitch, (hipStream_t)stream);
 }
 

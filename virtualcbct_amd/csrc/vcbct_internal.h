@@ -237,6 +237,8 @@ int half_fan_params(int W, int n, const double* xshift, const double* yshift, do
 // in [n][H][W] -> out [n][H][Wo]; d_par: n parameters in device memory.  Allocates nothing.
 int launch_offset_weight(const float* in, float* out, int H, int W, int Wo, int off, int n, const OffsetPar* d_par,
                          hipStream_t st);
+// The weighting's exponent 2w per (projection, real column), FP32, [n][W] (host).
+void offset_exponent_table(const OffsetPar* par, int n, int W, float* out);
 
 // Fault injection for allocation failure (VCBCT_FAULT_ALLOC=<n>: the n-th device allocation fails).
 hipError_t dev_malloc(void** p, size_t bytes);
