@@ -375,12 +375,14 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
         const float iL = rL2 * ipW;
         const float wgt = rL2 * rL2 * (float)(dbeta * xfactor);
         const __amdgpu_buffer_rsrc_t P = bp_rsrc(sino + (size_t)th * H * W, H * W * 4);
-        // the last column: horizontal weight 0, so the lerps keep its one tap (a + 0 (b - a) = a)
+        // the last column: horizontal weight 0, so the lerps keep its one tap (a + 0 (b - a) = a for finite taps;
+        // the unused neighbour is the next row's first column, or 0 past the projection's end)
         const float wf = ucol ? (float)w : 0.f;
         const unsigned UU4 = inpanel ? (unsigned)UU * 4u : 0u;
         // Branch-free: the four taps are loaded unconditionally (row clamped into the panel; a tap past the
-        // projection's end reads 0) and weights and selects keep the reference's cases, so every kept value is
-        // the one the branching form computes, and no angle waits for the previous angle's loads.
+        // projection's end reads 0) and weights and selects keep the reference's cases, so for finite filtered
+        // projections every kept value is the one the branching form computes, and no angle waits for the
+        // previous angle's loads.
 #pragma unroll
         for (int i = 0; i < ZB; ++i) {
             const float ZETA = fmaf(zz[i], iL, hH);
