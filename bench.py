@@ -742,7 +742,13 @@ def main():
             "config": {"workload": "25 cm water cylinder, 512x512x128 voxels (0.5 mm x 0.5 mm x 1 mm), 6 uint16 "
                                    "elemental maps, 360 angles, 400x400 detector, 115 mm half-fan offset, "
                                    "91 energy bins 10-100 keV, FP32 projections"},
-            "roofline": roofline, "tile": tile}
+            "roofline": roofline, "tile": tile,
+            # context only (vs_baseline stays null: the reference publishes no benchmark)
+            "reference_gpu_comment": {
+                "rays_bins_per_s": 360 * 400 * 400 * 101 / 289.0,
+                "source": "P/reprojectionOnGPU.cu:975-983, a developer comment beside the shipped launch shape: "
+                          "4 min 49 s on an unnamed GeForce, read as 360 angles x 400^2 rays x 101 bins (the "
+                          "comment states neither); an inference, not a published number"}}
 
     if not args.no_extras:
         bench_fdk(line, inp, dev, stream, out)
