@@ -321,11 +321,11 @@ int vcbct_fdk_plan_slices_per_thread(const vcbct_fdk_plan* p);
  * same result bit for bit. */
 int vcbct_fdk_plan_bp_staged(const vcbct_fdk_plan* p);
 /* Half-fan FDK plan (no reference counterpart; the weighting of vcbct_fdk_offset_weight below, inside the
- * plan): the plan runs on the padded panel (its width: vcbct_fdk_plan_width), and its filter call weights and
- * pads the caller's [n][H][W] projections into a buffer the plan owns before filtering, so dev_filtered is
- * [n][H][vcbct_fdk_plan_width(p)].  Execution still allocates nothing and does not synchronise (capturable);
- * calls on one half-fan plan must not overlap (they share its padded buffer).  Errors as
- * vcbct_fdk_offset_weight. */
+ * plan): the plan runs on the padded panel (its width: vcbct_fdk_plan_width), and its filter reads the caller's
+ * [n][H][W] projections, weighting and padding them as it loads them (2w (-log p) per real column, from a table
+ * built at creation; 0 in the padding), so dev_filtered is [n][H][vcbct_fdk_plan_width(p)] and equals the
+ * separate weighting + filter to FP32 rounding.  Execution allocates nothing, writes no plan state and does not
+ * synchronise (capturable).  Errors as vcbct_fdk_offset_weight. */
 int vcbct_fdk_plan_create_half_fan(vcbct_fdk_plan** out, int proj_height, int proj_width, int n_proj,
                                    int recon_size, int recon_slices, int zstart_cm, double recon_step,
                                    double recon_scale, int xfactor, const double* host_angles_rad,
