@@ -235,6 +235,19 @@ def bench_fdk(line, inp, dev, stream, out):
                    "sample": f"{nz} slices of the 270^2 reconstruction from all 360 views, {sec:.1f} s"}
         log(f"cpu reference backprojection: {cpu_ref['gups']:.3f} GUPS ({cpu_ref['sample']})")
         del fh
+    # the reference's CPU filter (F/filterSinogram.cpp, OpenMP over projections) on a bounded sample of the
+    # same projections: a multiple of the thread count, sized to a few seconds
+    cpu_filt = None
+    if O.ref_available():
+        nthreads = host_cpu()["threads"]
+        ph = proj.cpu().numpy()
+        _, s1 = O.ref_filter_sinogram(ph[:nthreads], inp["xs_fdk"][:nthreads], inp["ys_fdk"][:nthreads])
+        na = int(min(w["n_ang"], nthreads * max(1, round(4.0 / max(s1, 1e-3)))))
+        _, secf = O.ref_filter_sinogram(ph[:na], inp["xs_fdk"][:na], inp["ys_fdk"][:na])
+        cpu_filt = {"rows_per_s": na * w["H"] / secf, "cores": nthreads, "kind": "reference",
+                    "sample": f"{na} of the {w['n_ang']} projections ({w['H']} rows of {w['W']}), {secf:.1f} s"}
+        log(f"cpu reference filter: {cpu_filt['rows_per_s']:.3e} rows/s ({cpu_filt['sample']})")
+        del ph
     from virtualcbct_amd.batch import autotune_log
     updates = w["R"] * w["R"] * w["Z"] * w["n_ang"]
     line["fdk"] = {"gups": updates / (np.median(tb) * 1e-3) / 1e9, "bp_ms": float(np.median(tb)),
@@ -243,7 +256,7 @@ def bench_fdk(line, inp, dev, stream, out):
                                      "width": plan_hf.width},
                    "ram_lak_plan": {"window_alpha": 1.0, "filter_ms": float(np.median(trl)),
                                     "bp_ms": float(np.median(trb))},
-                   "cpu_reference_backprojection": cpu_ref,
+                   "cpu_reference_backprojection": cpu_ref, "cpu_reference_filter": cpu_filt,
                    "filter_rows_per_s": w["n_ang"] * w["H"] / (np.median(tf) * 1e-3),
                    "updates": updates, "shape": [w["Z"], w["R"], w["R"]],
                    # the backprojector is bound by VALU issue (its taps hit in L2), so its bound is stated in
