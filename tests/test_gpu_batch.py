@@ -182,19 +182,21 @@ def test_fdk_plan_untuned_equals_one_shot(torch, monkeypatch):
     plan.close()
 
 
-def test_fdk_plan_half_fan(torch):
+@pytest.mark.parametrize("side", [1.0, -1.0])
+def test_fdk_plan_half_fan(torch, side):
     """A half-fan plan weights and pads the projections inside its filter's load (2w (-log p) for a real column,
     0 for a padding one): its filtered panel equals the one-shot weighting + filter (-log of the FP32 p^(2w)) to
     FP32 rounding, its reconstruction the one-shot half-fan FDK and the oracle's chain (FP64 filter and
     backprojection of the oracle's weighted, padded panel); captured in a batch graph, every replay equals the
-    eager result."""
+    eager result.  Both shift signs: the padding after the real columns, and before them (the filter's load
+    then reads the caller's column k - P for panel column k)."""
     from virtualcbct_amd import fdk
     from virtualcbct_amd.batch import Batch
     rng = np.random.default_rng(6)
     n, H, W = 24, 40, 64
     proj = torch.from_numpy(rng.uniform(0.1, 1.0, (n, H, W)).astype(np.float32)).cuda()
     ang = np.deg2rad(np.arange(n) * 15.0 - 170.0001)
-    xs = 7.5 + 0.05 * rng.standard_normal(n)  # ~12 of 32 half-panel pixels off centre (0.625 cm per pixel)
+    xs = side * (7.5 + 0.05 * rng.standard_normal(n))  # ~12 of 32 half-panel pixels off centre (0.625 cm per pixel)
     ys = np.full(n, -0.2)
     plan = fdk.FdkPlan(H, W, n, 40, 9, -2, 0.45, 0.4, 1000, ang, xs, ys, half_fan=True)
     assert plan.width == fdk.half_fan_width(W, xs) > W
