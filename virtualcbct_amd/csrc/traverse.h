@@ -463,6 +463,14 @@ __host__ __device__ __forceinline__ bool make_step(const ProjGeom& g, const RayC
     return inprev;
 }
 
+// The first lane's value of x (in scalar registers).
+__device__ __forceinline__ double first_lane(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // True when the condition holds on any lane of the wave (the host build walks one lane at a time).
 __host__ __device__ __forceinline__ bool any_lane(bool b) {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -524,9 +532,24 @@ __host__ __device__ __forceinline__ bool in_circle(const RayCtx& r, const Charge
 // single lane's data: the rare cases (a piece near the circle's boundary, a tie of the crossing distances)
 // are wave-uniform branches around their extra arithmetic.
 template <class V>
-__host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const RayCtx& r, bool active, int kmin,
+__host__ __device__ __forceinline__ void traverse_table(const ProjGeom& g, const RayCtx& r_in, bool active, int kmin,
                                                         int kmax, V& vis, WaveLds* lds) {
 #pragma clang fp contract(fast)
+#ifdef __HIP_DEVICE_COMPILE__
+    // Every lane took the same box entry here (the caller's condition), so the entry and exit along both axes,
+    // the plane grid (uin, pha) and the minor direction are the same on every lane, bit for bit: read from
+    // the first lane they sit in scalar registers, not in vector registers live across the walk.
+    RayCtx r = r_in;
+    r.ui = first_lane(r_in.ui);
+    r.uf = first_lane(r_in.uf);
+    r.vi = first_lane(r_in.vi);
+    r.vf = first_lane(r_in.vf);
+    r.uin = first_lane(r_in.uin);
+    r.pha = first_lane(r_in.pha);
+    r.dirv = first_lane(r_in.dirv);
+#else
+    const RayCtx& r = r_in;
+#endif
     const double lim_u = g.lim_u;
     const bool on = active && r.hit;
     const ChargeC c = charge_consts(g, r);
@@ -695,12 +718,6 @@ namespace vcbct {
 // Walks the rays of one wave: lane = detector row j (inactive lanes, j >= H, walk row 0 and charge
 // nothing), all lanes one detector column jj at the angle whose trig row is (c, sn, xc, yc).  The plane
 // steps walked are the union of the lanes' z ranges (wave-wide min/max).  Returns the lane's cos^3 factor.
-__device__ __forceinline__ double first_lane(double x) {
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readfirstlane((int)b);
-    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
 
 // The fields of a ray's xy path that do not depend on which box hit is the entry are the same, bit for
 // bit, on every lane of the wave (the lanes differ only in z): the major axis (|xi - xf| > |yi - yf| is
