@@ -329,64 +329,9 @@ __device__ __forceinline__ float bp_ld(__amdgpu_buffer_rsrc_t r, int voff, int s
 // or one tap.  (A precomputed per-(column, angle) table was measured slower: each z-block re-reads it.)
 // A slice's taps are raw buffer loads of one projection: one 32-bit byte offset per slice (row * 4W + 4 column,
 // a 24-bit multiply-add; the launcher checks H < 2^24, 4W < 2^24 and 4HW < 2^31), the next column as the
-// instruction's immediate offset and the next row as its scalar offset.  The scalar offset is added after the
-// buffer's range check, so the next-row taps of the last row read row H: for every projection but the last
-// that is the next projection's first row (unused: the last row keeps its one row of taps), and the last
-// projection (NEXTROW_IN_VOFFSET) adds the next row into the checked vector offset instead, so that no load
-// leaves the caller's buffer (past its end they read 0).
-template <int ZB, bool NEXTROW_IN_VOFFSET>
-__device__ __forceinline__ void bp_angle(float (&acc)[ZB], const float (&zz)[ZB], double xx, double yy,
-                                         const float* __restrict__ sino, int th, int H, int W, int n,
-                                         const double* __restrict__ atab, double xfactor, double pitchW, double rpW,
-                                         double hW, float hH, float ipW, int rowb, float hm1) {
-    const double c = atab[th], s = atab[n + th], dbeta = atab[2 * n + th], xs = atab[3 * n + th];
-    // horizontal: FP64 in the reference's operation order (its base column floor(W/2 + U + xs - 0.5)
-    // and weight frac(U) jump at different points, so U must round as the reference's does)
-    double X = xx * c + yy * s;
-    double Y = -xx * s + yy * c;
-    double L2 = 1.0 - div_known(Y, 100.0, 0.01);  // = 1.0 - Y / 100.0 bit for bit
-    double U = div_known(X / L2, pitchW, rpW);    // = X / L2 / pitchW bit for bit
-    double fU = floor(U);
-    double w = U - fU;
-    int UU = (int)floor(hW + U + xs - 0.5);
-    const bool ucol = (UU >= 0 && UU < W - 1), uedge = (UU == W - 1);
-    const bool inpanel = ucol || uedge;
-    // vertical: an ordinary bilinear tap, continuous in the row coordinate, and the angle weight: FP32
-    const float rL2 = __builtin_amdgcn_rcpf((float)L2);
-    const float iL = rL2 * ipW;
-    const float wgt = rL2 * rL2 * (float)(dbeta * xfactor);
-    const __amdgpu_buffer_rsrc_t P = bp_rsrc(sino + (size_t)th * H * W, H * W * 4);
-    // the last column: horizontal weight 0, so the lerps keep its one tap (a + 0 (b - a) = a for finite taps;
-    // the unused neighbour is the next row's first column, or 0 past the projection's end)
-    const float wf = ucol ? (float)w : 0.f;
-    const unsigned UU4 = inpanel ? (unsigned)UU * 4u : 0u;
-    // Branch-free: the four taps are loaded unconditionally (row clamped into the panel) and weights and
-    // selects keep the reference's cases, so for finite filtered projections every kept value is the one the
-    // branching form computes, and no angle waits for the previous angle's loads.
-#pragma unroll
-    for (int i = 0; i < ZB; ++i) {
-        const float ZETA = fmaf(zz[i], iL, hH);
-        const float fz = floorf(ZETA);
-        const float wz = ZETA - fz;
-        const float fzc = __builtin_amdgcn_fmed3f(fz, 0.f, hm1);  // the row clamped into the panel
-        const int vo = (int)(__umul24((unsigned)(int)fzc, (unsigned)rowb) + UU4);
-        const float a0 = bp_ld(P, vo, 0), a1 = bp_ld(P, vo + 4, 0);
-        float b0, b1;
-        if (NEXTROW_IN_VOFFSET) {
-            b0 = bp_ld(P, vo + rowb, 0);
-            b1 = bp_ld(P, vo + rowb + 4, 0);
-        } else {
-            b0 = bp_ld(P, vo, rowb);
-            b1 = bp_ld(P, vo + 4, rowb);
-        }
-        const float h0 = bp_lerp(a0, a1, wf), h1 = bp_lerp(b0, b1, wf);
-        const float sum = fz < hm1 ? bp_lerp(h0, h1, wz) : h0;  // the last row: its one row of taps
-        // a tap set the reference skips (a row outside the panel: fz clamped) adds 0 x wgt (the weight is
-        // always finite): acc unchanged
-        acc[i] = fmaf((inpanel && fzc == fz) ? sum : 0.f, wgt, acc[i]);
-    }
-}
-
+// instruction's immediate offset and the next row as its scalar offset.  The resource's range check covers
+// the sum of the vector, scalar and immediate offsets (on gfx950 a load whose scalar offset alone carries it
+// past the record count reads 0), so the unused next-row taps of the last row read 0 on every projection.
 template <int ZB>
 __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, const float* __restrict__ sino, int R,
                                                      int Z, double zstart, double step, double scale, int H, int W,
@@ -414,11 +359,48 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
     const double rpW = 1.0 / pitchW;  // correctly rounded: the divisions by pitchW below are exact products
     const int rowb = W * 4;           // byte offset of the next row
     const float hm1 = (float)(H - 1);  // the last row (rows are exact in FP32: H < 2^24)
-    for (int th = 0; th < n - 1; ++th)
-        bp_angle<ZB, false>(acc, zz, xx, yy, sino, th, H, W, n, atab, xfactor, pitchW, rpW, hW, hH, ipW, rowb, hm1);
-    if (n > 0)
-        bp_angle<ZB, true>(acc, zz, xx, yy, sino, n - 1, H, W, n, atab, xfactor, pitchW, rpW, hW, hH, ipW, rowb,
-                           hm1);
+    for (int th = 0; th < n; ++th) {
+        const double c = atab[th], s = atab[n + th], dbeta = atab[2 * n + th], xs = atab[3 * n + th];
+        // horizontal: FP64 in the reference's operation order (its base column floor(W/2 + U + xs - 0.5)
+        // and weight frac(U) jump at different points, so U must round as the reference's does)
+        double X = xx * c + yy * s;
+        double Y = -xx * s + yy * c;
+        double L2 = 1.0 - div_known(Y, 100.0, 0.01);  // = 1.0 - Y / 100.0 bit for bit
+        double U = div_known(X / L2, pitchW, rpW);    // = X / L2 / pitchW bit for bit
+        double fU = floor(U);
+        double w = U - fU;
+        int UU = (int)floor(hW + U + xs - 0.5);
+        const bool ucol = (UU >= 0 && UU < W - 1), uedge = (UU == W - 1);
+        const bool inpanel = ucol || uedge;
+        // vertical: an ordinary bilinear tap, continuous in the row coordinate, and the angle weight: FP32
+        const float rL2 = __builtin_amdgcn_rcpf((float)L2);
+        const float iL = rL2 * ipW;
+        const float wgt = rL2 * rL2 * (float)(dbeta * xfactor);
+        const __amdgpu_buffer_rsrc_t P = bp_rsrc(sino + (size_t)th * H * W, H * W * 4);
+        // the last column: horizontal weight 0, so the lerps keep its one tap (a + 0 (b - a) = a for finite taps;
+        // the unused neighbour is the next row's first column, or 0 past the projection's end)
+        const float wf = ucol ? (float)w : 0.f;
+        const unsigned UU4 = inpanel ? (unsigned)UU * 4u : 0u;
+        // Branch-free: the four taps are loaded unconditionally (row clamped into the panel; a tap past the
+        // projection's end reads 0) and weights and selects keep the reference's cases, so for finite filtered
+        // projections every kept value is the one the branching form computes, and no angle waits for the
+        // previous angle's loads.
+#pragma unroll
+        for (int i = 0; i < ZB; ++i) {
+            const float ZETA = fmaf(zz[i], iL, hH);
+            const float fz = floorf(ZETA);
+            const float wz = ZETA - fz;
+            const float fzc = __builtin_amdgcn_fmed3f(fz, 0.f, hm1);  // the row clamped into the panel
+            const int vo = (int)(__umul24((unsigned)(int)fzc, (unsigned)rowb) + UU4);
+            const float a0 = bp_ld(P, vo, 0), a1 = bp_ld(P, vo + 4, 0);
+            const float b0 = bp_ld(P, vo, rowb), b1 = bp_ld(P, vo + 4, rowb);
+            const float h0 = bp_lerp(a0, a1, wf), h1 = bp_lerp(b0, b1, wf);
+            const float sum = fz < hm1 ? bp_lerp(h0, h1, wz) : h0;  // the last row: its one row of taps
+            // a tap set the reference skips (a row outside the panel: fz clamped) adds 0 x wgt (the weight is
+            // always finite): acc unchanged
+            acc[i] = fmaf((inpanel && fzc == fz) ? sum : 0.f, wgt, acc[i]);
+        }
+    }
 #pragma unroll
     for (int i = 0; i < ZB; ++i)
         if (z0 + i < Z) recon[((size_t)(z0 + i) * R + y) * R + x] += acc[i];
