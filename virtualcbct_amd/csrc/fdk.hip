@@ -150,8 +150,9 @@ static int get_plan(int W, double pitch_iso, FilterPlan** out, double alpha = kR
 // separate into the real and imaginary parts).  Each is zero-padded to L = 2^logL, transformed with a
 // Stockham FFT in LDS (radix-4 stages, one radix-2 stage when logL is odd; each of the 256 threads does one
 // butterfly per stage), multiplied by the response H, transformed back, and cosine-weighted at the output
-// pixel.  A block filters FSIG row pairs in turn, so the twiddle and response tables it stages in LDS are
-// read from L2 once per 2 FSIG rows (up to L = 2048; beyond, they are read from global memory).
+// pixel.  A block filters FSIG row pairs in turn.  The twiddle and response tables are read from global
+// memory (they stay in the caches): staging them in LDS as well doubled the block's LDS, and the occupancy it
+// cost was more than the loads it saved (L = 2048: 64 KB per block, 2 blocks per CU).
 constexpr int FT = 256;
 constexpr int FSIG = 8;
 constexpr int FPF = 8;  // pixels per thread per row held in registers: W <= FPF * FT
@@ -218,25 +219,14 @@ template <bool HF>
 __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, float* __restrict__ out, int H, int W,
                                                long nrows, int L, int logL, const float2* __restrict__ Hf,
                                                const float2* __restrict__ tw, const float* __restrict__ xc_tab,
-                                               const float* __restrict__ yc_tab, float pitchW, int staged,
+                                               const float* __restrict__ yc_tab, float pitchW,
                                                const float* __restrict__ ecol, int Wsrc, int off) {
-    // LDS: [A][B] ping-pong buffers, then (staged) the twiddle and response tables; without staging (L above
-    // 2048, where the four would exceed 64 KB) the tables are read from global memory
+    // LDS: the [A][B] ping-pong buffers
     extern __shared__ float2 lds[];
     float2* A = lds;
     float2* B = lds + L;
     const float2* T = tw;
     const float2* HL = Hf;
-    if (staged) {
-        float2* t = lds + 2 * L;
-        float2* h = lds + 3 * L;
-        for (int k = threadIdx.x; k < L; k += FT) {
-            t[k] = tw[k];
-            h[k] = Hf[k];
-        }
-        T = t;
-        HL = h;
-    }
     // the next row pair's pixels are loaded into registers while the current pair is transformed (the
     // block barriers wait for LDS traffic only, so these loads stay in flight across them)
     float pa[FPF], pb[FPF];
@@ -611,16 +601,12 @@ static int launch_filter(const FilterPlan* P, int H, int W, int n, const float* 
     if (nblk > INT_MAX) return fail(VCBCT_EINVAL, "filter_sinogram: too many rows");
     const float pitchW = (float)(pitch_iso * 800 / W);
     const size_t sig_bytes = 2 * (size_t)P->L * sizeof(float2);  // the two signal buffers
-    const size_t tab_bytes = 2 * (size_t)P->L * sizeof(float2);
-    const int staged = sig_bytes + tab_bytes <= 65536;
     if (ecol)
-        hipLaunchKernelGGL(k_filter<true>, dim3((unsigned)nblk), dim3(FT), sig_bytes + (staged ? tab_bytes : 0), st,
-                           dp, dq, H, W, nrows, P->L, P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, staged,
-                           ecol, Wsrc, off);
+        hipLaunchKernelGGL(k_filter<true>, dim3((unsigned)nblk), dim3(FT), sig_bytes, st, dp, dq, H, W, nrows, P->L,
+                           P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, ecol, Wsrc, off);
     else
-        hipLaunchKernelGGL(k_filter<false>, dim3((unsigned)nblk), dim3(FT), sig_bytes + (staged ? tab_bytes : 0), st,
-                           dp, dq, H, W, nrows, P->L, P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, staged,
-                           nullptr, W, 0);
+        hipLaunchKernelGGL(k_filter<false>, dim3((unsigned)nblk), dim3(FT), sig_bytes, st, dp, dq, H, W, nrows, P->L,
+                           P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, nullptr, W, 0);
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
