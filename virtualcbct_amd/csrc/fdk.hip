@@ -74,7 +74,7 @@ static void fft64(std::vector<std::complex<double>>& a) {
 }
 
 struct FilterPlan {
-    int W, L, logL;
+    int W, L, logL, r5;  // L = 2^logL, or (r5) 5 x 2^logL: the shorter one holding the 2W - 1 taps
     float2* d_H;    // [L] frequency response of the circularly wrapped h, scaled by 1/L
     float2* d_tw;   // twiddles per stage, contiguous in k (see fft_lds); L entries
 };
@@ -110,9 +110,21 @@ static int get_plan(int W, double pitch_iso, FilterPlan** out, double alpha = kR
     }
     FilterPlan P;
     P.W = W;
+    // the circular convolution of length L equals the linear one on the W outputs when L >= 2W - 1: the
+    // shortest power of two (at least 64), or 5 x 2^m when that is shorter (630 columns: 1280, not 2048)
     P.L = 64;
     P.logL = 6;
     while (P.L < 2 * W - 1) { P.L <<= 1; P.logL++; }
+    P.r5 = 0;
+    {
+        int L5 = 80, m = 4;
+        while (L5 < 2 * W - 1) { L5 <<= 1; m++; }
+        if (L5 < P.L) {
+            P.L = L5;
+            P.logL = m;
+            P.r5 = 1;
+        }
+    }
     if (P.L > 4096) return fail(VCBCT_EINVAL, "filter: projection width %d too large (max 2048)", W);
     std::vector<std::complex<double>> h(P.L, 0.0);
     h[0] = ramp_centre(W, pitch_iso, alpha);
@@ -120,23 +132,47 @@ static int get_plan(int W, double pitch_iso, FilterPlan** out, double alpha = kR
         h[d] = ramp_offcentre(d, pitch_iso, W, alpha);           // out[m] gets h(d) q[m-d]
         h[P.L - d] = ramp_offcentre(-d, pitch_iso, W, alpha);
     }
-    fft64(h);
+    if (!P.r5) {
+        fft64(h);
+    } else {  // a direct FP64 DFT over the 2W - 1 nonzero taps (once per plan)
+        std::vector<std::complex<double>> e(P.L), Hd(P.L, 0.0);
+        for (int m = 0; m < P.L; ++m) e[m] = std::complex<double>(cos(-2.0 * kPI * m / P.L), sin(-2.0 * kPI * m / P.L));
+        for (int k = 0; k < P.L; ++k) {
+            std::complex<double> acc = h[0];
+            for (int d = 1; d <= W - 1; ++d)
+                acc += h[d] * e[(size_t)d * k % P.L] + h[P.L - d] * e[(size_t)(P.L - d) * k % P.L];
+            Hd[k] = acc;
+        }
+        h.swap(Hd);
+    }
     std::vector<float2> Hf(P.L), tw(P.L);
     for (int k = 0; k < P.L; ++k) Hf[k] = make_float2((float)(h[k].real() / P.L), (float)(h[k].imag() / P.L));
     for (int k = 0; k < P.L; ++k) tw[k] = make_float2((float)cos(-2.0 * kPI * k / P.L), (float)sin(-2.0 * kPI * k / P.L));
-    // per-stage layout: radix-4 stage s (Ns = 4^s) holds w^(k r) = tw[(k r) << (logL - 2 - 2s)] at
-    // [base_s + (r - 1) Ns + k], r = 1..3, k < Ns; a closing radix-2 stage holds tw[k], k < L/2, after them
+    // per-stage layout: radix-4 stage s (Ns = 4^s) holds w^(k r), w = exp(-2 pi i / (4 Ns)), at
+    // [base_s + (r - 1) Ns + k], r = 1..3, k < Ns; a radix-2 stage (odd logL) holds w^k, w = exp(-2 pi i /
+    // (2 Ns)), after them; with r5 a closing radix-5 stage (Ns = L / 5) holds w^(k r), w = exp(-2 pi i / L),
+    // r = 1..4, last.  For a power of two these are tw[] entries (w^(k r) = tw[(k r) << (logL - 2 - 2s)]).
     std::vector<float2> tc(P.L, make_float2(0.f, 0.f));
+    auto wr = [](double num, double den) {
+        return make_float2((float)cos(-2.0 * kPI * num / den), (float)sin(-2.0 * kPI * num / den));
+    };
     int base = 0, Ns = 1;
     for (int st = 0; st < P.logL / 2; ++st) {
         const int shift = P.logL - 2 - 2 * st;
         for (int r = 1; r <= 3; ++r)
-            for (int k = 0; k < Ns; ++k) tc[base + (r - 1) * Ns + k] = tw[(k * r) << shift];
+            for (int k = 0; k < Ns; ++k)
+                tc[base + (r - 1) * Ns + k] = P.r5 ? wr((double)k * r, 4.0 * Ns) : tw[(k * r) << shift];
         base += 3 * Ns;
         Ns <<= 2;
     }
-    if (P.logL & 1)
-        for (int k = 0; k < Ns; ++k) tc[base + k] = tw[k];
+    if (P.logL & 1) {
+        for (int k = 0; k < Ns; ++k) tc[base + k] = P.r5 ? wr((double)k, 2.0 * Ns) : tw[k];
+        base += Ns;
+        Ns <<= 1;
+    }
+    if (P.r5)
+        for (int r = 1; r <= 4; ++r)
+            for (int k = 0; k < Ns; ++k) tc[base + (r - 1) * Ns + k] = tw[k * r];
     VCBCT_HIP(dev_malloc((void**)&P.d_H, P.L * sizeof(float2)));
     VCBCT_HIP(dev_malloc((void**)&P.d_tw, P.L * sizeof(float2)));
     VCBCT_HIP(hipMemcpy(P.d_H, Hf.data(), P.L * sizeof(float2), hipMemcpyHostToDevice));
@@ -147,9 +183,10 @@ static int get_plan(int W, double pitch_iso, FilterPlan** out, double alpha = kR
 }
 
 // Two detector rows form one complex signal z = q1 + i q2 (the kernel is real, so the two convolutions
-// separate into the real and imaginary parts).  Each is zero-padded to L = 2^logL, transformed with a
-// Stockham FFT in LDS (radix-4 stages, one radix-2 stage when logL is odd; each of the 256 threads does one
-// butterfly per stage), multiplied by the response H, transformed back, and cosine-weighted at the output
+// separate into the real and imaginary parts).  Each is zero-padded to L (2^logL, or 5 x 2^logL when that is
+// shorter), transformed with a Stockham FFT in LDS (radix-4 stages, one radix-2 stage when logL is odd, a
+// closing radix-5 stage for 5 x 2^logL; the 256 threads share each stage's butterflies), multiplied by the
+// response H, transformed back, and cosine-weighted at the output
 // pixel.  A block filters FSIG row pairs in turn.  The twiddle and response tables are read from global
 // memory (they stay in the caches): staging them in LDS as well doubled the block's LDS, and the occupancy it
 // cost was more than the loads it saved (L = 2048: 64 KB per block, 2 blocks per CU).
@@ -160,12 +197,13 @@ constexpr int FPF = 8;  // pixels per thread per row held in registers: W <= FPF
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
 // x -> DFT(x) (INV: the unscaled inverse) through the ping-pong buffers x, y; returns the buffer holding it.
-// Stage s of radix R has sub-transform size Ns = R^s: butterfly j reads x[j + r L / R], scales input r by
+// L = 2^logL, or (R5) 5 x 2^logL.  A stage of radix R whose earlier stages' radices multiply to Ns (its
+// sub-transform size) has L / R butterflies: butterfly j reads x[j + r L / R], scales input r by
 // w^(k r), w = exp(-2 pi i / (R Ns)), k = j mod Ns, and writes y[(j / Ns) R Ns + k + r Ns].  T holds each
-// stage's twiddles contiguous in k (w^(k r) at [base + (r - 1) Ns + k], base advancing by 3 Ns per radix-4
-// stage), so the lanes of a wave read consecutive or equal addresses: no LDS bank conflicts (a single
-// exp(-2 pi i m / L) table read at (k r) << shift had lanes 16 to 64 float2 apart, all in one bank).
-template <bool INV>
+// stage's twiddles contiguous in k (w^(k r) at [base + (r - 1) Ns + k], base advancing by (R - 1) Ns per
+// stage), so the lanes of a wave read consecutive or equal addresses (a single exp(-2 pi i m / L) table read
+// at (k r) << shift had lanes 16 to 64 float2 apart).
+template <bool INV, bool R5>
 __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T, int L, int logL) {
     int Ns = 1, base = 0;
     const int q = L >> 2;
@@ -207,6 +245,45 @@ __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T
         }
         __syncthreads();
         float2* t = x; x = y; y = t;
+        base += Ns;
+        Ns <<= 1;
+    }
+    if (R5) {  // closing radix 5: Ns = L / 5, so k = j and the outputs go to j + r Ns
+        // DFT5 with c_m = cos(2 pi m / 5), s_m = sin(2 pi m / 5): out0 = v0 + t1 + t2,
+        // out1,4 = v0 + c1 t1 + c2 t2 -+ i (s1 t3 + s2 t4), out2,3 = v0 + c2 t1 + c1 t2 -+ i (s2 t3 - s1 t4)
+        // (t1 = v1 + v4, t2 = v2 + v3, t3 = v1 - v4, t4 = v2 - v3; the inverse swaps the signs of i)
+        const float c1 = 0.309016994374947424f, c2 = -0.809016994374947424f;
+        const float s1 = 0.951056516295153572f, s2 = 0.587785252292473129f;
+        const float2* Ts = T + base;
+        for (int j = threadIdx.x; j < Ns; j += FT) {
+            float2 v[5];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) v[r] = x[j + r * Ns];
+#pragma unroll
+            for (int r = 1; r < 5; ++r) {
+                float2 w = Ts[(r - 1) * Ns + j];
+                if (INV) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+            const float2 t1 = make_float2(v[1].x + v[4].x, v[1].y + v[4].y);
+            const float2 t2 = make_float2(v[2].x + v[3].x, v[2].y + v[3].y);
+            const float2 t3 = make_float2(v[1].x - v[4].x, v[1].y - v[4].y);
+            const float2 t4 = make_float2(v[2].x - v[3].x, v[2].y - v[3].y);
+            const float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+            const float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+            const float2 b1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+            const float2 b2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+            // -i b = (b.y, -b.x); +i b = (-b.y, b.x)
+            const float2 ib1 = INV ? make_float2(-b1.y, b1.x) : make_float2(b1.y, -b1.x);
+            const float2 ib2 = INV ? make_float2(-b2.y, b2.x) : make_float2(b2.y, -b2.x);
+            y[j] = make_float2(v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y);
+            y[j + Ns] = make_float2(a1.x + ib1.x, a1.y + ib1.y);
+            y[j + 2 * Ns] = make_float2(a2.x + ib2.x, a2.y + ib2.y);
+            y[j + 3 * Ns] = make_float2(a2.x - ib2.x, a2.y - ib2.y);
+            y[j + 4 * Ns] = make_float2(a1.x - ib1.x, a1.y - ib1.y);
+        }
+        __syncthreads();
+        float2* t = x; x = y; y = t;
     }
     return x;
 }
@@ -215,7 +292,7 @@ __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T
 // placed at column `off` of the W-column padded panel; a real column c of projection th contributes
 // ecol[th][c] (-log p) (= -log p^(2w), the weighted pixel's line integral; exponent 0 gives 0 whatever p), a
 // padding column 0 (-log 1).  HF false: no weighting, Wsrc = W and off = 0.
-template <bool HF>
+template <bool HF, bool R5>
 __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, float* __restrict__ out, int H, int W,
                                                long nrows, int L, int logL, const float2* __restrict__ Hf,
                                                const float2* __restrict__ tw, const float* __restrict__ xc_tab,
@@ -274,11 +351,11 @@ __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, f
         for (int k = W + threadIdx.x; k < L; k += FT) A[k] = make_float2(0.f, 0.f);
         __syncthreads();
         if (sig + 1 < FSIG && r1 + 2 < nrows) prefetch(r1 + 2);
-        float2* X = fft_lds<false>(A, B, T, L, logL);
+        float2* X = fft_lds<false, R5>(A, B, T, L, logL);
         float2* Y = (X == A) ? B : A;
         for (int k = threadIdx.x; k < L; k += FT) X[k] = cmul(X[k], HL[k]);
         __syncthreads();
-        X = fft_lds<true>(X, Y, T, L, logL);
+        X = fft_lds<true, R5>(X, Y, T, L, logL);
         // cosine weight at the output pixel (F/filterSinogram.cpp:92-99, applied at :103 and :108)
         for (int s = 0; s < 2; ++s) {
             const long r = r1 + s;
@@ -601,12 +678,17 @@ static int launch_filter(const FilterPlan* P, int H, int W, int n, const float* 
     if (nblk > INT_MAX) return fail(VCBCT_EINVAL, "filter_sinogram: too many rows");
     const float pitchW = (float)(pitch_iso * 800 / W);
     const size_t sig_bytes = 2 * (size_t)P->L * sizeof(float2);  // the two signal buffers
-    if (ecol)
-        hipLaunchKernelGGL(k_filter<true>, dim3((unsigned)nblk), dim3(FT), sig_bytes, st, dp, dq, H, W, nrows, P->L,
-                           P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, ecol, Wsrc, off);
-    else
-        hipLaunchKernelGGL(k_filter<false>, dim3((unsigned)nblk), dim3(FT), sig_bytes, st, dp, dq, H, W, nrows, P->L,
-                           P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, nullptr, W, 0);
+#define VCBCT_FILT(HFV, R5V, EC, WS, OF)                                                                              \
+    hipLaunchKernelGGL((k_filter<HFV, R5V>), dim3((unsigned)nblk), dim3(FT), sig_bytes, st, dp, dq, H, W, nrows, P->L, \
+                       P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, EC, WS, OF)
+    if (ecol) {
+        if (P->r5) VCBCT_FILT(true, true, ecol, Wsrc, off);
+        else VCBCT_FILT(true, false, ecol, Wsrc, off);
+    } else {
+        if (P->r5) VCBCT_FILT(false, true, nullptr, W, 0);
+        else VCBCT_FILT(false, false, nullptr, W, 0);
+    }
+#undef VCBCT_FILT
     VCBCT_HIP(hipGetLastError());
     return 0;
 }
