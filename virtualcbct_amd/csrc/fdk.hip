@@ -207,6 +207,7 @@ template <bool INV, bool R5>
 __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T, int L, int logL) {
     int Ns = 1, base = 0;
     const int q = L >> 2;
+#pragma unroll
     for (int s = 0; s < logL / 2; ++s) {
         const float2* Ts = T + base;
         for (int j = threadIdx.x; j < q; j += FT) {
@@ -292,12 +293,16 @@ __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T
 // placed at column `off` of the W-column padded panel; a real column c of projection th contributes
 // ecol[th][c] (-log p) (= -log p^(2w), the weighted pixel's line integral; exponent 0 gives 0 whatever p), a
 // padding column 0 (-log 1).  HF false: no weighting, Wsrc = W and off = 0.
-template <bool HF, bool R5>
+// LOGLC: logL as a compile-time constant for the common lengths (1024; 1280 = 5 x 256), so that the stage
+// loops unroll and every stride and twiddle offset is an immediate; 0: L and logL as given at run time.
+template <bool HF, bool R5, int LOGLC>
 __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, float* __restrict__ out, int H, int W,
-                                               long nrows, int L, int logL, const float2* __restrict__ Hf,
+                                               long nrows, int L_rt, int logL_rt, const float2* __restrict__ Hf,
                                                const float2* __restrict__ tw, const float* __restrict__ xc_tab,
                                                const float* __restrict__ yc_tab, float pitchW,
                                                const float* __restrict__ ecol, int Wsrc, int off) {
+    const int L = LOGLC ? ((R5 ? 5 : 1) << LOGLC) : L_rt;
+    const int logL = LOGLC ? LOGLC : logL_rt;
     // LDS: the [A][B] ping-pong buffers
     extern __shared__ float2 lds[];
     float2* A = lds;
@@ -678,15 +683,21 @@ static int launch_filter(const FilterPlan* P, int H, int W, int n, const float* 
     if (nblk > INT_MAX) return fail(VCBCT_EINVAL, "filter_sinogram: too many rows");
     const float pitchW = (float)(pitch_iso * 800 / W);
     const size_t sig_bytes = 2 * (size_t)P->L * sizeof(float2);  // the two signal buffers
-#define VCBCT_FILT(HFV, R5V, EC, WS, OF)                                                                              \
-    hipLaunchKernelGGL((k_filter<HFV, R5V>), dim3((unsigned)nblk), dim3(FT), sig_bytes, st, dp, dq, H, W, nrows, P->L, \
-                       P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, EC, WS, OF)
+#define VCBCT_FILT(HFV, R5V, LG, EC, WS, OF)                                                                          \
+    hipLaunchKernelGGL((k_filter<HFV, R5V, LG>), dim3((unsigned)nblk), dim3(FT), sig_bytes, st, dp, dq, H, W, nrows,  \
+                       P->L, P->logL, P->d_H, P->d_tw, d_ctab, d_ctab + n, pitchW, EC, WS, OF)
+    // the lengths with a compile-time instantiation: 1024 (400 columns) and 1280 (630 columns)
+    const bool l1024 = !P->r5 && P->logL == 10, l1280 = P->r5 && P->logL == 8;
     if (ecol) {
-        if (P->r5) VCBCT_FILT(true, true, ecol, Wsrc, off);
-        else VCBCT_FILT(true, false, ecol, Wsrc, off);
+        if (l1280) VCBCT_FILT(true, true, 8, ecol, Wsrc, off);
+        else if (l1024) VCBCT_FILT(true, false, 10, ecol, Wsrc, off);
+        else if (P->r5) VCBCT_FILT(true, true, 0, ecol, Wsrc, off);
+        else VCBCT_FILT(true, false, 0, ecol, Wsrc, off);
     } else {
-        if (P->r5) VCBCT_FILT(false, true, nullptr, W, 0);
-        else VCBCT_FILT(false, false, nullptr, W, 0);
+        if (l1280) VCBCT_FILT(false, true, 8, nullptr, W, 0);
+        else if (l1024) VCBCT_FILT(false, false, 10, nullptr, W, 0);
+        else if (P->r5) VCBCT_FILT(false, true, 0, nullptr, W, 0);
+        else VCBCT_FILT(false, false, 0, nullptr, W, 0);
     }
 #undef VCBCT_FILT
     VCBCT_HIP(hipGetLastError());
