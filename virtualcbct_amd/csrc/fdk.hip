@@ -203,6 +203,11 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(
 // stage's twiddles contiguous in k (w^(k r) at [base + (r - 1) Ns + k], base advancing by (R - 1) Ns per
 // stage), so the lanes of a wave read consecutive or equal addresses (a single exp(-2 pi i m / L) table read
 // at (k r) << shift had lanes 16 to 64 float2 apart).
+// Pruning, for the filter's signals (W <= L / 2 real samples, since L >= 2W - 1, and W <= L / 2 outputs
+// read): the first stage (Ns = 1) has unit twiddles, and in the forward transform its inputs at L / 2 and
+// beyond are zero padding, so it neither loads nor multiplies them; the inverse's last stage forms only the
+// outputs below L / 2.  The values are the unpruned ones (x + 0 = x and x (1 + 0i) = x up to the sign of a
+// zero), and the buffer returned by the inverse holds [0, L / 2) only.
 template <bool INV, bool R5>
 __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T, int L, int logL) {
     int Ns = 1, base = 0;
@@ -210,22 +215,39 @@ __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T
 #pragma unroll
     for (int s = 0; s < logL / 2; ++s) {
         const float2* Ts = T + base;
+        const bool first = s == 0;
+        const bool last_half = INV && !R5 && !(logL & 1) && s == logL / 2 - 1;  // outputs >= L / 2 unused
         for (int j = threadIdx.x; j < q; j += FT) {
             const int k = j & (Ns - 1);
-            float2 v0 = x[j], v1 = x[j + q], v2 = x[j + 2 * q], v3 = x[j + 3 * q];
-            float2 w1 = Ts[k], w2 = Ts[Ns + k], w3 = Ts[2 * Ns + k];
-            if (INV) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
-            v1 = cmul(v1, w1);
-            v2 = cmul(v2, w2);
-            v3 = cmul(v3, w3);
-            const float2 a0 = make_float2(v0.x + v2.x, v0.y + v2.y), a1 = make_float2(v0.x - v2.x, v0.y - v2.y);
-            const float2 a2 = make_float2(v1.x + v3.x, v1.y + v3.y), d = make_float2(v1.x - v3.x, v1.y - v3.y);
+            float2 a0, a1, a2, d;
+            if (first && !INV) {  // v2 = v3 = 0, unit twiddles
+                const float2 v0 = x[j], v1 = x[j + q];
+                a0 = v0;
+                a1 = v0;
+                a2 = v1;
+                d = v1;
+            } else {
+                float2 v0 = x[j], v1 = x[j + q], v2 = x[j + 2 * q], v3 = x[j + 3 * q];
+                if (!first) {
+                    float2 w1 = Ts[k], w2 = Ts[Ns + k], w3 = Ts[2 * Ns + k];
+                    if (INV) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+                    v1 = cmul(v1, w1);
+                    v2 = cmul(v2, w2);
+                    v3 = cmul(v3, w3);
+                }
+                a0 = make_float2(v0.x + v2.x, v0.y + v2.y);
+                a1 = make_float2(v0.x - v2.x, v0.y - v2.y);
+                a2 = make_float2(v1.x + v3.x, v1.y + v3.y);
+                d = make_float2(v1.x - v3.x, v1.y - v3.y);
+            }
             const float2 a3 = INV ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);  // (+-i) (v1 - v3)
             const int o = ((j - k) << 2) + k;
             y[o] = make_float2(a0.x + a2.x, a0.y + a2.y);
             y[o + Ns] = make_float2(a1.x + a3.x, a1.y + a3.y);
-            y[o + 2 * Ns] = make_float2(a0.x - a2.x, a0.y - a2.y);
-            y[o + 3 * Ns] = make_float2(a1.x - a3.x, a1.y - a3.y);
+            if (!last_half) {
+                y[o + 2 * Ns] = make_float2(a0.x - a2.x, a0.y - a2.y);
+                y[o + 3 * Ns] = make_float2(a1.x - a3.x, a1.y - a3.y);
+            }
         }
         __syncthreads();
         float2* t = x; x = y; y = t;
@@ -242,7 +264,7 @@ __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T
             const float2 bw = cmul(x[j + half], w);
             const int o = ((j - k) << 1) + k;
             y[o] = make_float2(a.x + bw.x, a.y + bw.y);
-            y[o + Ns] = make_float2(a.x - bw.x, a.y - bw.y);
+            if (!(INV && !R5)) y[o + Ns] = make_float2(a.x - bw.x, a.y - bw.y);  // the inverse's last: >= L / 2
         }
         __syncthreads();
         float2* t = x; x = y; y = t;
@@ -280,8 +302,10 @@ __device__ __forceinline__ float2* fft_lds(float2* x, float2* y, const float2* T
             y[j] = make_float2(v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y);
             y[j + Ns] = make_float2(a1.x + ib1.x, a1.y + ib1.y);
             y[j + 2 * Ns] = make_float2(a2.x + ib2.x, a2.y + ib2.y);
-            y[j + 3 * Ns] = make_float2(a2.x - ib2.x, a2.y - ib2.y);
-            y[j + 4 * Ns] = make_float2(a1.x - ib1.x, a1.y - ib1.y);
+            if (!INV) {  // the inverse's last stage: 3 Ns and 4 Ns are at or beyond L / 2
+                y[j + 3 * Ns] = make_float2(a2.x - ib2.x, a2.y - ib2.y);
+                y[j + 4 * Ns] = make_float2(a1.x - ib1.x, a1.y - ib1.y);
+            }
         }
         __syncthreads();
         float2* t = x; x = y; y = t;
@@ -353,7 +377,7 @@ __global__ __launch_bounds__(FT) void k_filter(const float* __restrict__ proj, f
             }
             A[k] = make_float2(q1, q2);
         }
-        for (int k = W + threadIdx.x; k < L; k += FT) A[k] = make_float2(0.f, 0.f);
+        for (int k = W + threadIdx.x; k < (L >> 1); k += FT) A[k] = make_float2(0.f, 0.f);  // read: [0, L / 2)
         __syncthreads();
         if (sig + 1 < FSIG && r1 + 2 < nrows) prefetch(r1 + 2);
         float2* X = fft_lds<false, R5>(A, B, T, L, logL);
