@@ -168,5 +168,14 @@ def check(rc):
     return rc
 
 
+def stream_arg(stream=None):
+    """The hipStream_t argument of a C-ABI call: `stream` (an integer handle) when given, else torch's current
+    stream on the current device."""
+    if stream is not None:
+        return C.c_void_p(stream)
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
 def ptr(a, t):
     return a.ctypes.data_as(t)

@@ -6,13 +6,7 @@ from __future__ import annotations
 import ctypes as C
 
 from ._lib import check, lib
-
-
-def _stream(stream):
-    if stream is not None:
-        return C.c_void_p(stream)
-    import torch
-    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+from ._lib import stream_arg as _stream
 
 
 class Batch:

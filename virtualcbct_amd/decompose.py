@@ -25,12 +25,8 @@ import numpy as np
 import torch
 
 from . import _lib
+from ._lib import stream_arg as _stream
 from .autograd import poly_logproj
-
-
-def _stream():
-    import ctypes as C
-    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
 def decompose_autograd(problems, iters=200, lr=0.01, init=None, nonneg=True, callback=None):

@@ -14,17 +14,12 @@ from __future__ import annotations
 import ctypes as C
 
 from ._lib import check, lib
+from ._lib import stream_arg as _stream
 
 
 def _torch():
     import torch
     return torch
-
-
-def _stream(stream):
-    if stream is not None:
-        return C.c_void_p(stream)
-    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
 
 
 def detector_counts(primary, scatter=None, I0=1.0e5, seed=0, raw=False, out=None, stream=None):

@@ -11,17 +11,12 @@ import ctypes as C
 import numpy as np
 
 from ._lib import P_D, check, lib, ptr
+from ._lib import stream_arg as _stream
 
 
 def _torch():
     import torch
     return torch
-
-
-def _stream(stream):
-    if stream is not None:
-        return C.c_void_p(stream)
-    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
 
 
 def prepareConstMap(projWidth, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752):

@@ -14,17 +14,12 @@ import ctypes as C
 import numpy as np
 
 from ._lib import check, lib
+from ._lib import stream_arg as _stream
 
 
 def _torch():
     import torch
     return torch
-
-
-def _stream(stream):
-    if stream is not None:
-        return C.c_void_p(stream)
-    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
 
 
 def lengthproj(projector, proj, ite=0, a0=0, a1=None, out=None, stream=None):

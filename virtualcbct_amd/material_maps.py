@@ -16,6 +16,7 @@ import os
 import numpy as np
 
 from ._lib import P_F, P_U16, VcbctError, check, lib
+from ._lib import stream_arg as _stream
 
 MATERIALS = ("H", "C", "N", "O", "P", "Ca")
 
@@ -23,12 +24,6 @@ MATERIALS = ("H", "C", "N", "O", "P", "Ca")
 def _torch():
     import torch
     return torch
-
-
-def _stream(stream):
-    if stream is not None:
-        return C.c_void_p(stream)
-    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
 
 
 def material_table(media_csv, organs_csv, randomize=True, seed=1):

@@ -13,18 +13,12 @@ import numpy as np
 
 from . import _lib
 from ._lib import P_D, P_F, P_U16, check, lib, ptr
+from ._lib import stream_arg as _stream
 
 
 def _torch():
     import torch
     return torch
-
-
-def _stream(stream):
-    if stream is not None:
-        return C.c_void_p(stream)
-    torch = _torch()
-    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
 class Projector:
