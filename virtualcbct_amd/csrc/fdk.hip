@@ -462,7 +462,8 @@ __global__ __launch_bounds__(256) void k_backproject(float* __restrict__ recon, 
         double X = xx * c + yy * s;
         double Y = -xx * s + yy * c;
         double L2 = 1.0 - div_known(Y, 100.0, 0.01);  // = 1.0 - Y / 100.0 bit for bit
-        double U = div_known(X / L2, pitchW, rpW);    // = X / L2 / pitchW bit for bit
+        // = X / L2 / pitchW bit for bit (|X| below ~1e3 cm and either 0 or above ~1e-17, L2 near 1)
+        double U = div_known(div_unscaled(X, L2), pitchW, rpW);
         double fU = floor(U);
         double w = U - fU;
         int UU = (int)floor(hW + U + xs - 0.5);
@@ -571,7 +572,8 @@ __global__ __launch_bounds__(256) void k_backproject_lds(float* __restrict__ rec
         double X = xx * c + yy * s;
         double Y = -xx * s + yy * c;
         double L2 = 1.0 - div_known(Y, 100.0, 0.01);  // = 1.0 - Y / 100.0 bit for bit
-        double U = div_known(X / L2, pitchW, rpW);    // = X / L2 / pitchW bit for bit
+        // = X / L2 / pitchW bit for bit (|X| below ~1e3 cm and either 0 or above ~1e-17, L2 near 1)
+        double U = div_known(div_unscaled(X, L2), pitchW, rpW);
         double fU = floor(U);
         double w = U - fU;
         int UU = (int)floor(hW + U + xs - 0.5);

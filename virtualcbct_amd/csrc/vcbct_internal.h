@@ -219,6 +219,22 @@ __host__ __device__ inline double div_known(double a, double b, double rb) {
     return fma(r, rb, q);
 }
 
+// n / d correctly rounded, for d and n / d normal and far from overflow and n normal or zero: the FP64
+// division sequence the compiler emits (hardware reciprocal, two Newton steps, quotient, one correction with
+// the remainder) without its scaling and special-case steps (v_div_scale, v_div_fixup, and v_div_fmas's
+// scaling), which leave such operands unchanged, so the quotient is the IEEE one bit for bit (for n = 0 up to
+// the sign of the zero).  Three instructions fewer than n / d.
+__device__ inline double div_unscaled(double n, double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    const double q = n * r;
+    const double res = fma(-d, q, n);
+    return fma(res, r, q);
+}
+
 // Zero-fill kernel (bytes must be a multiple of 4).
 int launch_zero(void* p, size_t bytes, hipStream_t st);
 
