@@ -236,6 +236,35 @@ def test_fdk_plan_half_fan(torch, side):
     b.close()
 
 
+@pytest.mark.parametrize("W,shift_px,L", [(300, 70.0, 1024), (400, 115.0, 1280)])
+def test_fdk_plan_half_fan_filter_lengths(torch, W, shift_px, L):
+    """The half-fan plan's fused filter on the transform lengths compiled in: 300 columns padded to 440 (a
+    1024-point transform) and 400 columns padded to 630 (1280 = 5 x 256, the 115 mm offset's panel).  Equal to the
+    one-shot weighting + filter to FP32 rounding, and to the oracle's filter of the oracle's weighted panel."""
+    from virtualcbct_amd import fdk
+    rng = np.random.default_rng(W)
+    n, H = 5, 6
+    proj = torch.from_numpy(rng.uniform(0.1, 1.0, (n, H, W)).astype(np.float32)).cuda()
+    ang = np.deg2rad(np.arange(n) * 40.0 - 90.0)
+    xs = (shift_px + 0.5 * rng.standard_normal(n)) * 40.0 / W  # the reference's unit: W / 40 columns
+    ys = np.full(n, 0.3)
+    plan = fdk.FdkPlan(H, W, n, 16, 2, 0, 0.45, 0.4, 1000, ang, xs, ys, half_fan=True)
+    Wp = plan.width
+    Lp = min(1 << (2 * Wp - 2).bit_length(), 80 << max(0, (2 * Wp - 2) // 80).bit_length())
+    print(f"W {W}: padded to {Wp} columns, transform length {Lp}")
+    assert Wp == fdk.half_fan_width(W, xs) and Lp == L
+    filt = plan.filter(proj)
+    pw, xo, yo, pit = fdk.offset_weight_device(proj, xs, ys)
+    one_f = fdk.filter_sinogram_device(pw, xo, yo, pit)
+    ef = float((filt - one_f).abs().max()) / float(one_f.abs().max())
+    pw_o, xo_o, yo_o = O.offset_weight(proj.cpu().numpy(), xs, ys)
+    ref = O.filter_sinogram(pw_o, xo_o, yo_o, pitch_iso=pit, acc64=True)
+    eo = float(np.abs(filt.cpu().numpy() - ref).max() / np.abs(ref).max())
+    print(f"  vs one-shot weighting + filter {ef:.3e}, vs the oracle's FP64 chain {eo:.3e} of max")
+    plan.close()
+    assert ef < 1e-5 and eo < 5e-5
+
+
 @pytest.mark.parametrize("R,Z,scale,n,H,W", [
     (40, 9, 0.4, 24, 48, 64),   # 6.4 cm blocks: footprints wider than the LDS tile, taps read from global memory
     (37, 5, 0.05, 30, 33, 50),  # 0.8 cm blocks: footprints staged; odd sizes, partial blocks
