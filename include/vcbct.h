@@ -264,6 +264,12 @@ int vcbct_prepare_const_map(int proj_width, double det_pitch_iso_cm, double* hos
  * truncated kernel (centre tap coeff, off-centre h(m-k), |m-k| <= W-1) done as a batched FFT
  * convolution, then the cosine weight Cm at the output pixel.  Device [n][H][W] in and out.
  * xshift/yshift are host arrays of n_proj entries. */
+/* No counterpart in the reference (its filter is a direct convolution): the FFT length that
+ * vcbct_filter_sinogram and the FDK plans use for a panel of proj_width columns, the shorter of the smallest
+ * power of two >= 64 and the smallest 5 x 2^m >= 80 holding the 2 proj_width - 1 samples of the linear
+ * convolution.  Host only, no device call.  VCBCT_EINVAL for proj_width outside 1..2048. */
+int vcbct_filter_fft_length(int proj_width);
+
 int vcbct_filter_sinogram(int proj_height, int proj_width, int n_proj, const float* dev_proj, float* dev_filtered,
                           const double* host_xshift, const double* host_yshift, double det_pitch_iso_cm,
                           void* stream);

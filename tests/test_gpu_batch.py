@@ -250,7 +250,7 @@ def test_fdk_plan_half_fan_filter_lengths(torch, W, shift_px, L):
     ys = np.full(n, 0.3)
     plan = fdk.FdkPlan(H, W, n, 16, 2, 0, 0.45, 0.4, 1000, ang, xs, ys, half_fan=True)
     Wp = plan.width
-    Lp = min(1 << (2 * Wp - 2).bit_length(), 80 << max(0, (2 * Wp - 2) // 80).bit_length())
+    Lp = fdk.filter_fft_length(Wp)
     print(f"W {W}: padded to {Wp} columns, transform length {Lp}")
     assert Wp == fdk.half_fan_width(W, xs) and Lp == L
     filt = plan.filter(proj)

@@ -232,3 +232,21 @@ def test_compat_shim_reprojection_angle_range(tmp_path):
         r = subprocess.run([exe, str(total), str(end), str(out), *extra], capture_output=True, text=True)
         assert r.returncode == 1 and msg in r.stderr, (r.returncode, r.stderr)
         assert not os.path.exists(str(out) + ".raw")
+
+
+def test_filter_fft_length_rule():
+    """The filter's transform length for every panel width it takes: at least 2W - 1 (so the circular
+    convolution equals the linear one on the W outputs), a power of two >= 64 or 5 x 2^m >= 80, and the shortest
+    such length; out-of-range widths are errors, not lengths."""
+    from virtualcbct_amd import fdk
+    pow2 = [1 << k for k in range(6, 13)]
+    five = [80 << k for k in range(0, 7)]
+    for W in range(1, 2049):
+        L = fdk.filter_fft_length(W)
+        ok = [c for c in pow2 + five if c >= 2 * W - 1]
+        assert L == min(ok), (W, L)
+    assert [fdk.filter_fft_length(W) for W in (16, 33, 300, 400, 630, 700, 1100, 2048)] == \
+        [64, 80, 640, 1024, 1280, 2048, 2560, 4096]
+    for W in (0, -3, 2049):
+        with pytest.raises(VcbctError):
+            fdk.filter_fft_length(W)

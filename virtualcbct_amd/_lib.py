@@ -98,6 +98,7 @@ SIGNATURES = {
                                 c_double, c_double, c_double, c_void_p]),
     "vcbct_projector_logproj_backward_saved": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "vcbct_prepare_const_map": (c_int, [c_int, c_double, P_D]),
+    "vcbct_filter_fft_length": (c_int, [c_int]),
     "vcbct_filter_sinogram": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, P_D, P_D, c_double, c_void_p]),
     "vcbct_reconstruct_image": (c_int, [c_int, c_int, c_int, c_double, c_double, c_void_p, c_int, c_int, c_int,
                                         c_void_p, c_int, P_D, P_D, c_double, c_void_p]),

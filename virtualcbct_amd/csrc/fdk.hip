@@ -98,6 +98,20 @@ struct FilterKeyHash {
 };
 static std::unordered_map<FilterKey, FilterPlan, FilterKeyHash> g_plans;  // per device, width, pitch and window
 
+// The transform length for a W-column panel: the circular convolution of length L equals the linear one on the
+// W outputs when L >= 2W - 1; L is the shorter of the smallest power of two (at least 64) and the smallest
+// 5 x 2^m (at least 80) that hold them (630 columns: 1280, not 2048).  Returns L; logL is log2 of its
+// power-of-two factor, r5 whether it has the factor 5.
+static int filter_length(int W, int* logL, int* r5) {
+    int L = 64, lg = 6;
+    while (L < 2 * W - 1) { L <<= 1; lg++; }
+    int L5 = 80, m = 4;
+    while (L5 < 2 * W - 1) { L5 <<= 1; m++; }
+    *r5 = L5 < L;
+    *logL = *r5 ? m : lg;
+    return *r5 ? L5 : L;
+}
+
 static int get_plan(int W, double pitch_iso, FilterPlan** out, double alpha = kRefAlpha) {
     int dev = 0;
     VCBCT_HIP(hipGetDevice(&dev));
@@ -110,21 +124,7 @@ static int get_plan(int W, double pitch_iso, FilterPlan** out, double alpha = kR
     }
     FilterPlan P;
     P.W = W;
-    // the circular convolution of length L equals the linear one on the W outputs when L >= 2W - 1: the
-    // shortest power of two (at least 64), or 5 x 2^m when that is shorter (630 columns: 1280, not 2048)
-    P.L = 64;
-    P.logL = 6;
-    while (P.L < 2 * W - 1) { P.L <<= 1; P.logL++; }
-    P.r5 = 0;
-    {
-        int L5 = 80, m = 4;
-        while (L5 < 2 * W - 1) { L5 <<= 1; m++; }
-        if (L5 < P.L) {
-            P.L = L5;
-            P.logL = m;
-            P.r5 = 1;
-        }
-    }
+    P.L = filter_length(W, &P.logL, &P.r5);
     if (P.L > 4096) return fail(VCBCT_EINVAL, "filter: projection width %d too large (max 2048)", W);
     std::vector<std::complex<double>> h(P.L, 0.0);
     h[0] = ramp_centre(W, pitch_iso, alpha);
@@ -990,6 +990,12 @@ extern "C" int vcbct_prepare_const_map(int W, double pitch_iso, double* out) {
     for (int m = 0; m < W; ++m)
         for (int n = 0; n < W; ++n) out[(size_t)m * W + n] = (m == n) ? 0.0 : ramp_offcentre(m - n, pitch_iso, W);
     return 0;
+}
+
+extern "C" int vcbct_filter_fft_length(int W) {
+    if (W < 1 || W > 2048) return fail(VCBCT_EINVAL, "filter_fft_length: projection width %d outside 1..2048", W);
+    int lg = 0, r5 = 0;
+    return filter_length(W, &lg, &r5);
 }
 
 extern "C" int vcbct_filter_sinogram(int H, int W, int n, const float* dev_proj, float* dev_out, const double* xshift,

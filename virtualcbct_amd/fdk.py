@@ -26,6 +26,17 @@ def prepareConstMap(projWidth, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752):
     return out
 
 
+def filter_fft_length(projWidth):
+    """The FFT length the filter uses for a panel of projWidth columns (vcbct_filter_fft_length)."""
+    return _nonneg(lib().vcbct_filter_fft_length(int(projWidth)))
+
+
+def _nonneg(rc):
+    if rc < 0:
+        check(rc)
+    return rc
+
+
 def filter_sinogram_device(proj, xshift, yshift, DETECTOR_PITCH_CM_AT_ISO_CENT=0.032509752, out=None, stream=None):
     """Device form of filterSinogram: proj, out are torch float32 [n][H][W] on the device."""
     torch = _torch()
