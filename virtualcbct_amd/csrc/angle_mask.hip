@@ -80,8 +80,11 @@ __global__ __launch_bounds__(256) void k_offset_weight(const float* __restrict__
     const int nr = min(OW_ROWS, H - j0);
     const float* src = p + ((size_t)k * H + j0) * W;
     float* dst = out + ((size_t)k * H + j0) * Wo;
+    // the block's nr x Wo outputs in order, 256 apart; a thread's (row, column) advances by 256 outputs (no
+    // division per output)
+    int row = threadIdx.x / Wo, oc = threadIdx.x - row * Wo;
     for (int i = threadIdx.x; i < nr * Wo; i += 256) {
-        const int row = i / Wo, col = i - row * Wo - off;  // real column (outside [0, W): padding)
+        const int col = oc - off;  // real column (outside [0, W): padding)
         float v = 1.0f;
         if (col >= 0 && col < W) {
             const double e = e_col[col];
@@ -89,6 +92,11 @@ __global__ __launch_bounds__(256) void k_offset_weight(const float* __restrict__
             v = (float)(e == 2.0 ? x * x : (e == 0.0 ? 1.0 : pow(x, e)));
         }
         dst[i] = v;
+        oc += 256;
+        while (oc >= Wo) {
+            oc -= Wo;
+            ++row;
+        }
     }
 }
 
