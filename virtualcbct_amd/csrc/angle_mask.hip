@@ -89,7 +89,8 @@ __global__ __launch_bounds__(256) void k_offset_weight(const float* __restrict__
         if (col >= 0 && col < W) {
             const double e = e_col[col];
             const double x = (double)src[(size_t)row * W + col];
-            v = (float)(e == 2.0 ? x * x : (e == 0.0 ? 1.0 : pow(x, e)));
+            // p^e across the overlap as 2^(e log2 p) in FP64: ~1e-15 relative error, far below the FP32 output's
+            v = (float)(e == 2.0 ? x * x : (e == 0.0 ? 1.0 : exp2(e * log2(x))));
         }
         dst[i] = v;
         oc += 256;
