@@ -46,12 +46,13 @@ def test_filter_w400_matches_fp64_oracle(torch):
     assert err < 2e-5
 
 
-@pytest.mark.parametrize("W,nrows", [(33, 7), (300, 6), (630, 6), (700, 9), (1100, 5)])
+@pytest.mark.parametrize("W,nrows", [(2, 3), (33, 7), (300, 6), (630, 6), (700, 9), (1100, 5), (2048, 3)])
 def test_filter_other_widths_match_fp64_oracle(torch, W, nrows):
     """FFT lengths 80 = 5 x 16 (radix-4 stages and a closing radix-5 stage), 640 = 5 x 128 (a radix-2 stage
     before the radix-5 one), 1280 = 5 x 256 (the half-fan panel's 630 columns), 2048 (a power of two: 5 x 256
-    is too short for 700 columns) and 2560 = 5 x 512 (1100 columns: shorter than 4096), odd row counts (the
-    last row pair has one row) and block tails."""
+    is too short for 700 columns) and 2560 = 5 x 512 (1100 columns: shorter than 4096), the limits (2 columns:
+    64 points; 2048 columns: 4096 points, the 64 KB of LDS a block may hold and every pixel register of a
+    thread), odd row counts (the last row pair has one row) and block tails."""
     from virtualcbct_amd import fdk
     rng = np.random.default_rng(W)
     proj = rng.uniform(0.02, 1.0, (2, nrows, W)).astype(np.float32)
